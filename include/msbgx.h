@@ -1,0 +1,152 @@
+/*
+ * msbgx.h -- C ABI of the MI355X-native MSBG FLIP pressure projection.
+ *
+ * Drop-in boundary for the reference solver entry points used by flip_sim and
+ * msbg_demo.  Every function returns MSBGX_OK (0) or a negative error code;
+ * msbgx_last_error() returns a message for the calling thread's last error.
+ * No torch or HIP types cross this boundary: plain pointers and sizes only.
+ * Device memory is owned by the handle and never exposed.
+ *
+ * Each entry point names the reference interface it replaces (src/ paths are
+ * relative to the reference repository).
+ */
+#ifndef MSBGX_H
+#define MSBGX_H
+
+#include <stdint.h>
+#include "msbgx_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSBGX_OK 0
+#define MSBGX_EINVAL (-1)   /* invalid argument                         */
+#define MSBGX_ENOMEM (-2)   /* device or host allocation failed         */
+#define MSBGX_EHIP (-3)     /* HIP runtime error                        */
+#define MSBGX_ESTATE (-4)   /* call out of order (e.g. solve w/o prepare) */
+
+/* Domain boundary kinds (src/msbg.h:191-192). */
+#define MSBGX_DBC_OPEN 0
+#define MSBGX_DBC_SOLID 1
+
+/* Channel ids for inspection and the per-kernel entry points. */
+#define MSBGX_CH_PRESSURE 0     /* CH_PRESSURE                          */
+#define MSBGX_CH_DIVERGENCE 1   /* CH_DIVERGENCE (right-hand side)      */
+#define MSBGX_CH_R 2            /* CH_FLOAT_2   PCG residual / MG rhs   */
+#define MSBGX_CH_Z 3            /* CH_FLOAT_3   preconditioned residual */
+#define MSBGX_CH_D 4            /* CH_FLOAT_4   search direction        */
+#define MSBGX_CH_Q 5            /* CH_FLOAT_6   A*d / MG residual       */
+#define MSBGX_CH_FACE_U 7       /* CH_FACE_AREA x                       */
+#define MSBGX_CH_FACE_V 8
+#define MSBGX_CH_FACE_W 9
+#define MSBGX_CH_DIAGONAL 10    /* CH_DIAGONAL (inverse diagonal)       */
+#define MSBGX_CH_FLAGS 11       /* CH_CELL_FLAGS (uint16 per cell)      */
+
+typedef struct msbgx_grid msbgx_grid;
+
+/* Reference: MSBG::FlipPressureCallbacks (src/msbg.h:829-835). */
+typedef struct msbgx_callbacks {
+  uint16_t (*sample_cell_type)(void *user, int x, int y, int z);
+  float (*sample_face_coeff)(void *user, int dir, int x, int y, int z);
+  float (*sample_rhs)(void *user, int x, int y, int z);
+  void *user;
+} msbgx_callbacks;
+
+/* Reference: MSBG::FlipPressureSolveParams (src/msbg.h:837-844). */
+typedef struct msbgx_solve_params {
+  int n_pre;        /* 3    */
+  int n_post;       /* 3    */
+  int n_coarse;     /* 24   */
+  int max_iter;     /* 500  */
+  float tol;        /* 5e-4 */
+} msbgx_solve_params;
+
+/* Solver option bits for msbgx_set_options(). */
+#define MSBGX_OPT_LITERAL_VCYCLE 0x1u  /* run residual/restriction/prolongation even when every
+                                          coarser level has no relaxation block             */
+#define MSBGX_OPT_NO_GRAPH 0x2u        /* launch PCG iterations eagerly instead of replaying a
+                                          captured hipGraph                                  */
+
+const char *msbgx_last_error(void);
+void msbgx_default_solve_params(msbgx_solve_params *p);
+
+/* Reference: MultiresSparseGrid::create (src/msbg.h:859-875, src/msbg.cpp:117-1106),
+ * with blockSize0 given and nResLevels in [1,5].  Grid dims must be multiples
+ * of bsx0; bsx0 in {4,8,16,32} and bsx0 >> (n_levels-1) >= 2. */
+int msbgx_create(msbgx_grid **out, int sx0, int sy0, int sz0, int bsx0, int n_levels);
+/* Reference: MultiresSparseGrid::destroy (src/msbg.h:877). */
+void msbgx_destroy(msbgx_grid *g);
+int msbgx_num_blocks(const msbgx_grid *g);
+int msbgx_set_options(msbgx_grid *g, unsigned options);
+
+/* Reference: setDomainBoundarySpec_ (src/msbg.h:935-940, src/msbg.cpp:2462). */
+int msbgx_set_domain_bc(msbgx_grid *g, int left, int right, int bottom, int top, int front, int back);
+/* Reference: regularizeRefinementMap (src/msbg.h:922-926, src/msbg.cpp:1202-1420).
+ * `out` may equal `in`. */
+int msbgx_regularize_refinement_map(const msbgx_grid *g, const int *in, int *out);
+/* Reference: setRefinementMap (src/msbg.h:928-934, src/msbg.cpp:1428-1736),
+ * doResetChannels = true. */
+int msbgx_set_refinement_map(msbgx_grid *g, const int *levels);
+
+/* Reference: preparePressureSolveFLIP (src/msbg.h:849-852, src/msbg3.cpp:3041-3095).
+ * Callbacks are invoked serially on the calling thread with level-local
+ * coordinates, as in the reference (src/msbg3.cpp:2494-2576). */
+int msbgx_prepare_pressure_solve(msbgx_grid *g, const msbgx_callbacks *cb);
+/* Bulk variant of the above: callbacks that read fine-grid arrays of
+ * sx0*sy0*sz0 values (x fastest).  rhs may be NULL (all zero). */
+int msbgx_prepare_pressure_solve_dense(msbgx_grid *g, const uint16_t *flags, const float *face_u,
+                                       const float *face_v, const float *face_w, const float *rhs);
+/* Bulk variant over an analytic scene: cell types and face coefficients are
+ * sampled on the host, the divergence right-hand side is computed on the GPU
+ * from the scene's MAC velocity (flipRhsCB, src/flip_sim.cpp:329-338). */
+int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene);
+
+/* Reference: solvePressureFLIP (src/msbg.h:854-857, src/msbg3.cpp:3098-3271).
+ * Returns the iteration count in *iters (max_iter when not converged) and
+ * |r|/|b| in *rel_residual. */
+int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters, float *rel_residual);
+/* Relative residual after each PCG iteration of the last solve (up to cap). */
+int msbgx_residual_history(const msbgx_grid *g, double *out, int cap);
+/* Device time of the last solve in milliseconds (HIP events around the PCG). */
+double msbgx_last_solve_ms(const msbgx_grid *g);
+
+/* Inspection.  Block `bid` of channel `ch` at MG level `mg`, at the block's
+ * level there: edge^3 floats (uint16 for MSBGX_CH_FLAGS).  Reference access
+ * path: getFloatChannel(ch, level, mg)->getBlockDataPtr(bid) (src/msbg.h:2999-3140).
+ * Returns the block edge (>0), 0 when the channel has no data there. */
+int msbgx_get_channel_block(const msbgx_grid *g, int ch, int mg, int bid, void *dst);
+int msbgx_set_channel_block(msbgx_grid *g, int ch, int mg, int bid, const float *src);
+/* Per-block level and BLK_* flags at MG level mg (src/msbg.h:558-576, 3560-3590). */
+int msbgx_block_info(const msbgx_grid *g, int mg, int *level, uint32_t *flags);
+/* Relaxation block list at MG level mg (src/msbg3.cpp:2868-2879); returns its length. */
+int msbgx_relax_list(const msbgx_grid *g, int mg, int *out, int cap);
+float msbgx_omega(const msbgx_grid *g);
+
+/* Per-kernel entry points (each is one reference operation at MG level mg):
+ *   relax        MultiresSparseGrid::relax           src/msbg3.cpp:2245-2369
+ *   multiply     multiplyLaplacianMatrixOpt          src/msbg3.cpp:1757-1912
+ *   downsample   downsampleChannel<float,float>      src/msbg4.cpp:1513-1915
+ *   prolongate   prolongateChannel<float>            src/msbg4.cpp:1921-2029
+ *   zero         zeroChannelInPlace                  src/msbg3.cpp:2375-2399
+ *   vcycle       z=0; vCycle(0, z, r, q)             src/msbg3.cpp:2412-2465
+ * Channels follow the MSBGX_CH_* ids; at MG levels >= 1 only R, Z and Q exist. */
+int msbgx_relax(msbgx_grid *g, int mg, int reverse, int n_iter, int ch_x, int ch_b);
+int msbgx_multiply(msbgx_grid *g, int mg, int residual, int ch_x, int ch_b, int ch_dst);
+int msbgx_downsample(msbgx_grid *g, int mg_coarse, int ch_src, int ch_dst);
+int msbgx_prolongate(msbgx_grid *g, int mg_fine, int ch);
+int msbgx_zero_channel(msbgx_grid *g, int ch, int mg);
+int msbgx_copy_channel(msbgx_grid *g, int ch_src, int ch_dst);
+int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse);
+
+/* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
+ * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
+ * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);
+ * level 0 for d <= d0, 1 for d == d0+1, else n_levels-1.  Not regularised. */
+int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSBGX_H */

@@ -1,0 +1,330 @@
+/*
+ * extern "C" entry points of libmsbgx (declared in include/msbgx.h).
+ * Argument validation, error strings and the three prepare input paths
+ * (callbacks, dense arrays, analytic scene) live here; the device work is in
+ * solver.hip.
+ */
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "msbgx_internal.h"
+
+namespace msbgx {
+
+static thread_local std::string g_lastError;
+void setError(const std::string &msg) { g_lastError = msg; }
+
+namespace {
+
+struct CallbackSampler : Sampler {
+  msbgx_callbacks cb;
+  uint16_t cellType(int x, int y, int z) override { return cb.sample_cell_type(cb.user, x, y, z); }
+  float faceCoeff(int d, int x, int y, int z) override { return cb.sample_face_coeff(cb.user, d, x, y, z); }
+  float rhs(int x, int y, int z) override { return cb.sample_rhs(cb.user, x, y, z); }
+};
+
+struct DenseSampler : Sampler {
+  int nx, ny;
+  const uint16_t *flags;
+  const float *w[3], *r;
+  size_t idx(int x, int y, int z) const { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z); }
+  uint16_t cellType(int x, int y, int z) override { return flags[idx(x, y, z)]; }
+  float faceCoeff(int d, int x, int y, int z) override { return w[d][idx(x, y, z)]; }
+  float rhs(int x, int y, int z) override { return r ? r[idx(x, y, z)] : 0.0f; }
+  bool parallelSafe() const override { return true; }
+};
+
+/* Scene sampling on the host for cell types and face coefficients; the
+ * divergence right-hand side is evaluated on the device. */
+struct SceneSampler : Sampler {
+  const msbgx_scene *s;
+  uint16_t cellType(int x, int y, int z) override { return msbgx_scene_cell_type(s, x, y, z); }
+  float faceCoeff(int d, int x, int y, int z) override { return msbgx_scene_face_coeff(s, d, x, y, z); }
+  float rhs(int, int, int) override { return 0.0f; }
+  bool parallelSafe() const override { return true; }
+  bool rhsOnDevice() const override { return true; }
+};
+
+int checkGrid(const msbgx_grid *g)
+{
+  if (!g) {
+    setError("null grid handle");
+    return MSBGX_EINVAL;
+  }
+  return MSBGX_OK;
+}
+
+int prepareWith(Grid &g, Sampler &s, const msbgx_scene *scene)
+{
+  std::vector<float> rhs;
+  int rc = buildMetadata(g, s, rhs);
+  if (rc) return rc;
+  rc = deviceUpload(g, rhs, scene);
+  if (rc) return rc;
+  g.prepared = true;
+  return MSBGX_OK;
+}
+
+}  // namespace
+}  // namespace msbgx
+
+using namespace msbgx;
+
+struct msbgx_grid : Grid {};
+
+extern "C" {
+
+const char *msbgx_last_error(void) { return g_lastError.c_str(); }
+
+void msbgx_default_solve_params(msbgx_solve_params *p)
+{
+  if (!p) return;
+  p->n_pre = 3;
+  p->n_post = 3;
+  p->n_coarse = 24;
+  p->max_iter = 500;
+  p->tol = 5e-4f;
+}
+
+int msbgx_create(msbgx_grid **out, int sx0, int sy0, int sz0, int bsx0, int n_levels)
+{
+  if (!out) { setError("out is null"); return MSBGX_EINVAL; }
+  *out = nullptr;
+  if (bsx0 < 4 || bsx0 > 32 || (bsx0 & (bsx0 - 1))) { setError("bsx0 must be 4, 8, 16 or 32"); return MSBGX_EINVAL; }
+  if (n_levels < 1 || n_levels > kMaxLevels || (bsx0 >> (n_levels - 1)) < 2) {
+    setError("n_levels out of range for the block size");
+    return MSBGX_EINVAL;
+  }
+  if (sx0 <= 0 || sy0 <= 0 || sz0 <= 0 || sx0 % bsx0 || sy0 % bsx0 || sz0 % bsx0) {
+    setError("grid dimensions must be positive multiples of the block size");
+    return MSBGX_EINVAL;
+  }
+  msbgx_grid *g = new (std::nothrow) msbgx_grid();
+  if (!g) { setError("out of host memory"); return MSBGX_ENOMEM; }
+  g->sx0 = sx0; g->sy0 = sy0; g->sz0 = sz0; g->bsx0 = bsx0; g->nLevels = n_levels;
+  g->nbx = sx0 / bsx0; g->nby = sy0 / bsx0; g->nbz = sz0 / bsx0;
+  g->nBlocks = g->nbx * g->nby * g->nbz;
+  for (int i = 0; i < 3; i++) g->domainBc[i][0] = g->domainBc[i][1] = MSBGX_DBC_OPEN;
+  std::vector<int> zero(g->nBlocks, 0);
+  setRefinementMap(*g, zero.data());
+  *out = g;
+  return MSBGX_OK;
+}
+
+void msbgx_destroy(msbgx_grid *g)
+{
+  if (!g) return;
+  deviceFree(*g);
+  delete g;
+}
+
+int msbgx_num_blocks(const msbgx_grid *g) { return g ? g->nBlocks : 0; }
+
+int msbgx_set_options(msbgx_grid *g, unsigned options)
+{
+  if (int rc = checkGrid(g)) return rc;
+  g->options = options;
+  return MSBGX_OK;
+}
+
+int msbgx_set_domain_bc(msbgx_grid *g, int l, int r, int b, int t, int f, int k)
+{
+  if (int rc = checkGrid(g)) return rc;
+  const int v[6] = {l, r, b, t, f, k};
+  for (int i = 0; i < 6; i++)
+    if (v[i] != MSBGX_DBC_OPEN && v[i] != MSBGX_DBC_SOLID) { setError("unknown boundary kind"); return MSBGX_EINVAL; }
+  g->domainBc[0][0] = l; g->domainBc[0][1] = r;
+  g->domainBc[1][0] = b; g->domainBc[1][1] = t;
+  g->domainBc[2][0] = f; g->domainBc[2][1] = k;
+  g->prepared = false;
+  return MSBGX_OK;
+}
+
+int msbgx_regularize_refinement_map(const msbgx_grid *g, const int *in, int *out)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!in || !out) { setError("null map"); return MSBGX_EINVAL; }
+  if (g->nLevels < 2) { std::copy(in, in + g->nBlocks, out); return MSBGX_OK; }
+  regularize(*g, in, out);
+  return MSBGX_OK;
+}
+
+int msbgx_set_refinement_map(msbgx_grid *g, const int *levels)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!levels) { setError("null map"); return MSBGX_EINVAL; }
+  for (int i = 0; i < g->nBlocks; i++)
+    if (levels[i] < 0 || levels[i] >= g->nLevels) { setError("block level out of range"); return MSBGX_EINVAL; }
+  /* 2:1 grading over the 26-neighbourhood is required (src/msbg.cpp:1468-1494) */
+  for (int bid = 0; bid < g->nBlocks; bid++) {
+    int bx, by, bz;
+    g->blockCoords(bid, bx, by, bz);
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if (!g->inRaster(bx + dx, by + dy, bz + dz)) continue;
+          const int l2 = levels[g->blockIndex(bx + dx, by + dy, bz + dz)];
+          if (std::abs(l2 - levels[bid]) > 1) { setError("refinement map is not 2:1 graded"); return MSBGX_EINVAL; }
+        }
+  }
+  setRefinementMap(*g, levels);
+  return MSBGX_OK;
+}
+
+int msbgx_prepare_pressure_solve(msbgx_grid *g, const msbgx_callbacks *cb)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!cb || !cb->sample_cell_type || !cb->sample_face_coeff || !cb->sample_rhs) { setError("incomplete callbacks"); return MSBGX_EINVAL; }
+  CallbackSampler s;
+  s.cb = *cb;
+  return prepareWith(*g, s, nullptr);
+}
+
+int msbgx_prepare_pressure_solve_dense(msbgx_grid *g, const uint16_t *flags, const float *fu, const float *fv,
+                                       const float *fw, const float *rhs)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!flags || !fu || !fv || !fw) { setError("null input array"); return MSBGX_EINVAL; }
+  DenseSampler s;
+  s.nx = g->sx0; s.ny = g->sy0; s.flags = flags; s.w[0] = fu; s.w[1] = fv; s.w[2] = fw; s.r = rhs;
+  return prepareWith(*g, s, nullptr);
+}
+
+int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!scene || scene->nx != g->sx0 || scene->ny != g->sy0 || scene->nz != g->sz0) {
+    setError("scene size must equal the grid size");
+    return MSBGX_EINVAL;
+  }
+  if (scene->n_spheres < 0 || scene->n_spheres > MSBGX_SCENE_MAX_SPHERES || scene->n_boxes < 0 ||
+      scene->n_boxes > MSBGX_SCENE_MAX_BOXES || !(scene->dt != 0.0f)) {
+    setError("invalid scene");
+    return MSBGX_EINVAL;
+  }
+  SceneSampler s;
+  s.s = scene;
+  return prepareWith(*g, s, scene);
+}
+
+int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters, float *rel)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!p || !iters || !rel) { setError("null argument"); return MSBGX_EINVAL; }
+  if (!g->prepared) { setError("solve before prepare"); return MSBGX_ESTATE; }
+  if (p->n_pre < 0 || p->n_post < 0 || p->n_coarse < 0 || p->max_iter < 0 || !(p->tol >= 0.0f)) {
+    setError("invalid solve parameters");
+    return MSBGX_EINVAL;
+  }
+  return deviceSolve(*g, *p, iters, rel);
+}
+
+int msbgx_residual_history(const msbgx_grid *g, double *out, int cap)
+{
+  if (!g || !out) return 0;
+  const int n = std::min<int>(cap, (int)g->history.size());
+  for (int i = 0; i < n; i++) out[i] = g->history[i];
+  return (int)g->history.size();
+}
+
+double msbgx_last_solve_ms(const msbgx_grid *g) { return g ? g->lastSolveMs : 0.0; }
+
+int msbgx_get_channel_block(const msbgx_grid *g, int ch, int mg, int bid, void *dst)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (mg < 0 || mg >= g->nLevels || bid < 0 || bid >= g->nBlocks || !dst) { setError("bad block address"); return MSBGX_EINVAL; }
+  if (!g->prepared) { setError("not prepared"); return MSBGX_ESTATE; }
+  return deviceGetBlock(*g, ch, mg, bid, dst);
+}
+
+int msbgx_set_channel_block(msbgx_grid *g, int ch, int mg, int bid, const float *src)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (mg < 0 || mg >= g->nLevels || bid < 0 || bid >= g->nBlocks || !src) { setError("bad block address"); return MSBGX_EINVAL; }
+  if (!g->prepared) { setError("not prepared"); return MSBGX_ESTATE; }
+  return deviceSetBlock(*g, ch, mg, bid, src);
+}
+
+int msbgx_block_info(const msbgx_grid *g, int mg, int *level, uint32_t *flags)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (mg < 0 || mg >= g->nLevels || !level || !flags) { setError("bad argument"); return MSBGX_EINVAL; }
+  for (int b = 0; b < g->nBlocks; b++) {
+    level[b] = g->mg[mg].level[b];
+    flags[b] = g->mg[mg].bflags[b];
+  }
+  return MSBGX_OK;
+}
+
+int msbgx_relax_list(const msbgx_grid *g, int mg, int *out, int cap)
+{
+  if (!g || mg < 0 || mg >= g->nLevels) return 0;
+  const std::vector<int> &r = g->mg[mg].relax;
+  for (int i = 0; i < (int)r.size() && i < cap; i++) out[i] = r[i];
+  return (int)r.size();
+}
+
+float msbgx_omega(const msbgx_grid *g) { return g ? g->omega : 0.0f; }
+
+#define MSBGX_PREPARED_OR_FAIL(g)                 \
+  do {                                             \
+    if (int rc_ = checkGrid(g)) return rc_;        \
+    if (!(g)->prepared) {                          \
+      setError("not prepared");                    \
+      return MSBGX_ESTATE;                         \
+    }                                              \
+  } while (0)
+
+int msbgx_relax(msbgx_grid *g, int mg, int reverse, int n_iter, int ch_x, int ch_b)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (mg < 0 || mg >= g->nLevels || n_iter < 0) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceRelax(*g, mg, reverse != 0, n_iter, ch_x, ch_b);
+}
+int msbgx_multiply(msbgx_grid *g, int mg, int residual, int ch_x, int ch_b, int ch_dst)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (mg < 0 || mg >= g->nLevels) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceMultiply(*g, mg, residual != 0, ch_x, ch_b, ch_dst);
+}
+int msbgx_downsample(msbgx_grid *g, int mg_coarse, int ch_src, int ch_dst)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (mg_coarse < 1 || mg_coarse >= g->nLevels) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceDownsample(*g, mg_coarse, ch_src, ch_dst);
+}
+int msbgx_prolongate(msbgx_grid *g, int mg_fine, int ch)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (mg_fine < 0 || mg_fine + 1 >= g->nLevels) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceProlongate(*g, mg_fine, ch);
+}
+int msbgx_zero_channel(msbgx_grid *g, int ch, int mg)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (mg < 0 || mg >= g->nLevels) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceZero(*g, ch, mg);
+}
+int msbgx_copy_channel(msbgx_grid *g, int ch_src, int ch_dst)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  return deviceCopy(*g, ch_src, ch_dst);
+}
+int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  return deviceVCycle(*g, n_pre, n_post, n_coarse);
+}
+
+int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)
+{
+  const int rc = levelMapFromScene(s, bsx0, n_levels, mode, d0, out);
+  if (rc) setError("invalid scene or block size");
+  return rc;
+}
+
+}  // extern "C"

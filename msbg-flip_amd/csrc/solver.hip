@@ -1,0 +1,1028 @@
+/*
+ * solver.hip -- CDNA4 (gfx950) kernels and device orchestration of the MSBG
+ * FLIP pressure projection: block-halo staging, the in-place block
+ * Gauss-Seidel smoother, the stencil apply / residual, restriction,
+ * prolongation, the V-cycle and the MG-preconditioned CG.
+ *
+ * Kernel map (one workgroup per sparse block unless noted):
+ *   k_relax<B>      red-black block relaxation pass; the block and its
+ *                   one-cell halo live in LDS; 5 local sweeps, the first four
+ *                   as hyperplane wavefronts, the last a Jacobi pass
+ *                   (src/msbg3.cpp:419-1506, 2171-2240)
+ *   k_multiply<B>   y = A x or r = b - A x, optional FP64 partial of <x, y>
+ *                   (src/msbg3.cpp:1282-1321, 1757-1912)
+ *   k_restrict      2x2x2 mean / block copy / zero (src/msbg4.cpp:1513-1915)
+ *   k_prolong       piecewise-constant copy-add (src/msbg4.cpp:1921-2029)
+ *   k_update_xr, k_dot, k_axpy, k_finalize_*: PCG vector work with FP64
+ *                   accumulation (src/msbg3.cpp:3098-3259)
+ *
+ * Arithmetic follows the oracle's operation sequence exactly (explicit fmaf,
+ * FP32 storage, FP64 accumulation), and the file is compiled with
+ * -ffp-contract=off, so each kernel reproduces the CPU restatement bit for
+ * bit; only the FP64 reduction order differs.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "msbgx_internal.h"
+
+namespace msbgx {
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      setError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);   \
+      return MSBGX_EHIP;                                                               \
+    }                                                                                  \
+  } while (0)
+
+/* ------------------------------------------------------------------ device */
+
+struct LevelArgs {
+  const int *level;
+  const int64_t *off;
+  const uint16_t *flags;
+  const float *W0, *W1, *W2, *D;
+  int nbx, nby, nbz, bsx0;
+  float bcW[3];  // face coefficient at the +side domain boundary per direction
+};
+
+struct PcgState {
+  double bnorm2, rho, rhoNew, dq, rr, tolSq;
+  float alpha, beta;
+  int done, conv, iter, brk, maxIter, pad;
+};
+
+template <int B> struct Cfg {
+  static constexpr int S = B >= 8 ? 8 : B;     // SIMD segment width (src/msbg3.cpp:1700-1743)
+  static constexpr int NSEG = B / S;
+  static constexpr int T = NSEG + 2 * (B - 1);  // hyperplanes seg + y + z per sweep
+  static constexpr int NT = B * B;
+  static constexpr int NIN = B * B * B;
+  static constexpr int NF = B * B;
+};
+
+enum FaceKind { FK_SAME = 0, FK_DOMAIN = 1, FK_COARSER = 2, FK_FINER = 3 };
+
+/* Stage the block (h*x, zero on all-non-fluid segments) and its six halo
+ * faces into LDS (src/msbg3.cpp:564-1044).  Returns per-thread skip mask over
+ * z and the face kinds/neighbour offsets used for the +face coefficients. */
+template <int B>
+__device__ __forceinline__ void stageTile(float *tile, uint32_t &skipmask, const LevelArgs &a, int bid, int lev,
+                                          float h, int64_t off, const float *__restrict__ X, int *fkind,
+                                          int64_t *foff)
+{
+  constexpr int S = Cfg<B>::S, NIN = Cfg<B>::NIN, NF = Cfg<B>::NF;
+  const int tx = threadIdx.x % B, ty = threadIdx.x / B;
+  const int lane = __lane_id();
+  uint32_t sm = 0;
+#pragma unroll 4
+  for (int z = 0; z < B; z++) {
+    const int v = tx + B * ty + B * B * z;
+    const bool nonFluid = (a.flags[off + v] & (CELL_SOLID | CELL_VOID)) != 0;
+    const uint64_t bal = __ballot(nonFluid);
+    const uint64_t segBits = (bal >> (lane & ~(S - 1))) & ((1ull << S) - 1);
+    const bool skip = segBits == ((1ull << S) - 1);
+    sm |= (skip ? 1u : 0u) << z;
+    tile[v] = skip ? 0.0f : X[off + v] * h;
+  }
+  skipmask = sm;
+
+  int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  const float a13 = (float)(1.0 * (1. / 3.));      // DPMR_GRAD_COEFF_CF (src/msbg.h:85-87)
+  const float a23 = (float)(1.0 * 2 * (1. / 3.));  // 2 * DPMR_GRAD_COEFF_CF
+  const float *Wd[3] = {a.W0, a.W1, a.W2};
+  const int u = tx, w = ty;
+#pragma unroll
+  for (int face = 0; face < 6; face++) {
+    const int dir = face >> 1, side = face & 1;
+    int nbxx = bx, nbyy = by, nbzz = bz;
+    if (dir == 0) nbxx += side ? 1 : -1;
+    else if (dir == 1) nbyy += side ? 1 : -1;
+    else nbzz += side ? 1 : -1;
+    float val = 0.0f;
+    int kind;
+    int64_t off2 = 0;
+    if (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz) {
+      kind = FK_DOMAIN;
+    } else {
+      const int nb = nbxx + a.nbx * (nbyy + a.nby * nbzz);
+      const int lev2 = a.level[nb];
+      off2 = a.off[nb];
+      if (lev2 == lev) {
+        kind = FK_SAME;
+        const int e = side ? 0 : B - 1;
+        const int src = dir == 0 ? e + B * u + B * B * w : (dir == 1 ? u + B * e + B * B * w : u + B * w + B * B * e);
+        val = X[off2 + src] * h;
+      } else if (lev2 > lev) {
+        /* fine block, coarser neighbour: one coarse cell, own -face coefficient */
+        kind = FK_COARSER;
+        constexpr int Bc = B / 2 > 0 ? B / 2 : 1;
+        const int e = side ? 0 : Bc - 1;
+        const int cu = u / 2, cw = w / 2;
+        const int src = dir == 0 ? e + Bc * cu + Bc * Bc * cw
+                                 : (dir == 1 ? cu + Bc * e + Bc * Bc * cw : cu + Bc * cw + Bc * Bc * e);
+        float aa = a23;
+        if (!side) {
+          const int own = dir == 0 ? B * u + B * B * w : (dir == 1 ? u + B * B * w : u + B * w);
+          aa *= Wd[dir][off + own];
+        }
+        const double sv = (double)(aa * X[off2 + src]);
+        val = (float)(sv * (double)h);
+      } else {
+        /* coarse block, finer neighbour: four fine cells on the shared face */
+        kind = FK_FINER;
+        constexpr int Bf = 2 * B;
+        const int e = side ? 0 : Bf - 1;
+        double sv = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int d1 = c & 1, d2 = c >> 1;
+          const int fu = 2 * u + d1, fw = 2 * w + d2;
+          const int src = dir == 0 ? e + Bf * fu + Bf * Bf * fw
+                                   : (dir == 1 ? fu + Bf * e + Bf * Bf * fw : fu + Bf * fw + Bf * Bf * e);
+          float aa = a13;
+          if (side) aa *= Wd[dir][off2 + src];
+          sv += (double)(aa * X[off2 + src]);
+        }
+        val = (float)(sv * (double)h);
+      }
+    }
+    tile[NIN + face * NF + u + B * w] = val;
+    fkind[face] = kind;
+    foff[face] = off2;
+  }
+}
+
+/* One cell of the block operator: F0..F6 from the LDS tile, coefficients from
+ * global memory with the staged face overrides (src/msbg3.cpp:1192-1335). */
+template <int B>
+__device__ __forceinline__ void cellStencil(const float *tile, const LevelArgs &a, int64_t off, const int *fkind,
+                                            const int64_t *foff, int x, int y, int z, float &F0, float &S, float &D)
+{
+  constexpr int NIN = Cfg<B>::NIN, NF = Cfg<B>::NF;
+  const int v = x + B * y + B * B * z;
+  F0 = tile[v];
+  const float F1 = x > 0 ? tile[v - 1] : tile[NIN + 0 * NF + y + B * z];
+  const float F2 = x < B - 1 ? tile[v + 1] : tile[NIN + 1 * NF + y + B * z];
+  const float F3 = y > 0 ? tile[v - B] : tile[NIN + 2 * NF + x + B * z];
+  const float F4 = y < B - 1 ? tile[v + B] : tile[NIN + 3 * NF + x + B * z];
+  const float F5 = z > 0 ? tile[v - B * B] : tile[NIN + 4 * NF + x + B * y];
+  const float F6 = z < B - 1 ? tile[v + B * B] : tile[NIN + 5 * NF + x + B * y];
+  const int64_t c = off + v;
+  float W1 = a.W0[c], W3 = a.W1[c], W5 = a.W2[c];
+  if (x == 0 && fkind[0] >= FK_COARSER) W1 = 1.0f;
+  if (y == 0 && fkind[2] >= FK_COARSER) W3 = 1.0f;
+  if (z == 0 && fkind[4] >= FK_COARSER) W5 = 1.0f;
+  float W2, W4, W6;
+  if (x < B - 1) W2 = a.W0[c + 1];
+  else W2 = fkind[1] == FK_SAME ? a.W0[foff[1] + B * y + B * B * z] : (fkind[1] == FK_DOMAIN ? a.bcW[0] : 1.0f);
+  if (y < B - 1) W4 = a.W1[c + B];
+  else W4 = fkind[3] == FK_SAME ? a.W1[foff[3] + x + B * B * z] : (fkind[3] == FK_DOMAIN ? a.bcW[1] : 1.0f);
+  if (z < B - 1) W6 = a.W2[c + B * B];
+  else W6 = fkind[5] == FK_SAME ? a.W2[foff[5] + x + B * y] : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  float s = F1 * W1;
+  s = fmaf(F2, W2, s);
+  s = fmaf(F3, W3, s);
+  s = fmaf(F4, W4, s);
+  s = fmaf(F5, W5, s);
+  s = fmaf(F6, W6, s);
+  S = s;
+  D = a.D[c];
+}
+
+/* Red-black block relaxation, one colour pass, one workgroup per block. */
+template <int B>
+__global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__restrict__ blocks,
+                                                      float *__restrict__ X, const float *__restrict__ Bv,
+                                                      float omega, int reverse, int nIter, const int *done)
+{
+  if (*done) return;
+  constexpr int S = Cfg<B>::S, T = Cfg<B>::T;
+  __shared__ float tile[Cfg<B>::NIN + 6 * Cfg<B>::NF];
+  const int bid = blocks[blockIdx.x];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const float h = (float)(1 << lev);
+  uint32_t skipmask;
+  int fkind[6];
+  int64_t foff[6];
+  stageTile<B>(tile, skipmask, a, bid, lev, h, off, X, fkind, foff);
+  __syncthreads();
+
+  const int x = threadIdx.x % B, y = threadIdx.x / B;
+  const int o = x / S + y;
+  const float om = omega, omm = 1.0f - omega;
+  for (int k = 0; k < nIter - 1; k++) {
+    const bool rev = (reverse != 0) == ((k & 1) != 0);
+    for (int t = 0; t < T; t++) {
+      const int hp = rev ? T - 1 - t : t;
+      const int z = hp - o;
+      if (z >= 0 && z < B && !((skipmask >> z) & 1u)) {
+        float F0, Sm, D;
+        cellStencil<B>(tile, a, off, fkind, foff, x, y, z, F0, Sm, D);
+        const float bb = Bv[off + x + B * y + B * B * z];
+        const float Y = D != 0.0f ? fmaf(om * (bb + Sm), D, omm * F0) : F0;
+        tile[x + B * y + B * B * z] = Y;
+      }
+      __syncthreads();
+    }
+  }
+  /* last local sweep: every neighbour read from the staged state, result /h
+   * written to the channel (src/msbg3.cpp:1341-1349, 1436-1438) */
+  const float HR = 1.0f / h;
+  for (int z = 0; z < B; z++) {
+    if ((skipmask >> z) & 1u) continue;
+    float F0, Sm, D;
+    cellStencil<B>(tile, a, off, fkind, foff, x, y, z, F0, Sm, D);
+    const int64_t c = off + x + B * y + B * B * z;
+    const float Y = D != 0.0f ? fmaf(om * (Bv[c] + Sm), D, omm * F0) : F0;
+    X[c] = Y * HR;
+  }
+}
+
+__device__ __forceinline__ double blockReduceSum(double v, double *scratch)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int nw = (blockDim.x + 63) / 64;
+  __syncthreads();
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < nw; i++) r += scratch[i];
+  return r;
+}
+
+/* y = A x (or r = b - A x); non-fluid segments are not written. */
+template <int B>
+__global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int *__restrict__ blocks,
+                                                         const float *__restrict__ X, const float *__restrict__ Bv,
+                                                         float *__restrict__ Y, int residual, double *partials,
+                                                         int partialBase, const int *done)
+{
+  if (*done) return;
+  __shared__ float tile[Cfg<B>::NIN + 6 * Cfg<B>::NF];
+  __shared__ double red[16];
+  const int bid = blocks[blockIdx.x];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const float h = (float)(1 << lev);
+  uint32_t skipmask;
+  int fkind[6];
+  int64_t foff[6];
+  stageTile<B>(tile, skipmask, a, bid, lev, h, off, X, fkind, foff);
+  __syncthreads();
+  const int x = threadIdx.x % B, y = threadIdx.x / B;
+  double acc = 0.0;
+  for (int z = 0; z < B; z++) {
+    if ((skipmask >> z) & 1u) continue;
+    float F0, Sm, D;
+    cellStencil<B>(tile, a, off, fkind, foff, x, y, z, F0, Sm, D);
+    const int64_t c = off + x + B * y + B * B * z;
+    float r = D != 0.0f ? fmaf(1.0f / D, F0, -Sm) : 0.0f;
+    if (residual) r = Bv[c] - r;
+    Y[c] = r;
+    acc += (double)F0 * (double)r;
+  }
+  if (partials) {
+    const double s = blockReduceSum(lev == 0 ? acc : 0.0, red);
+    if (threadIdx.x == 0) partials[partialBase + blockIdx.x] = s;
+  }
+}
+
+/* Restriction into the coarse right-hand side of MG level Lc from the fine
+ * residual of level Lc-1: 2x2x2 mean for coarsened blocks; for blocks with
+ * the same level at both MG levels, a copy (first V-cycle on the channel
+ * set) or zero (later V-cycles), which is what the reference's block
+ * reference plus in-place zeroing produces. */
+__global__ void k_restrict(const int *__restrict__ blocks, const int *__restrict__ level0, const int *__restrict__ levC,
+                           const int64_t *__restrict__ offC, const int64_t *__restrict__ offF, int Lc, int bsx0,
+                           const float *__restrict__ src, float *__restrict__ dst, int copyMode, const int *done)
+{
+  if (*done) return;
+  const int bid = blocks[blockIdx.x];
+  const int lev = levC[bid];
+  const int b = bsx0 >> lev, n = b * b * b;
+  const int64_t oc = offC[bid], of = offF[bid];
+  if (level0[bid] >= Lc) {
+    for (int v = threadIdx.x; v < n; v += blockDim.x) dst[oc + v] = copyMode ? src[of + v] : 0.0f;
+    return;
+  }
+  const int bf = 2 * b;
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    const int vx = v % b, vy = (v / b) % b, vz = v / (b * b);
+    float val = 0.0f;
+    for (int k = 0; k < 2; k++)
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+          const float f = src[of + (2 * vx + i) + bf * ((2 * vy + j) + bf * (2 * vz + k))];
+          val = (float)((double)val + (double)f * 0.125);
+        }
+    dst[oc + v] = val;
+  }
+}
+
+/* x_L += P x_{L+1} on non-SOLID/VOID cells. */
+__global__ void k_prolong(const int *__restrict__ blocks, const int *__restrict__ level0, const int *__restrict__ levF,
+                          const int64_t *__restrict__ offF, const int64_t *__restrict__ offC,
+                          const uint16_t *__restrict__ flagsF, int Lf, int bsx0, float *__restrict__ xf,
+                          const float *__restrict__ xc, const int *done)
+{
+  if (*done) return;
+  const int bid = blocks[blockIdx.x];
+  const int lev = levF[bid];
+  const int b = bsx0 >> lev, n = b * b * b;
+  const int64_t of = offF[bid], oc = offC[bid];
+  const bool same = level0[bid] >= Lf + 1;
+  const int bc = b / 2;
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    if (flagsF[of + v] & (CELL_SOLID | CELL_VOID)) continue;
+    float e;
+    if (same) e = xc[oc + v];
+    else {
+      const int vx = v % b, vy = (v / b) % b, vz = v / (b * b);
+      e = xc[oc + (vx >> 1) + bc * ((vy >> 1) + bc * (vz >> 1))];
+    }
+    xf[of + v] = xf[of + v] + e;
+  }
+}
+
+__global__ void k_zero_blocks(const int *__restrict__ blocks, const int *__restrict__ lev, const int64_t *__restrict__ off,
+                              int bsx0, float *__restrict__ x, const int *done)
+{
+  if (*done) return;
+  const int bid = blocks[blockIdx.x];
+  const int b = bsx0 >> lev[bid], n = b * b * b;
+  const int64_t o = off[bid];
+  for (int v = threadIdx.x; v < n; v += blockDim.x) x[o + v] = 0.0f;
+}
+
+__global__ void k_copy_blocks(const int *__restrict__ blocks, const int *__restrict__ lev, const int64_t *__restrict__ off,
+                              int bsx0, const float *__restrict__ s, float *__restrict__ d, const int *done)
+{
+  if (*done) return;
+  const int bid = blocks[blockIdx.x];
+  const int b = bsx0 >> lev[bid], n = b * b * b;
+  const int64_t o = off[bid];
+  for (int v = threadIdx.x; v < n; v += blockDim.x) d[o + v] = s[o + v];
+}
+
+constexpr int kVecBlocks = 1024, kVecThreads = 256;
+
+/* partial <a, b> over [0, n) (FP64 products) */
+__global__ void __launch_bounds__(kVecThreads) k_dot(const float *__restrict__ a, const float *__restrict__ b, int64_t n,
+                                                     double *partials, const int *done)
+{
+  if (*done) return;
+  __shared__ double red[kVecThreads / 64];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+/* r = b over [0, n) and partial |b|^2 */
+__global__ void __launch_bounds__(kVecThreads) k_copy_norm(const float *__restrict__ b, float *__restrict__ r, int64_t n,
+                                                           double *partials)
+{
+  __shared__ double red[kVecThreads / 64];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = b[i];
+    r[i] = v;
+    acc += (double)v * (double)v;
+  }
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+/* x += alpha d; r -= alpha q; partial |r|^2 (src/msbg3.cpp:3175-3195) */
+__global__ void __launch_bounds__(kVecThreads) k_update_xr(float *__restrict__ x, float *__restrict__ r,
+                                                           const float *__restrict__ d, const float *__restrict__ q,
+                                                           int64_t n, const PcgState *st, double *partials)
+{
+  if (st->done) return;
+  __shared__ double red[kVecThreads / 64];
+  const float alpha = st->alpha;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = fmaf(alpha, d[i], x[i]);
+    const float rv = fmaf(-alpha, q[i], r[i]);
+    r[i] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+/* d = z + beta d (src/msbg3.cpp:3224-3235) */
+__global__ void __launch_bounds__(kVecThreads) k_axpy_d(float *__restrict__ d, const float *__restrict__ z, int64_t n,
+                                                        const PcgState *st)
+{
+  if (st->done) return;
+  const float beta = st->beta;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = fmaf(beta, d[i], z[i]);
+}
+
+/* Deterministic fixed-order sum of partials by one workgroup of 256 threads. */
+__device__ double sumPartials(const double *p, int n)
+{
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += p[i];
+  const double s = blockReduceSum(acc, red);
+  __shared__ double out;
+  if (threadIdx.x == 0) out = s;
+  __syncthreads();
+  return out;
+}
+
+enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL };
+
+__global__ void __launch_bounds__(256) k_finalize(PcgState *st, const double *partials, int n, int op, double *out)
+{
+  const bool skipIfDone = (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA);
+  if (skipIfDone && st->done) return;
+  const double s = sumPartials(partials, n);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case FIN_BNORM:
+      st->bnorm2 = s;
+      st->done = (s <= 1e-30) ? 1 : 0;
+      st->conv = -1; st->iter = 0; st->brk = (s <= 1e-30) ? 2 : 0;
+      break;
+    case FIN_RHO0: st->rho = s; break;
+    case FIN_ALPHA: {
+      st->dq = s;
+      const double rho = st->rho;
+      if (!(s > 1e-30) || !(rho > 1e-30) || !isfinite(s) || !isfinite(rho)) { st->done = 1; st->brk = 1; break; }
+      const double al = rho / s;
+      if (!isfinite(al)) { st->done = 1; st->brk = 1; break; }
+      st->alpha = (float)al;
+      break;
+    }
+    case FIN_CONV:
+      st->rr = s;
+      st->iter += 1;
+      if (out) out[st->iter - 1] = s;
+      if (s <= st->tolSq * st->bnorm2) { st->done = 1; st->conv = st->iter; }
+      else if (st->iter >= st->maxIter) st->done = 1;
+      break;
+    case FIN_BETA: {
+      if (!(s > 1e-30) || !isfinite(s)) { st->done = 1; st->brk = 1; break; }
+      const double be = s / st->rho;
+      if (!isfinite(be)) { st->done = 1; st->brk = 1; break; }
+      st->beta = (float)be;
+      st->rho = s;
+      break;
+    }
+    case FIN_FINAL: if (out) out[0] = s; break;
+  }
+}
+
+/* Divergence right-hand side of level-0 blocks from the scene's MAC velocity
+ * (flipRhsCB, src/flip_sim.cpp:329-338; sampled as in src/msbg3.cpp:2551-2579). */
+__global__ void k_scene_rhs(msbgx_scene s, const int *__restrict__ blocks, const int64_t *__restrict__ off, int nbx,
+                            int nby, int bsx0, float *__restrict__ Bout)
+{
+  const int bid = blocks[blockIdx.x];
+  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  const int n = bsx0 * bsx0 * bsx0;
+  const int64_t o = off[bid];
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    const int vx = v % bsx0, vy = (v / bsx0) % bsx0, vz = v / (bsx0 * bsx0);
+    Bout[o + v] = msbgx_scene_rhs(&s, bx * bsx0 + vx, by * bsx0 + vy, bz * bsx0 + vz);
+  }
+}
+
+/* -------------------------------------------------------------------- host */
+
+struct Dev {
+  hipStream_t stream = nullptr;
+  PcgState *state = nullptr;
+  double *partials = nullptr;
+  int partialsCap = 0;
+  double *hist = nullptr;
+  int histCap = 0;
+  int *zeroFlag = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graphChunk = 0;
+};
+
+static Dev &dev(Grid &g) { return *(Dev *)g.dev; }
+
+template <typename T> static int dalloc(T **p, size_t n)
+{
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+  return MSBGX_OK;
+}
+
+static void freeLevel(MgLevel &m)
+{
+  auto f = [](void *p) { if (p) (void)hipFree(p); };
+  f(m.d_level); f(m.d_off); f(m.d_flags); f(m.d_D); f(m.d_lists);
+  for (int d = 0; d < 3; d++) { f(m.d_W[d]); m.d_W[d] = nullptr; }
+  for (int c = 0; c < N_VEC; c++) { f(m.d_ch[c]); m.d_ch[c] = nullptr; }
+  m.d_level = nullptr; m.d_off = nullptr; m.d_flags = nullptr; m.d_D = nullptr; m.d_lists = nullptr;
+}
+
+void deviceFree(Grid &g)
+{
+  for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
+  if (g.dev) {
+    Dev &d = dev(g);
+    if (d.graph) (void)hipGraphExecDestroy(d.graph);
+    if (d.state) (void)hipFree(d.state);
+    if (d.partials) (void)hipFree(d.partials);
+    if (d.hist) (void)hipFree(d.hist);
+    if (d.zeroFlag) (void)hipFree(d.zeroFlag);
+    if (d.ev0) (void)hipEventDestroy(d.ev0);
+    if (d.ev1) (void)hipEventDestroy(d.ev1);
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+    delete &d;
+    g.dev = nullptr;
+  }
+}
+
+static LevelArgs levelArgs(const Grid &g, int L)
+{
+  const MgLevel &m = g.mg[L];
+  LevelArgs a;
+  a.level = m.d_level; a.off = m.d_off; a.flags = m.d_flags;
+  a.W0 = m.d_W[0]; a.W1 = m.d_W[1]; a.W2 = m.d_W[2]; a.D = m.d_D;
+  a.nbx = g.nbx; a.nby = g.nby; a.nbz = g.nbz; a.bsx0 = g.bsx0;
+  for (int d = 0; d < 3; d++) a.bcW[d] = g.domainBc[d][1] != MSBGX_DBC_OPEN ? 0.0f : 1.0f;
+  return a;
+}
+
+/* Which channels exist at which MG level. */
+static float *chanPtr(const Grid &g, int ch, int L)
+{
+  if (ch < 0 || ch >= N_VEC) return nullptr;
+  return g.mg[L].d_ch[ch];
+}
+
+int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scene)
+{
+  if (!g.dev) {
+    Dev *d = new Dev();
+    g.dev = d;
+    HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&d->ev0));
+    HIPCHK(hipEventCreate(&d->ev1));
+    if (int rc = dalloc(&d->state, 1)) return rc;
+    if (int rc = dalloc(&d->zeroFlag, 1)) return rc;
+    HIPCHK(hipMemset(d->zeroFlag, 0, sizeof(int)));
+  }
+  Dev &d = dev(g);
+  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
+  int maxBlocksPerLaunch = std::max(g.nBlocks, kVecBlocks);
+  if (d.partialsCap < maxBlocksPerLaunch) {
+    if (d.partials) (void)hipFree(d.partials);
+    d.partials = nullptr;
+    if (int rc = dalloc(&d.partials, maxBlocksPerLaunch)) return rc;
+    d.partialsCap = maxBlocksPerLaunch;
+  }
+  for (int L = 0; L < g.nLevels; L++) {
+    MgLevel &m = g.mg[L];
+    const size_t nc = (size_t)m.ncells, nb = (size_t)g.nBlocks;
+    if (int rc = dalloc(&m.d_level, nb)) return rc;
+    if (int rc = dalloc(&m.d_off, nb)) return rc;
+    if (int rc = dalloc(&m.d_flags, nc)) return rc;
+    if (int rc = dalloc(&m.d_D, nc)) return rc;
+    HIPCHK(hipMemcpy(m.d_level, m.level.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.d_off, m.off.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.d_flags, m.flags.data(), nc * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.d_D, m.D.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; k++) {
+      if (int rc = dalloc(&m.d_W[k], nc)) return rc;
+      HIPCHK(hipMemcpy(m.d_W[k], m.W[k].data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    }
+    const int chans0[] = {C_X, C_B, C_R, C_Z, C_D, C_Q};
+    const int chansL[] = {C_R, C_Z, C_Q};
+    const int *cl = L == 0 ? chans0 : chansL;
+    const int ncl = L == 0 ? 6 : 3;
+    for (int i = 0; i < ncl; i++) {
+      if (int rc = dalloc(&m.d_ch[cl[i]], nc)) return rc;
+      HIPCHK(hipMemsetAsync(m.d_ch[cl[i]], 0, nc * sizeof(float), d.stream));
+    }
+    /* lists: act[lev] ..., relaxCL[c][lev] ..., actAll */
+    std::vector<int> all;
+    std::vector<std::pair<int, int>> spans;
+    for (int l = 0; l < kMaxLevels; l++) { spans.push_back({(int)all.size(), (int)m.act[l].size()}); all.insert(all.end(), m.act[l].begin(), m.act[l].end()); }
+    for (int c = 0; c < 2; c++)
+      for (int l = 0; l < kMaxLevels; l++) {
+        spans.push_back({(int)all.size(), (int)m.relaxCL[c][l].size()});
+        all.insert(all.end(), m.relaxCL[c][l].begin(), m.relaxCL[c][l].end());
+      }
+    if (int rc = dalloc(&m.d_lists, all.size())) return rc;
+    if (!all.empty()) HIPCHK(hipMemcpy(m.d_lists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    int si = 0;
+    for (int l = 0; l < kMaxLevels; l++, si++) m.dAct[l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+    for (int c = 0; c < 2; c++)
+      for (int l = 0; l < kMaxLevels; l++, si++) m.dRelax[c][l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+  }
+  MgLevel &m0 = g.mg[0];
+  if (scene) {
+    /* level-0 blocks holding liquid are the head of the pool (act[0]) */
+    if (m0.dAct[0].n > 0)
+      hipLaunchKernelGGL(k_scene_rhs, dim3(m0.dAct[0].n), dim3(256), 0, d.stream, *scene, m0.dAct[0].ptr, m0.d_off,
+                         g.nbx, g.nby, g.bsx0, m0.d_ch[C_B]);
+    HIPCHK(hipGetLastError());
+  } else if (!rhs.empty()) {
+    HIPCHK(hipMemcpyAsync(m0.d_ch[C_B], rhs.data(), rhs.size() * sizeof(float), hipMemcpyHostToDevice, d.stream));
+  }
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+/* ---------------------------------------------------------- launch helpers */
+
+template <int B>
+static void launchRelaxB(const Grid &g, int L, const DevList &l, float *X, const float *Bv, int reverse, const int *done,
+                         hipStream_t s)
+{
+  if (l.n == 0) return;
+  hipLaunchKernelGGL(k_relax<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, g.omega, reverse, 5,
+                     done);
+}
+
+static void launchRelaxLev(const Grid &g, int L, int lev, const DevList &l, float *X, const float *Bv, int reverse,
+                           const int *done, hipStream_t s)
+{
+  switch (g.bsx0 >> lev) {
+    case 2: launchRelaxB<2>(g, L, l, X, Bv, reverse, done, s); break;
+    case 4: launchRelaxB<4>(g, L, l, X, Bv, reverse, done, s); break;
+    case 8: launchRelaxB<8>(g, L, l, X, Bv, reverse, done, s); break;
+    case 16: launchRelaxB<16>(g, L, l, X, Bv, reverse, done, s); break;
+    case 32: launchRelaxB<32>(g, L, l, X, Bv, reverse, done, s); break;
+  }
+}
+
+/* relax(): nIter outer iterations of two colour passes (src/msbg3.cpp:2245-2369) */
+static void relaxLaunch(const Grid &g, int L, bool reverse, int nIter, float *X, const float *Bv, const int *done,
+                        hipStream_t s)
+{
+  const MgLevel &m = g.mg[L];
+  if (!m.anyRelax) return;
+  for (int it = 0; it < nIter; it++)
+    for (int pass = 0; pass < 2; pass++) {
+      const int parity = reverse ? pass : 1 - pass;
+      for (int lev = L; lev < g.nLevels; lev++)
+        launchRelaxLev(g, L, lev, m.dRelax[parity][lev], X, Bv, reverse ? 1 : 0, done, s);
+    }
+}
+
+template <int B>
+static void launchMultiplyB(const Grid &g, int L, const DevList &l, const float *X, const float *Bv, float *Y, int residual,
+                            double *partials, int base, const int *done, hipStream_t s)
+{
+  if (l.n == 0) return;
+  hipLaunchKernelGGL(k_multiply<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
+                     partials, base, done);
+}
+
+/* multiplyLaplacianMatrixOpt over all non-NO_FLUID blocks of MG level L;
+ * when partials is given, FP64 partials of <x, y> over level-0 blocks are
+ * written at partials[0 .. returned count). */
+static int multiplyLaunch(const Grid &g, int L, bool residual, const float *X, const float *Bv, float *Y,
+                          double *partials, const int *done, hipStream_t s)
+{
+  const MgLevel &m = g.mg[L];
+  int base = 0;
+  for (int lev = L; lev < g.nLevels; lev++) {
+    const DevList &l = m.dAct[lev];
+    switch (g.bsx0 >> lev) {
+      case 2: launchMultiplyB<2>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+      case 4: launchMultiplyB<4>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+      case 8: launchMultiplyB<8>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+      case 16: launchMultiplyB<16>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+      case 32: launchMultiplyB<32>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+    }
+    base += l.n;
+  }
+  return base;
+}
+
+static void zeroActive(const Grid &g, int L, float *X, const int *done, hipStream_t s)
+{
+  const MgLevel &m = g.mg[L];
+  for (int lev = L; lev < g.nLevels; lev++)
+    if (m.dAct[lev].n)
+      hipLaunchKernelGGL(k_zero_blocks, dim3(m.dAct[lev].n), dim3(256), 0, s, m.dAct[lev].ptr, m.d_level, m.d_off,
+                         g.bsx0, X, done);
+}
+
+static void restrictLaunch(const Grid &g, int Lc, const float *src, float *dst, int copyMode, const int *done,
+                           hipStream_t s)
+{
+  const MgLevel &c = g.mg[Lc], &f = g.mg[Lc - 1];
+  const int *level0 = g.mg[0].d_level;
+  for (int lev = Lc; lev < g.nLevels; lev++)
+    if (c.dAct[lev].n)
+      hipLaunchKernelGGL(k_restrict, dim3(c.dAct[lev].n), dim3(256), 0, s, c.dAct[lev].ptr, level0, c.d_level, c.d_off,
+                         f.d_off, Lc, g.bsx0, src, dst, copyMode, done);
+}
+
+static void prolongLaunch(const Grid &g, int Lf, float *xf, const float *xc, const int *done, hipStream_t s)
+{
+  const MgLevel &f = g.mg[Lf], &c = g.mg[Lf + 1];
+  const int *level0 = g.mg[0].d_level;
+  for (int lev = Lf; lev < g.nLevels; lev++)
+    if (f.dAct[lev].n)
+      hipLaunchKernelGGL(k_prolong, dim3(f.dAct[lev].n), dim3(256), 0, s, f.dAct[lev].ptr, level0, f.d_level, f.d_off,
+                         c.d_off, f.d_flags, Lf, g.bsx0, xf, xc, done);
+}
+
+/* True when every MG level below L has an empty relaxation list: the coarse
+ * correction of vCycle(L+1) is then exactly zero. */
+static bool coarseIsInert(const Grid &g, int L)
+{
+  for (int l = L + 1; l < g.nLevels; l++)
+    if (g.mg[l].anyRelax) return false;
+  return true;
+}
+
+/* vCycle (src/msbg3.cpp:2412-2465) on channels z (x), r (rhs), q (residual). */
+static void vcycleLaunch(Grid &g, int L, int nPre, int nPost, int nCoarse, const int *done, hipStream_t s)
+{
+  MgLevel &m = g.mg[L];
+  float *X = m.d_ch[C_Z], *Bv = m.d_ch[C_R], *R = m.d_ch[C_Q];
+  if (L >= g.nLevels - 1) {
+    relaxLaunch(g, L, false, nCoarse, X, Bv, done, s);
+    return;
+  }
+  relaxLaunch(g, L, false, nPre, X, Bv, done, s);
+  const bool inert = coarseIsInert(g, L) && !(g.options & MSBGX_OPT_LITERAL_VCYCLE);
+  if (!inert) {
+    multiplyLaunch(g, L, true, X, Bv, R, nullptr, done, s);
+    restrictLaunch(g, L + 1, R, g.mg[L + 1].d_ch[C_R], g.aliasEstablished[L + 1] ? 0 : 1, done, s);
+    g.aliasEstablished[L + 1] = true;
+    zeroActive(g, L + 1, g.mg[L + 1].d_ch[C_Z], done, s);
+    vcycleLaunch(g, L + 1, nPre, nPost, nCoarse, done, s);
+    prolongLaunch(g, L, X, g.mg[L + 1].d_ch[C_Z], done, s);
+  } else {
+    /* the reference still creates the coarse right-hand-side block references */
+    for (int l = L + 1; l < g.nLevels; l++) g.aliasEstablished[l] = true;
+  }
+  relaxLaunch(g, L, true, nPost, X, Bv, done, s);
+}
+
+/* ------------------------------------------------------------------- PCG */
+
+static void finalize(Dev &d, int n, int op, double *out)
+{
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.state, d.partials, n, op, out);
+}
+
+/* One PCG iteration (src/msbg3.cpp:3149-3236), every kernel a no-op once
+ * the device-side done flag is set. */
+static void pcgIteration(Grid &g, const msbgx_solve_params &p)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *done = &d.state->done;
+  const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+  const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
+  finalize(d, nPart, FIN_ALPHA, nullptr);
+  hipLaunchKernelGGL(k_update_xr, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
+  finalize(d, kVecBlocks, FIN_CONV, d.hist);
+  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
+                     done);
+  finalize(d, kVecBlocks, FIN_BETA, nullptr);
+  hipLaunchKernelGGL(k_axpy_d, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], n0,
+                     d.state);
+}
+
+int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+  if (d.histCap < p.max_iter + 1) {
+    if (d.hist) (void)hipFree(d.hist);
+    d.hist = nullptr;
+    if (int rc = dalloc(&d.hist, p.max_iter + 1)) return rc;
+    d.histCap = p.max_iter + 1;
+  }
+  PcgState init{};
+  init.tolSq = (double)p.tol * (double)p.tol;
+  init.maxIter = p.max_iter;
+  HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
+  HIPCHK(hipEventRecord(d.ev0, d.stream));
+
+  /* r = b and |b|^2 over the level-0 blocks; r = b elsewhere is zero */
+  hipLaunchKernelGGL(k_copy_norm, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_B], m0.d_ch[C_R], n0,
+                     d.partials);
+  finalize(d, kVecBlocks, FIN_BNORM, nullptr);
+  const int *done = &d.state->done;
+  /* z = M^-1 r; d = z on every block of MG0; rho = <r, z> */
+  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+  for (int lev = 0; lev < g.nLevels; lev++)
+    if (m0.dAct[lev].n)
+      hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
+                         m0.d_off, g.bsx0, m0.d_ch[C_Z], m0.d_ch[C_D], done);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
+                     done);
+  finalize(d, kVecBlocks, FIN_RHO0, nullptr);
+  HIPCHK(hipGetLastError());
+
+  const bool useGraph = !(g.options & MSBGX_OPT_NO_GRAPH);
+  const int chunk = 4;
+  PcgState hs{};
+  for (int done_it = 0; done_it < p.max_iter;) {
+    const int n = std::min(chunk, p.max_iter - done_it);
+    if (useGraph && n == chunk) {
+      if (!d.graph) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < chunk; i++) pcgIteration(g, p);
+        HIPCHK(hipStreamEndCapture(d.stream, &graph));
+        HIPCHK(hipGraphInstantiate(&d.graph, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+      }
+      HIPCHK(hipGraphLaunch(d.graph, d.stream));
+    } else {
+      for (int i = 0; i < n; i++) pcgIteration(g, p);
+    }
+    done_it += n;
+    HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    if (hs.done) break;
+  }
+  /* final |r| */
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_R], n0, d.partials,
+                     d.zeroFlag);
+  finalize(d, kVecBlocks, FIN_FINAL, d.hist + p.max_iter);
+  HIPCHK(hipEventRecord(d.ev1, d.stream));
+  HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+  double fin = 0.0;
+  HIPCHK(hipMemcpyAsync(&fin, d.hist + p.max_iter, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+  g.lastSolveMs = ms;
+  g.history.assign(hs.iter, 0.0);
+  if (hs.iter > 0) {
+    std::vector<double> rr(hs.iter);
+    HIPCHK(hipMemcpy(rr.data(), d.hist, hs.iter * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < hs.iter; i++) g.history[i] = std::sqrt(rr[i] / hs.bnorm2);
+  }
+  if (hs.brk == 2) {  // |b|^2 <= 1e-30
+    *iters = 0;
+    *rel = 0.0f;
+    return MSBGX_OK;
+  }
+  *rel = (float)std::sqrt(fin / hs.bnorm2);
+  *iters = hs.conv > 0 ? hs.conv : p.max_iter;
+  return MSBGX_OK;
+}
+
+/* ------------------------------------------------------ per-kernel hooks */
+
+static int needChan(const Grid &g, int ch, int L, float **p)
+{
+  *p = chanPtr(g, ch, L);
+  if (!*p) {
+    setError("channel does not exist at this MG level");
+    return MSBGX_EINVAL;
+  }
+  return MSBGX_OK;
+}
+
+int deviceGetBlock(const Grid &g, int ch, int L, int bid, void *dst)
+{
+  const MgLevel &m = g.mg[L];
+  const int b = g.bsxL(m.level[bid]);
+  const size_t n = (size_t)b * b * b;
+  const int64_t o = m.off[bid];
+  const void *src = nullptr;
+  size_t es = sizeof(float);
+  if (ch == MSBGX_CH_FLAGS) { src = m.d_flags + o; es = sizeof(uint16_t); }
+  else if (ch >= MSBGX_CH_FACE_U && ch <= MSBGX_CH_FACE_W) src = m.d_W[ch - MSBGX_CH_FACE_U] + o;
+  else if (ch == MSBGX_CH_DIAGONAL) src = m.d_D + o;
+  else {
+    float *p = chanPtr(g, ch, L);
+    if (!p) return 0;
+    src = p + o;
+  }
+  HIPCHK(hipMemcpy(dst, src, n * es, hipMemcpyDeviceToHost));
+  return b;
+}
+
+int deviceSetBlock(Grid &g, int ch, int L, int bid, const float *src)
+{
+  float *p;
+  if (int rc = needChan(g, ch, L, &p)) return rc;
+  const MgLevel &m = g.mg[L];
+  const int b = g.bsxL(m.level[bid]);
+  HIPCHK(hipMemcpy(p + m.off[bid], src, (size_t)b * b * b * sizeof(float), hipMemcpyHostToDevice));
+  return b;
+}
+
+int deviceRelax(Grid &g, int L, bool reverse, int nIter, int chX, int chB)
+{
+  float *X, *Bv;
+  if (int rc = needChan(g, chX, L, &X)) return rc;
+  if (int rc = needChan(g, chB, L, &Bv)) return rc;
+  Dev &d = dev(g);
+  relaxLaunch(g, L, reverse, nIter, X, Bv, d.zeroFlag, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst)
+{
+  float *X, *Y, *Bv = nullptr;
+  if (int rc = needChan(g, chX, L, &X)) return rc;
+  if (int rc = needChan(g, chDst, L, &Y)) return rc;
+  if (residual)
+    if (int rc = needChan(g, chB, L, &Bv)) return rc;
+  Dev &d = dev(g);
+  multiplyLaunch(g, L, residual, X, Bv, Y, nullptr, d.zeroFlag, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst)
+{
+  float *S, *D;
+  if (int rc = needChan(g, chSrc, Lc - 1, &S)) return rc;
+  if (int rc = needChan(g, chDst, Lc, &D)) return rc;
+  Dev &d = dev(g);
+  restrictLaunch(g, Lc, S, D, g.aliasEstablished[Lc] ? 0 : 1, d.zeroFlag, d.stream);
+  g.aliasEstablished[Lc] = true;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceProlongate(Grid &g, int Lf, int ch)
+{
+  float *F, *C;
+  if (int rc = needChan(g, ch, Lf, &F)) return rc;
+  if (int rc = needChan(g, ch, Lf + 1, &C)) return rc;
+  Dev &d = dev(g);
+  prolongLaunch(g, Lf, F, C, d.zeroFlag, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceZero(Grid &g, int ch, int L)
+{
+  float *X;
+  if (int rc = needChan(g, ch, L, &X)) return rc;
+  Dev &d = dev(g);
+  zeroActive(g, L, X, d.zeroFlag, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceCopy(Grid &g, int src, int dst)
+{
+  float *S, *D;
+  if (int rc = needChan(g, src, 0, &S)) return rc;
+  if (int rc = needChan(g, dst, 0, &D)) return rc;
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  for (int lev = 0; lev < g.nLevels; lev++)
+    if (m0.dAct[lev].n)
+      hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
+                         m0.d_off, g.bsx0, S, D, d.zeroFlag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse)
+{
+  Dev &d = dev(g);
+  zeroActive(g, 0, g.mg[0].d_ch[C_Z], d.zeroFlag, d.stream);
+  vcycleLaunch(g, 0, nPre, nPost, nCoarse, d.zeroFlag, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+}  // namespace msbgx

@@ -1,0 +1,324 @@
+"""Python host mirror of the msbgx C ABI (include/msbgx.h).
+
+Mirrors the reference's MultiresSparseGrid pressure-solver interface
+(src/msbg.h:829-940): ``MultiresSparseGrid.create`` / ``regularize_refinement_map``
+/ ``set_refinement_map`` / ``set_domain_boundary_spec`` /
+``prepare_pressure_solve_flip`` / ``solve_pressure_flip``, over ctypes.  The
+product path is the HIP library ``libmsbgx.so`` next to this file; importing
+this module fails loudly when that library has not been built.
+
+Also holds the synthetic scene builder for the five benchmark configurations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsbgx.so")
+
+MAX_SPHERES = 64
+MAX_BOXES = 4
+
+CH_PRESSURE, CH_DIVERGENCE, CH_R, CH_Z, CH_D, CH_Q = 0, 1, 2, 3, 4, 5
+CH_FACE_U, CH_FACE_V, CH_FACE_W, CH_DIAGONAL, CH_FLAGS = 7, 8, 9, 10, 11
+DBC_OPEN, DBC_SOLID = 0, 1
+OPT_LITERAL_VCYCLE, OPT_NO_GRAPH = 0x1, 0x2
+CELL_SOLID, CELL_VOID = 0x0001, 0x1000
+BLK_COARSE_FINE, BLK_FINE_COARSE, BLK_NO_FLUID, BLK_EXISTS = 1 << 2, 1 << 3, 1 << 5, 1 << 12
+
+
+class Scene(C.Structure):
+    """ctypes image of ``msbgx_scene`` (include/msbgx_scene.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("n_spheres", C.c_int32), ("n_boxes", C.c_int32), ("wall_faces_closed", C.c_int32),
+        ("seed", C.c_uint64), ("dt", C.c_float), ("vel_scale", C.c_float),
+        ("sphere", (C.c_double * 4) * MAX_SPHERES), ("box", (C.c_double * 6) * MAX_BOXES),
+    ]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("n_pre", C.c_int), ("n_post", C.c_int), ("n_coarse", C.c_int),
+                ("max_iter", C.c_int), ("tol", C.c_float)]
+
+
+class Callbacks(C.Structure):
+    _fields_ = [("sample_cell_type", C.CFUNCTYPE(C.c_uint16, C.c_void_p, C.c_int, C.c_int, C.c_int)),
+                ("sample_face_coeff", C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)),
+                ("sample_rhs", C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int)),
+                ("user", C.c_void_p)]
+
+
+def make_scene(n, spheres=(), boxes=(), seed=1234, dt=1.0, vel_scale=1.0, walls_closed=True):
+    """Build a Scene for an n^3 domain (n may be a 3-tuple)."""
+    s = Scene()
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    s.nx, s.ny, s.nz = nx, ny, nz
+    if len(spheres) > MAX_SPHERES or len(boxes) > MAX_BOXES:
+        raise ValueError("too many primitives")
+    s.n_spheres, s.n_boxes = len(spheres), len(boxes)
+    for i, sp in enumerate(spheres):
+        for k in range(4):
+            s.sphere[i][k] = float(sp[k])
+    for i, b in enumerate(boxes):
+        for k in range(6):
+            s.box[i][k] = float(b[k])
+    s.seed, s.dt, s.vel_scale = int(seed), float(dt), float(vel_scale)
+    s.wall_faces_closed = 1 if walls_closed else 0
+    return s
+
+
+@dataclass
+class Config:
+    """One synthetic workload: scene plus block size, levels and level policy."""
+    name: str
+    n: int
+    bsx: int
+    levels: int
+    scene: Scene
+    level_mode: int  # -1: every block level 0; else the flip_sim policy mode
+    level_d0: int = 1
+
+
+def config_c1(n=256, seed=1234):
+    """Plain MG-PCG, single level: sphere r=0.3 at (0.5,0.5,0.4), 16^3 blocks."""
+    s = make_scene(n, spheres=[(0.5, 0.5, 0.4, 0.3)], seed=seed)
+    return Config("c1_sphere_single_level", n, 16, 3, s, level_mode=-1)
+
+
+def config_c2(n=512, bsx=32, seed=1234):
+    """Multires: bottom slab y<0.2 plus 64 spheres r~U[0.03,0.08], c~U[0.1,0.9]^3,
+    3 levels assigned by block distance to the liquid surface (fine within 2 blocks)."""
+    rng = np.random.default_rng(seed)
+    sph = [(*rng.uniform(0.1, 0.9, 3), rng.uniform(0.03, 0.08)) for _ in range(64)]
+    s = make_scene(n, spheres=sph, boxes=[(0.0, 1.0, 0.0, 0.2, 0.0, 1.0)], seed=seed)
+    return Config("c2_slab_spheres_multires", n, bsx, 3, s, level_mode=1, level_d0=2)
+
+
+def config_c4(n=256, seed=1234):
+    """Dam break: box x<0.4, y<0.6 plus a sphere r~U[0.1,0.2]."""
+    rng = np.random.default_rng(seed)
+    c = (rng.uniform(0.3, 0.7), rng.uniform(0.6, 0.9), rng.uniform(0.3, 0.7))
+    r = rng.uniform(0.1, 0.2)
+    s = make_scene(n, spheres=[(*c, r)], boxes=[(0.0, 0.4, 0.0, 0.6, 0.0, 1.0)], seed=seed)
+    return Config("c4_dam_break", n, 16, 3, s, level_mode=0, level_d0=1)
+
+
+def config_c5(n=1024, seed=1234):
+    """Full scale: dam-break box plus sphere r=0.15 at (0.7,0.8,0.5), 32^3 blocks, 4 levels."""
+    s = make_scene(n, spheres=[(0.7, 0.8, 0.5, 0.15)], boxes=[(0.0, 0.4, 0.0, 0.6, 0.0, 1.0)], seed=seed)
+    return Config("c5_dam_break_full", n, 32, 4, s, level_mode=0, level_d0=1)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libmsbgx.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "msbgx_last_error": (C.c_char_p, []),
+        "msbgx_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "msbgx_destroy": (None, [P]),
+        "msbgx_num_blocks": (C.c_int, [P]),
+        "msbgx_set_options": (C.c_int, [P, C.c_uint]),
+        "msbgx_set_domain_bc": (C.c_int, [P] + [C.c_int] * 6),
+        "msbgx_regularize_refinement_map": (C.c_int, [P, P, P]),
+        "msbgx_set_refinement_map": (C.c_int, [P, P]),
+        "msbgx_prepare_pressure_solve": (C.c_int, [P, C.POINTER(Callbacks)]),
+        "msbgx_prepare_pressure_solve_dense": (C.c_int, [P, P, P, P, P, P]),
+        "msbgx_prepare_pressure_solve_scene": (C.c_int, [P, C.POINTER(Scene)]),
+        "msbgx_solve_pressure": (C.c_int, [P, C.POINTER(SolveParams), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+        "msbgx_residual_history": (C.c_int, [P, P, C.c_int]),
+        "msbgx_last_solve_ms": (C.c_double, [P]),
+        "msbgx_get_channel_block": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
+        "msbgx_set_channel_block": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
+        "msbgx_block_info": (C.c_int, [P, C.c_int, P, P]),
+        "msbgx_relax_list": (C.c_int, [P, C.c_int, P, C.c_int]),
+        "msbgx_omega": (C.c_float, [P]),
+        "msbgx_relax": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "msbgx_multiply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "msbgx_downsample": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+        "msbgx_prolongate": (C.c_int, [P, C.c_int, C.c_int]),
+        "msbgx_zero_channel": (C.c_int, [P, C.c_int, C.c_int]),
+        "msbgx_copy_channel": (C.c_int, [P, C.c_int, C.c_int]),
+        "msbgx_vcycle": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+        "msbgx_level_map_from_scene": (C.c_int, [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    return lib
+
+
+lib = _load()
+
+#: every symbol declared in include/msbgx.h
+EXPORTED = [
+    "msbgx_last_error", "msbgx_default_solve_params", "msbgx_create", "msbgx_destroy", "msbgx_num_blocks",
+    "msbgx_set_options", "msbgx_set_domain_bc", "msbgx_regularize_refinement_map", "msbgx_set_refinement_map",
+    "msbgx_prepare_pressure_solve", "msbgx_prepare_pressure_solve_dense", "msbgx_prepare_pressure_solve_scene",
+    "msbgx_solve_pressure", "msbgx_residual_history", "msbgx_last_solve_ms", "msbgx_get_channel_block",
+    "msbgx_set_channel_block", "msbgx_block_info", "msbgx_relax_list", "msbgx_omega", "msbgx_relax",
+    "msbgx_multiply", "msbgx_downsample", "msbgx_prolongate", "msbgx_zero_channel", "msbgx_copy_channel",
+    "msbgx_vcycle", "msbgx_level_map_from_scene",
+]
+
+
+class MsbgxError(RuntimeError):
+    def __init__(self, rc, where):
+        super().__init__(f"{where} failed (rc={rc}): {lib.msbgx_last_error().decode()}")
+        self.rc = rc
+
+
+def _ck(rc, where):
+    if rc < 0:
+        raise MsbgxError(rc, where)
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def level_map_from_scene(scene, bsx, levels, mode, d0):
+    nb = (scene.nx // bsx) * (scene.ny // bsx) * (scene.nz // bsx)
+    out = np.zeros(nb, np.int32)
+    _ck(lib.msbgx_level_map_from_scene(C.byref(scene), bsx, levels, mode, d0, _ptr(out)), "level_map")
+    return out
+
+
+class MultiresSparseGrid:
+    """Handle over ``msbgx_grid``; method names follow src/msbg.h."""
+
+    def __init__(self, h):
+        self._h = h
+
+    @classmethod
+    def create(cls, sx0, sy0, sz0, block_size0=16, n_levels=3):
+        h = C.c_void_p()
+        _ck(lib.msbgx_create(C.byref(h), sx0, sy0, sz0, block_size0, n_levels), "create")
+        g = cls(h)
+        g.shape = (sx0, sy0, sz0)
+        g.bsx, g.n_levels = block_size0, n_levels
+        g.n_blocks = lib.msbgx_num_blocks(h)
+        return g
+
+    def destroy(self):
+        if self._h:
+            lib.msbgx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def set_options(self, opts):
+        _ck(lib.msbgx_set_options(self._h, opts), "set_options")
+
+    def set_domain_boundary_spec(self, left, right, bottom, top, front, back):
+        _ck(lib.msbgx_set_domain_bc(self._h, left, right, bottom, top, front, back), "set_domain_bc")
+
+    def regularize_refinement_map(self, levels):
+        a = np.ascontiguousarray(levels, np.int32)
+        out = np.empty_like(a)
+        _ck(lib.msbgx_regularize_refinement_map(self._h, _ptr(a), _ptr(out)), "regularize")
+        return out
+
+    def set_refinement_map(self, levels):
+        a = np.ascontiguousarray(levels, np.int32)
+        _ck(lib.msbgx_set_refinement_map(self._h, _ptr(a)), "set_refinement_map")
+
+    def prepare_pressure_solve_flip(self, scene=None, callbacks=None, dense=None):
+        if scene is not None:
+            self._scene = scene
+            return _ck(lib.msbgx_prepare_pressure_solve_scene(self._h, C.byref(scene)), "prepare(scene)")
+        if callbacks is not None:
+            self._cb = callbacks
+            return _ck(lib.msbgx_prepare_pressure_solve(self._h, C.byref(callbacks)), "prepare(callbacks)")
+        flags, wu, wv, ww, rhs = dense
+        self._dense = [np.ascontiguousarray(flags, np.uint16)] + [np.ascontiguousarray(a, np.float32)
+                                                                  for a in (wu, wv, ww, rhs)]
+        return _ck(lib.msbgx_prepare_pressure_solve_dense(self._h, *[_ptr(a) for a in self._dense]), "prepare(dense)")
+
+    def solve_pressure_flip(self, n_pre=3, n_post=3, n_coarse=24, max_iter=500, tol=5e-4):
+        p = SolveParams(n_pre, n_post, n_coarse, max_iter, tol)
+        it, rr = C.c_int(), C.c_float()
+        _ck(lib.msbgx_solve_pressure(self._h, C.byref(p), C.byref(it), C.byref(rr)), "solve")
+        return it.value, rr.value
+
+    def residual_history(self):
+        n = lib.msbgx_residual_history(self._h, None, 0)
+        out = np.zeros(max(n, 1))
+        lib.msbgx_residual_history(self._h, _ptr(out), n)
+        return out[:n]
+
+    def last_solve_ms(self):
+        return lib.msbgx_last_solve_ms(self._h)
+
+    def block_info(self, mg=0):
+        lv = np.zeros(self.n_blocks, np.int32)
+        fl = np.zeros(self.n_blocks, np.uint32)
+        _ck(lib.msbgx_block_info(self._h, mg, _ptr(lv), _ptr(fl)), "block_info")
+        return lv, fl
+
+    def relax_list(self, mg=0):
+        out = np.zeros(self.n_blocks, np.int32)
+        n = lib.msbgx_relax_list(self._h, mg, _ptr(out), self.n_blocks)
+        return out[:n].copy()
+
+    def omega(self):
+        return lib.msbgx_omega(self._h)
+
+    def get_block(self, ch, mg, bid, edge):
+        dt = np.uint16 if ch == CH_FLAGS else np.float32
+        out = np.zeros(edge ** 3, dt)
+        b = _ck(lib.msbgx_get_channel_block(self._h, ch, mg, bid, _ptr(out)), "get_block")
+        return out if b else None
+
+    def set_block(self, ch, mg, bid, data):
+        a = np.ascontiguousarray(data, np.float32)
+        _ck(lib.msbgx_set_channel_block(self._h, ch, mg, bid, _ptr(a)), "set_block")
+
+    def relax(self, mg, reverse, n_iter, ch_x, ch_b):
+        _ck(lib.msbgx_relax(self._h, mg, int(reverse), n_iter, ch_x, ch_b), "relax")
+
+    def multiply(self, mg, residual, ch_x, ch_b, ch_dst):
+        _ck(lib.msbgx_multiply(self._h, mg, int(residual), ch_x, ch_b, ch_dst), "multiply")
+
+    def downsample(self, mg_coarse, ch_src, ch_dst):
+        _ck(lib.msbgx_downsample(self._h, mg_coarse, ch_src, ch_dst), "downsample")
+
+    def prolongate(self, mg_fine, ch):
+        _ck(lib.msbgx_prolongate(self._h, mg_fine, ch), "prolongate")
+
+    def zero_channel(self, ch, mg):
+        _ck(lib.msbgx_zero_channel(self._h, ch, mg), "zero_channel")
+
+    def copy_channel(self, src, dst):
+        _ck(lib.msbgx_copy_channel(self._h, src, dst), "copy_channel")
+
+    def vcycle(self, n_pre=3, n_post=3, n_coarse=24):
+        _ck(lib.msbgx_vcycle(self._h, n_pre, n_post, n_coarse), "vcycle")
+
+
+def build_grid(cfg: Config, walls=DBC_SOLID):
+    """create + boundary spec + level map (regularised) + setRefinementMap."""
+    g = MultiresSparseGrid.create(cfg.n, cfg.n, cfg.n, cfg.bsx, cfg.levels)
+    g.set_domain_boundary_spec(*([walls] * 6))
+    lv = level_map(cfg)
+    g.set_refinement_map(g.regularize_refinement_map(lv))
+    return g
+
+
+def level_map(cfg: Config):
+    nb = (cfg.n // cfg.bsx) ** 3
+    if cfg.level_mode < 0:
+        return np.zeros(nb, np.int32)
+    return level_map_from_scene(cfg.scene, cfg.bsx, cfg.levels, cfg.level_mode, cfg.level_d0)

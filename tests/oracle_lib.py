@@ -1,0 +1,158 @@
+"""ctypes binding of the CPU oracle (oracle/libmsbg_oracle.so) -- test
+infrastructure only; mirrors the entry points of msbgx.MultiresSparseGrid."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from msbgx import Scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "libmsbg_oracle.so")
+
+_lib = C.CDLL(ORACLE_PATH)
+P = C.c_void_p
+for name, res, args in [
+    ("orc_create", P, [C.c_int] * 5),
+    ("orc_destroy", None, [P]),
+    ("orc_set_threads", None, [P, C.c_int]),
+    ("orc_num_blocks", C.c_int, [P]),
+    ("orc_set_domain_bc", None, [P] + [C.c_int] * 6),
+    ("orc_regularize_refinement_map", None, [P, P, P]),
+    ("orc_set_refinement_map", None, [P, P]),
+    ("orc_level_map_from_scene", None, [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, P]),
+    ("orc_prepare_scene", None, [P, C.POINTER(Scene)]),
+    ("orc_prepare_dense", None, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
+    ("orc_omega", C.c_float, [P]),
+    ("orc_block_info", None, [P, C.c_int, P, P]),
+    ("orc_relax_list", C.c_int, [P, C.c_int, P, C.c_int]),
+    ("orc_get_block", C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
+    ("orc_set_block", C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
+    ("orc_relax", None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("orc_multiply", None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("orc_downsample", None, [P, C.c_int, C.c_int, C.c_int]),
+    ("orc_prolongate", None, [P, C.c_int, C.c_int]),
+    ("orc_zero_channel", None, [P, C.c_int, C.c_int]),
+    ("orc_vcycle", None, [P, C.c_int, C.c_int, C.c_int]),
+    ("orc_copy_channel", None, [P, C.c_int, C.c_int]),
+    ("orc_solve", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float), P, C.c_int]),
+    ("orc_export_level0", None, [P, C.c_int, P]),
+]:
+    f = getattr(_lib, name)
+    f.restype, f.argtypes = res, args
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def level_map_from_scene(scene, bsx, levels, mode, d0):
+    nb = (scene.nx // bsx) * (scene.ny // bsx) * (scene.nz // bsx)
+    out = np.zeros(nb, np.int32)
+    _lib.orc_level_map_from_scene(C.byref(scene), bsx, levels, mode, d0, _p(out))
+    return out
+
+
+class Oracle:
+    def __init__(self, sx, sy, sz, bsx, levels, threads=None):
+        self._h = _lib.orc_create(sx, sy, sz, bsx, levels)
+        if not self._h:
+            raise ValueError("oracle create failed")
+        self._h = C.c_void_p(self._h)
+        self.n_blocks = _lib.orc_num_blocks(self._h)
+        self.bsx, self.n_levels = bsx, levels
+        _lib.orc_set_threads(self._h, threads or min(16, os.cpu_count() or 1))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.orc_destroy(self._h)
+            self._h = None
+
+    def set_domain_boundary_spec(self, *bc):
+        _lib.orc_set_domain_bc(self._h, *bc)
+
+    def regularize_refinement_map(self, levels):
+        a = np.ascontiguousarray(levels, np.int32)
+        out = np.empty_like(a)
+        _lib.orc_regularize_refinement_map(self._h, _p(a), _p(out))
+        return out
+
+    def set_refinement_map(self, levels):
+        a = np.ascontiguousarray(levels, np.int32)
+        _lib.orc_set_refinement_map(self._h, _p(a))
+
+    def prepare_scene(self, scene):
+        self._scene = scene
+        _lib.orc_prepare_scene(self._h, C.byref(scene))
+
+    def prepare_dense(self, n, flags, wu, wv, ww, rhs):
+        self._dense = [np.ascontiguousarray(flags, np.uint16)] + [np.ascontiguousarray(a, np.float32)
+                                                                  for a in (wu, wv, ww, rhs)]
+        _lib.orc_prepare_dense(self._h, n[0], n[1], n[2], *[_p(a) for a in self._dense])
+
+    def omega(self):
+        return _lib.orc_omega(self._h)
+
+    def block_info(self, mg=0):
+        lv = np.zeros(self.n_blocks, np.int32)
+        fl = np.zeros(self.n_blocks, np.uint32)
+        _lib.orc_block_info(self._h, mg, _p(lv), _p(fl))
+        return lv, fl
+
+    def relax_list(self, mg=0):
+        out = np.zeros(self.n_blocks, np.int32)
+        n = _lib.orc_relax_list(self._h, mg, _p(out), self.n_blocks)
+        return out[:n].copy()
+
+    def get_block(self, ch, mg, bid, edge):
+        out = np.zeros(edge ** 3, np.uint16 if ch == 11 else np.float32)
+        b = _lib.orc_get_block(self._h, ch, mg, bid, _p(out))
+        return out if b else None
+
+    def set_block(self, ch, mg, bid, data):
+        a = np.ascontiguousarray(data, np.float32)
+        _lib.orc_set_block(self._h, ch, mg, bid, _p(a))
+
+    def relax(self, mg, reverse, n_iter, ch_x, ch_b):
+        _lib.orc_relax(self._h, mg, int(reverse), n_iter, ch_x, ch_b)
+
+    def multiply(self, mg, residual, ch_x, ch_b, ch_dst):
+        _lib.orc_multiply(self._h, mg, int(residual), ch_x, ch_b, ch_dst)
+
+    def downsample(self, mg_coarse, ch_src, ch_dst):
+        _lib.orc_downsample(self._h, mg_coarse, ch_src, ch_dst)
+
+    def prolongate(self, mg_fine, ch):
+        _lib.orc_prolongate(self._h, mg_fine, ch)
+
+    def zero_channel(self, ch, mg):
+        _lib.orc_zero_channel(self._h, ch, mg)
+
+    def copy_channel(self, src, dst):
+        _lib.orc_copy_channel(self._h, src, dst)
+
+    def vcycle(self, n_pre=3, n_post=3, n_coarse=24):
+        _lib.orc_vcycle(self._h, n_pre, n_post, n_coarse)
+
+    def solve(self, n_pre=3, n_post=3, n_coarse=24, max_iter=500, tol=5e-4):
+        rr = C.c_float()
+        hist = np.zeros(max_iter + 1)
+        it = _lib.orc_solve(self._h, n_pre, n_post, n_coarse, max_iter, tol, C.byref(rr), _p(hist), max_iter + 1)
+        return it, rr.value, hist
+
+    def export_level0(self, ch, shape):
+        out = np.zeros(shape[0] * shape[1] * shape[2], np.float32)
+        _lib.orc_export_level0(self._h, ch, _p(out))
+        return out.reshape(shape[2], shape[1], shape[0])
+
+
+def build_oracle(cfg, walls=1, threads=None):
+    o = Oracle(cfg.n, cfg.n, cfg.n, cfg.bsx, cfg.levels, threads)
+    o.set_domain_boundary_spec(*([walls] * 6))
+    nb = (cfg.n // cfg.bsx) ** 3
+    if cfg.level_mode < 0:
+        lv = np.zeros(nb, np.int32)
+    else:
+        lv = level_map_from_scene(cfg.scene, cfg.bsx, cfg.levels, cfg.level_mode, cfg.level_d0)
+    o.set_refinement_map(o.regularize_refinement_map(lv))
+    return o
