@@ -1,0 +1,201 @@
+"""Benchmark: time to 1e-6 relative residual of the MSBG FLIP pressure solve
+(MG-PCG) on one MI355X, plus the smoother's HBM roofline.
+
+Workload (default): 512^3 effective domain, 32^3 base blocks, 3 levels
+assigned by block distance to the liquid surface; liquid = bottom slab
+y < 0.2 plus 64 seeded spheres; RHS = divergence of a seeded uniform [-1,1]
+MAC velocity; FP32 storage, FP64 dots.  A step = one solve from p = 0 to
+|r| <= 1e-6 |b| with the inputs resident in HBM.
+
+Prints one JSON line (see the repository's benchmark contract).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "msbg-flip_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--n", type=int, default=0, help="override the domain size")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iter", type=int, default=2000)
+    ap.add_argument("--cpu-iters", type=int, default=3, help="PCG iterations in the CPU-baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
+    ap.add_argument("--no-graph", action="store_true", help="launch PCG iterations eagerly (no hipGraph)")
+    ap.add_argument("--literal", action="store_true", help="run the V-cycle residual/transfer work even when "
+                    "every coarser level has no relaxation block")
+    return ap.parse_args()
+
+
+def make_cfg(msbgx, name, n):
+    if name == "c1":
+        return msbgx.config_c1(n or 256)
+    if name == "c2":
+        return msbgx.config_c2(n or 512, 32)
+    if name == "c4":
+        return msbgx.config_c4(n or 256)
+    return msbgx.config_c5(n or 1024)
+
+
+def gpu_run(msbgx, cfg, args, log):
+    t0 = time.time()
+    g = msbgx.build_grid(cfg)
+    g.set_options((msbgx.OPT_NO_GRAPH if args.no_graph else 0) | (msbgx.OPT_LITERAL_VCYCLE if args.literal else 0))
+    g.prepare_pressure_solve_flip(scene=cfg.scene)
+    log(f"prepare {cfg.name} n={cfg.n}: {time.time() - t0:.1f} s")
+    lv, fl = g.block_info(0)
+    fluid = int(((fl & msbgx.BLK_NO_FLUID) == 0).sum())
+    relax_blocks = len(g.relax_list(0))
+    for i in range(args.warmup):
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+        log(f"warmup {i}: iters {it} rel {rr:.3e} {g.last_solve_ms():.2f} ms")
+    times, iters, rels = [], [], []
+    wall0 = time.time()
+    for i in range(args.steps):
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+        times.append(g.last_solve_ms())
+        iters.append(it)
+        rels.append(rr)
+        log(f"step {i}: iters {it} rel {rr:.3e} {times[-1]:.2f} ms")
+    wall = time.time() - wall0
+    # smoother roofline: one colour pass of the dominant block size, HIP events
+    ms_relax, bytes_relax, cells_relax = g.time_kernel(0, reps=20)
+    ms_res, bytes_res, _ = g.time_kernel(1, reps=20)
+    ms_upd, bytes_upd, _ = g.time_kernel(2, reps=20)
+    return dict(g=g, times=times, iters=iters, rels=rels, wall=wall, fluid_blocks=fluid, relax_blocks=relax_blocks,
+                levels=[int(c) for c in __import__("numpy").bincount(lv, minlength=cfg.levels)],
+                relax=(ms_relax, bytes_relax, cells_relax), residual=(ms_res, bytes_res), update=(ms_upd, bytes_upd))
+
+
+def cpu_baseline(cfg, args, gpu_iters, log):
+    """CPU oracle (C++ restatement of the reference solver) timed on the host
+    cores on a bounded sample: `cpu_iters` PCG iterations of the same solve."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    outdir = tempfile.mkdtemp(prefix="msbg_oracle_")
+    so = os.path.join(outdir, "libmsbg_oracle.so")
+    try:
+        subprocess.run(["g++", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+                        "-std=c++17", "-shared", "-o", so, os.path.join(ROOT, "oracle", "msbg_oracle.cpp")],
+                       check=True, capture_output=True)
+        os.environ["MSBG_ORACLE_SO"] = so
+    except Exception as e:  # fall back to the prebuilt portable build
+        log(f"native oracle build failed ({e}); using oracle/libmsbg_oracle.so")
+    import oracle_lib
+    threads = min(args.cpu_threads, os.cpu_count() or 1)
+    o = oracle_lib.build_oracle(cfg, threads=threads)
+    t0 = time.time()
+    o.prepare_scene(cfg.scene)
+    t_prep = time.time() - t0
+    k = max(1, args.cpu_iters)
+    t0 = time.time()
+    it, rr, _ = o.solve(max_iter=k, tol=args.tol)
+    t_k = time.time() - t0
+    # the solve runs one preconditioner application before the loop: count it as one iteration
+    per_iter = t_k / (k + 1)
+    est = per_iter * (gpu_iters + 1)
+    log(f"cpu oracle: prepare {t_prep:.1f} s, {k} iterations {t_k:.1f} s, {per_iter*1e3:.1f} ms/iter, "
+        f"extrapolated to {gpu_iters} iterations: {est:.1f} s")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": (f"oracle/msbg_oracle.cpp (C++ restatement of the reference solver, -O3 -march=native, OpenMP "
+                       f"over the blocks of a colour), {k} PCG iterations (+1 initial V-cycle) of the same solve "
+                       f"timed = {t_k:.1f} s on {threads} threads of {model}; extrapolated linearly to the "
+                       f"{gpu_iters} iterations the solve needs (setup {t_prep:.1f} s excluded)")}
+
+
+def main():
+    args = parse()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import msbgx
+    cfg = make_cfg(msbgx, args.config, args.n)
+    r = gpu_run(msbgx, cfg, args, log)
+    ms = sorted(r["times"])[len(r["times"]) // 2]
+    ms_relax, bytes_relax, cells_relax = r["relax"]
+    achieved = bytes_relax / (ms_relax * 1e-3) / 1e9
+    line = {
+        "metric": "time to 1e-6 rel. residual (plain MG-PCG, 1 GPU); smoother HBM GB/s",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sum(r["times"]) / len(r["times"]), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 dots)",
+        "data": "synthetic (seeded scene, uniform [-1,1] MAC velocity divergence)",
+        "config": {
+            "workload": f"{cfg.name} {cfg.n}^3, {cfg.bsx}^3 base blocks, {cfg.levels} levels, FP32 storage",
+            "blocks_per_level": r["levels"],
+            "fluid_blocks_mg0": r["fluid_blocks"],
+            "relax_blocks_mg0": r["relax_blocks"],
+            "iterations": r["iters"][0],
+            "final_rel_residual": r["rels"][0],
+            "tol": args.tol,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_relax<{cfg.bsx}> (one red-black colour pass, 5 local sweeps)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launch_ms": round(ms_relax, 4),
+            "cells_per_launch": cells_relax,
+            "bytes_per_cell": 30,
+        },
+        "other_kernels": {
+            "residual_GBs": round(r["residual"][1] / (r["residual"][0] * 1e-3) / 1e9, 1),
+            "pcg_update_GBs": round(r["update"][1] / (r["update"][0] * 1e-3) / 1e9, 1),
+        },
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)
+        except Exception as e:
+            log(f"cpu baseline failed: {e}")
+            line["cpu_baseline"] = None
+    if args.secondary:
+        c1 = msbgx.config_c1(256)
+        a2 = argparse.Namespace(**vars(args))
+        a2.warmup, a2.steps = 1, 3
+        r1 = gpu_run(msbgx, c1, a2, log)
+        line["secondary"] = {"workload": "c1 256^3 single level 16^3 blocks", "ms": sorted(r1["times"])[1],
+                             "iterations": r1["iters"][0]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

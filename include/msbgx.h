@@ -139,6 +139,19 @@ int msbgx_zero_channel(msbgx_grid *g, int ch, int mg);
 int msbgx_copy_channel(msbgx_grid *g, int ch_src, int ch_dst);
 int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse);
 
+/* Measurement: runs `reps` launches of the kernel `which` on the handle's
+ * stream between HIP events and returns the mean duration per launch in ms
+ * and the algorithmic HBM bytes one launch moves (FP32 storage).
+ *   which 0: one red-black colour pass of the smoother over the largest
+ *            same-size block list at MG0 (30 B per cell: x read+write, b, D,
+ *            W_x, W_y, W_z, flags)
+ *   which 1: residual r = b - A x over the MG0 fluid blocks of the finest
+ *            level (30 B per cell)
+ *   which 2: PCG update x += a d, r -= a q, |r|^2 (24 B per cell)
+ * The channel contents after the call are unspecified (z, q, x, r used). */
+int msbgx_time_kernel(msbgx_grid *g, int which, int reps, double *ms_per_launch, double *bytes_per_launch,
+                      int *cells_per_launch);
+
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
  * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);

@@ -320,6 +320,13 @@ int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse)
   return deviceVCycle(*g, n_pre, n_post, n_coarse);
 }
 
+int msbgx_time_kernel(msbgx_grid *g, int which, int reps, double *ms, double *bytes, int *cells)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (which < 0 || which > 2 || reps < 1 || !ms || !bytes || !cells) { setError("bad argument"); return MSBGX_EINVAL; }
+  return deviceTimeKernel(*g, which, reps, ms, bytes, cells);
+}
+
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)
 {
   const int rc = levelMapFromScene(s, bsx0, n_levels, mode, d0, out);

@@ -124,6 +124,7 @@ int deviceProlongate(Grid &g, int Lf, int ch);
 int deviceZero(Grid &g, int ch, int L);
 int deviceCopy(Grid &g, int src, int dst);
 int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse);
+int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
 
 void setError(const std::string &msg);
 

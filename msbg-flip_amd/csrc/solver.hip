@@ -72,27 +72,26 @@ enum FaceKind { FK_SAME = 0, FK_DOMAIN = 1, FK_COARSER = 2, FK_FINER = 3 };
 /* Stage the block (h*x, zero on all-non-fluid segments) and its six halo
  * faces into LDS (src/msbg3.cpp:564-1044).  Returns per-thread skip mask over
  * z and the face kinds/neighbour offsets used for the +face coefficients. */
-template <int B>
-__device__ __forceinline__ void stageTile(float *tile, uint32_t &skipmask, const LevelArgs &a, int bid, int lev,
-                                          float h, int64_t off, const float *__restrict__ X, int *fkind,
-                                          int64_t *foff)
+/* True when every lane of the caller's SIMD segment (S consecutive lanes)
+ * reports a SOLID/VOID cell: such segments are skipped by the reference's
+ * block kernel (src/msbg3.cpp:589-617, 1201-1209). */
+template <int S>
+__device__ __forceinline__ bool segmentAllNonFluid(bool nonFluid)
 {
-  constexpr int S = Cfg<B>::S, NIN = Cfg<B>::NIN, NF = Cfg<B>::NF;
-  const int tx = threadIdx.x % B, ty = threadIdx.x / B;
-  const int lane = __lane_id();
-  uint32_t sm = 0;
-#pragma unroll 4
-  for (int z = 0; z < B; z++) {
-    const int v = tx + B * ty + B * B * z;
-    const bool nonFluid = (a.flags[off + v] & (CELL_SOLID | CELL_VOID)) != 0;
-    const uint64_t bal = __ballot(nonFluid);
-    const uint64_t segBits = (bal >> (lane & ~(S - 1))) & ((1ull << S) - 1);
-    const bool skip = segBits == ((1ull << S) - 1);
-    sm |= (skip ? 1u : 0u) << z;
-    tile[v] = skip ? 0.0f : X[off + v] * h;
-  }
-  skipmask = sm;
+  const uint64_t bal = __ballot(nonFluid);
+  const uint64_t segBits = (bal >> (__lane_id() & ~(S - 1))) & ((1ull << S) - 1);
+  return segBits == ((1ull << S) - 1);
+}
 
+/* Halo faces of a block into faces[6][B*B] (h*x of same-level neighbours,
+ * mixed-resolution aggregates, 0 at the domain boundary) plus per-face kinds
+ * and neighbour slot offsets (src/msbg3.cpp:665-1044, 46-161). */
+template <int B>
+__device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int bid, int lev, float h, int64_t off,
+                                           const float *__restrict__ X, int *fkind, int64_t *foff)
+{
+  constexpr int NF = Cfg<B>::NF;
+  const int tx = threadIdx.x % B, ty = threadIdx.x / B;
   int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
   const float a13 = (float)(1.0 * (1. / 3.));      // DPMR_GRAD_COEFF_CF (src/msbg.h:85-87)
   const float a23 = (float)(1.0 * 2 * (1. / 3.));  // 2 * DPMR_GRAD_COEFF_CF
@@ -153,10 +152,32 @@ __device__ __forceinline__ void stageTile(float *tile, uint32_t &skipmask, const
         val = (float)(sv * (double)h);
       }
     }
-    tile[NIN + face * NF + u + B * w] = val;
+    faces[face * NF + u + B * w] = val;
     fkind[face] = kind;
     foff[face] = off2;
   }
+}
+
+/* Stage the block interior (h*x, zero on all-non-fluid segments) and its six
+ * halo faces into an LDS tile of B^3 + 6 B^2 floats; skipmask bit z marks the
+ * caller's skipped segment at row z. */
+template <int B>
+__device__ __forceinline__ void stageTile(float *tile, uint32_t &skipmask, const LevelArgs &a, int bid, int lev,
+                                          float h, int64_t off, const float *__restrict__ X, int *fkind,
+                                          int64_t *foff)
+{
+  constexpr int S = Cfg<B>::S, NIN = Cfg<B>::NIN;
+  const int tx = threadIdx.x % B, ty = threadIdx.x / B;
+  uint32_t sm = 0;
+#pragma unroll 4
+  for (int z = 0; z < B; z++) {
+    const int v = tx + B * ty + B * B * z;
+    const bool skip = segmentAllNonFluid<S>((a.flags[off + v] & (CELL_SOLID | CELL_VOID)) != 0);
+    sm |= (skip ? 1u : 0u) << z;
+    tile[v] = skip ? 0.0f : X[off + v] * h;
+  }
+  skipmask = sm;
+  stageFaces<B>(tile + NIN, a, bid, lev, h, off, X, fkind, foff);
 }
 
 /* One cell of the block operator: F0..F6 from the LDS tile, coefficients from
@@ -243,6 +264,208 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
     const int64_t c = off + x + B * y + B * B * z;
     const float Y = D != 0.0f ? fmaf(om * (Bv[c] + Sm), D, omm * F0) : F0;
     X[c] = Y * HR;
+  }
+}
+
+/* Workgroup barrier that orders LDS only: outstanding global loads stay in
+ * flight across it (all cross-thread exchange in these kernels is via LDS). */
+__device__ __forceinline__ void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+/* Column-resident red-black block relaxation (one colour pass, one
+ * workgroup of B*B threads per block).
+ *
+ * Thread (x, y) owns the z-column of its cell row.  The staged h*x values of
+ * the column live in registers, and for B <= 16 so do all seven coefficients
+ * (REGC); for B = 32 the coefficients are re-read from memory each step.  Cell
+ * z sits in register slot (z + o) mod B with o = x/S + y, so inside a loop
+ * unrolled over one period of B hyperplanes the slot a thread touches at
+ * hyperplane hp is the compile-time constant hp mod B.  The four sequential
+ * local sweeps are hyperplane wavefronts over seg + y + z, forward and
+ * reverse as in src/msbg3.cpp:1157-1175; the fifth is the Jacobi pass whose
+ * result /h is stored (src/msbg3.cpp:1341-1349).  Per step each thread
+ * publishes three values to LDS -- old value at z, old value one step ahead,
+ * newest value one step behind -- which is exactly what its x/y neighbours
+ * read at that step; LDS is double-buffered, one barrier per step. */
+template <int B, bool REGC> struct ColRegs;
+template <int B> struct ColRegs<B, true> {
+  float xs[B], cd[B], cb[B], w1[B], w2[B], w3[B], w4[B], w5[B];
+};
+template <int B> struct ColRegs<B, false> {
+  float xs[B];
+};
+
+struct ColCtx {
+  const LevelArgs *a;
+  const float *Bv;
+  float *Xg;
+  const float *faces;
+  int64_t off;
+  int tx, ty, o, fk1, fk3, fk5, mixed0, mixed2, mixed4;
+  int64_t fo1, fo3, fo5;
+  float w6top, om, omm, HR;
+};
+
+template <int B>
+__device__ __forceinline__ void loadCoeffs(const ColCtx &c, int z, float &D, float &Bb, float &w1, float &w2, float &w3,
+                                           float &w4, float &w5, float &w6)
+{
+  const LevelArgs &a = *c.a;
+  const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
+  D = a.D[q];
+  Bb = c.Bv[q];
+  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : a.W0[q];
+  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : a.W1[q];
+  w5 = (z == 0 && c.mixed4) ? 1.0f : a.W2[q];
+  if (c.tx < B - 1) w2 = a.W0[q + 1];
+  else w2 = c.fk1 == FK_SAME ? a.W0[c.fo1 + B * c.ty + B * B * z] : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
+  if (c.ty < B - 1) w4 = a.W1[q + B];
+  else w4 = c.fk3 == FK_SAME ? a.W1[c.fo3 + c.tx + B * B * z] : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
+  w6 = z < B - 1 ? a.W2[q + B * B] : c.w6top;
+}
+
+template <int B, bool REGC, int DIR, bool UPDATE, int I>
+__device__ __forceinline__ void colStep(ColRegs<B, REGC> &r, uint32_t skip, float (*pub)[Cfg<B>::NT], const ColCtx &c,
+                                        int hp)
+{
+  constexpr int S = Cfg<B>::S, NF = Cfg<B>::NF, T = Cfg<B>::T;
+  constexpr int R0 = (T - 1) % B;
+  constexpr int sl = DIR > 0 ? I : ((R0 - I) % B + B) % B;
+  constexpr int slA = ((sl + DIR) % B + B) % B;   // z + d
+  constexpr int slC = ((sl - DIR) % B + B) % B;   // z - d
+  constexpr int slM = ((sl - 1) % B + B) % B;
+  constexpr int slP = ((sl + 1) % B + B) % B;
+  const int tid = threadIdx.x, tx = c.tx, ty = c.ty;
+  const int z = hp - c.o;
+  const bool active = z >= 0 && z < B && !((skip >> sl) & 1u);
+  float D = 0.f, Bb = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f, w6 = 0.f;
+  if (!REGC && active) loadCoeffs<B>(c, z, D, Bb, w1, w2, w3, w4, w5, w6);
+  float *pA = pub[0], *pB = pub[1], *pC = pub[2];
+  pA[tid] = r.xs[sl];
+  pB[tid] = r.xs[slA];
+  pC[tid] = r.xs[slC];
+  ldsBarrier();
+  if (active) {
+    const int lane = tx % S;
+    float F1, F2, F3, F4;
+    if (tx == 0) F1 = c.faces[0 * NF + ty + B * z];
+    else if (lane > 0) F1 = pA[tid - 1];
+    else F1 = DIR > 0 ? pC[tid - 1] : pB[tid - 1];
+    if (tx == B - 1) F2 = c.faces[1 * NF + ty + B * z];
+    else if (lane < S - 1) F2 = pA[tid + 1];
+    else F2 = DIR > 0 ? pB[tid + 1] : pC[tid + 1];
+    if (ty == 0) F3 = c.faces[2 * NF + tx + B * z];
+    else F3 = DIR > 0 ? pC[tid - B] : pB[tid - B];
+    if (ty == B - 1) F4 = c.faces[3 * NF + tx + B * z];
+    else F4 = DIR > 0 ? pB[tid + B] : pC[tid + B];
+    const float F0 = r.xs[sl];
+    const float F5 = z > 0 ? r.xs[slM] : c.faces[4 * NF + tx + B * ty];
+    const float F6 = z < B - 1 ? r.xs[slP] : c.faces[5 * NF + tx + B * ty];
+    if constexpr (REGC) {
+      D = r.cd[sl]; Bb = r.cb[sl]; w1 = r.w1[sl]; w2 = r.w2[sl]; w3 = r.w3[sl]; w4 = r.w4[sl]; w5 = r.w5[sl];
+      w6 = z < B - 1 ? r.w5[slP] : c.w6top;
+    }
+    float Sm = F1 * w1;
+    Sm = fmaf(F2, w2, Sm);
+    Sm = fmaf(F3, w3, Sm);
+    Sm = fmaf(F4, w4, Sm);
+    Sm = fmaf(F5, w5, Sm);
+    Sm = fmaf(F6, w6, Sm);
+    const float Y = D != 0.0f ? fmaf(c.om * (Bb + Sm), D, c.omm * F0) : F0;
+    if (UPDATE) r.xs[sl] = Y;
+    else c.Xg[c.off + tx + B * ty + B * B * z] = Y * c.HR;
+  }
+}
+
+template <int B, bool REGC, int DIR, bool UPDATE, int I>
+struct PeriodUnroll {
+  __device__ __forceinline__ static void run(ColRegs<B, REGC> &r, uint32_t skip, float (*pub)[3][Cfg<B>::NT],
+                                             const ColCtx &c, int base)
+  {
+    constexpr int T = Cfg<B>::T;
+    const int step = base + I;
+    if (step < T) {
+      const int hp = DIR > 0 ? step : T - 1 - step;
+      colStep<B, REGC, DIR, UPDATE, I>(r, skip, pub[I & 1], c, hp);
+    }
+    PeriodUnroll<B, REGC, DIR, UPDATE, I + 1>::run(r, skip, pub, c, base);
+  }
+};
+template <int B, bool REGC, int DIR, bool UPDATE>
+struct PeriodUnroll<B, REGC, DIR, UPDATE, B> {
+  __device__ __forceinline__ static void run(ColRegs<B, REGC> &, uint32_t, float (*)[3][Cfg<B>::NT], const ColCtx &,
+                                             int)
+  {
+  }
+};
+
+/* One local sweep: ceil(T/B) periods of B hyperplanes.  B is even, so the
+ * double-buffer index I & 1 alternates across period boundaries too. */
+template <int B, bool REGC, int DIR, bool UPDATE>
+__device__ __forceinline__ void colSweep(ColRegs<B, REGC> &r, uint32_t skip, float (*pub)[3][Cfg<B>::NT],
+                                         const ColCtx &c)
+{
+  constexpr int T = Cfg<B>::T;
+#pragma unroll 1
+  for (int base = 0; base < T; base += B) PeriodUnroll<B, REGC, DIR, UPDATE, 0>::run(r, skip, pub, c, base);
+}
+
+template <int B>
+__global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int *__restrict__ blocks,
+                                                          float *__restrict__ X, const float *__restrict__ Bv,
+                                                          float omega, int reverse, const int *done)
+{
+  if (*done) return;
+  constexpr bool REGC = B <= 16;
+  constexpr int S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
+  __shared__ float faces[6 * NF];
+  __shared__ float pub[2][3][NT];
+  const int bid = blocks[blockIdx.x];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const float h = (float)(1 << lev);
+  int fkind[6];
+  int64_t foff[6];
+  stageFaces<B>(faces, a, bid, lev, h, off, X, fkind, foff);
+
+  ColCtx c;
+  c.a = &a; c.Bv = Bv; c.Xg = X; c.faces = faces; c.off = off;
+  c.tx = threadIdx.x % B; c.ty = threadIdx.x / B; c.o = c.tx / S + c.ty;
+  c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
+  c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
+  c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
+  c.w6top = fkind[5] == FK_SAME ? a.W2[foff[5] + c.tx + B * c.ty] : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
+
+  ColRegs<B, REGC> r;
+  uint32_t skip = 0;
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    const int z = (sl - c.o + 2 * B) & (B - 1);
+    const int64_t q = off + c.tx + B * c.ty + B * B * z;
+    const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
+    skip |= (sk ? 1u : 0u) << sl;
+    r.xs[sl] = sk ? 0.0f : X[q] * h;
+    if constexpr (REGC) {
+      float w6;
+      loadCoeffs<B>(c, z, r.cd[sl], r.cb[sl], r.w1[sl], r.w2[sl], r.w3[sl], r.w4[sl], r.w5[sl], w6);
+    }
+  }
+  __syncthreads();
+  /* sweep k is reversed iff reverse == (k & 1) (src/msbg3.cpp:1157) */
+  if (reverse) {
+#pragma unroll 1
+    for (int pair = 0; pair < 2; pair++) {
+      colSweep<B, REGC, +1, true>(r, skip, pub, c);
+      colSweep<B, REGC, -1, true>(r, skip, pub, c);
+    }
+    colSweep<B, REGC, +1, false>(r, skip, pub, c);
+  } else {
+#pragma unroll 1
+    for (int pair = 0; pair < 2; pair++) {
+      colSweep<B, REGC, -1, true>(r, skip, pub, c);
+      colSweep<B, REGC, +1, true>(r, skip, pub, c);
+    }
+    colSweep<B, REGC, -1, false>(r, skip, pub, c);
   }
 }
 
@@ -656,8 +879,8 @@ static void launchRelaxB(const Grid &g, int L, const DevList &l, float *X, const
                          hipStream_t s)
 {
   if (l.n == 0) return;
-  hipLaunchKernelGGL(k_relax<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, g.omega, reverse, 5,
-                     done);
+  hipLaunchKernelGGL(k_relax_col<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, g.omega,
+                     reverse, done);
 }
 
 static void launchRelaxLev(const Grid &g, int L, int lev, const DevList &l, float *X, const float *Bv, int reverse,
@@ -1022,6 +1245,58 @@ int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse)
   vcycleLaunch(g, 0, nPre, nPost, nCoarse, d.zeroFlag, d.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, int *cells)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *nd = d.zeroFlag;
+  /* pick the largest same-size block list of one colour at MG0 */
+  int bestC = 0, bestL = 0, bestN = -1;
+  for (int c = 0; c < 2; c++)
+    for (int l = 0; l < g.nLevels; l++) {
+      const int b = g.bsxL(l);
+      const int n = m0.dRelax[c][l].n * b * b * b;
+      if (n > bestN) { bestN = n; bestC = c; bestL = l; }
+    }
+  auto launch = [&]() {
+    if (which == 0) {
+      launchRelaxLev(g, 0, bestL, m0.dRelax[bestC][bestL], m0.d_ch[C_Z], m0.d_ch[C_R], 0, nd, d.stream);
+    } else if (which == 1) {
+      const DevList &l = m0.dAct[0];
+      switch (g.bsx0) {
+        case 4: launchMultiplyB<4>(g, 0, l, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], 1, nullptr, 0, nd, d.stream); break;
+        case 8: launchMultiplyB<8>(g, 0, l, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], 1, nullptr, 0, nd, d.stream); break;
+        case 16: launchMultiplyB<16>(g, 0, l, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], 1, nullptr, 0, nd, d.stream); break;
+        case 32: launchMultiplyB<32>(g, 0, l, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], 1, nullptr, 0, nd, d.stream); break;
+      }
+    } else {
+      const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+      hipLaunchKernelGGL(k_update_xr, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                         m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
+    }
+  };
+  PcgState init{};
+  init.alpha = 1e-3f;
+  HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
+  launch();  // warm-up
+  HIPCHK(hipEventRecord(d.ev0, d.stream));
+  for (int i = 0; i < reps; i++) launch();
+  HIPCHK(hipEventRecord(d.ev1, d.stream));
+  HIPCHK(hipEventSynchronize(d.ev1));
+  HIPCHK(hipGetLastError());
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, d.ev0, d.ev1));
+  *ms = (double)t / reps;
+  int64_t c;
+  double bpc;
+  if (which == 0) { c = bestN; bpc = 30.0; }
+  else if (which == 1) { c = (int64_t)m0.dAct[0].n * g.bsx0 * g.bsx0 * g.bsx0; bpc = 30.0; }
+  else { c = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0; bpc = 24.0; }
+  *cells = (int)c;
+  *bytes = bpc * (double)c;
   return MSBGX_OK;
 }
 

@@ -147,6 +147,8 @@ def _load():
         "msbgx_zero_channel": (C.c_int, [P, C.c_int, C.c_int]),
         "msbgx_copy_channel": (C.c_int, [P, C.c_int, C.c_int]),
         "msbgx_vcycle": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+        "msbgx_time_kernel": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int)]),
         "msbgx_level_map_from_scene": (C.c_int, [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, P]),
     }
     for name, (res, args) in sig.items():
@@ -165,7 +167,7 @@ EXPORTED = [
     "msbgx_solve_pressure", "msbgx_residual_history", "msbgx_last_solve_ms", "msbgx_get_channel_block",
     "msbgx_set_channel_block", "msbgx_block_info", "msbgx_relax_list", "msbgx_omega", "msbgx_relax",
     "msbgx_multiply", "msbgx_downsample", "msbgx_prolongate", "msbgx_zero_channel", "msbgx_copy_channel",
-    "msbgx_vcycle", "msbgx_level_map_from_scene",
+    "msbgx_vcycle", "msbgx_time_kernel", "msbgx_level_map_from_scene",
 ]
 
 
@@ -303,6 +305,12 @@ class MultiresSparseGrid:
 
     def copy_channel(self, src, dst):
         _ck(lib.msbgx_copy_channel(self._h, src, dst), "copy_channel")
+
+    def time_kernel(self, which, reps=20):
+        """(ms per launch, algorithmic bytes per launch, cells per launch)"""
+        ms, by, ce = C.c_double(), C.c_double(), C.c_int()
+        _ck(lib.msbgx_time_kernel(self._h, which, reps, C.byref(ms), C.byref(by), C.byref(ce)), "time_kernel")
+        return ms.value, by.value, ce.value
 
     def vcycle(self, n_pre=3, n_post=3, n_coarse=24):
         _ck(lib.msbgx_vcycle(self._h, n_pre, n_post, n_coarse), "vcycle")

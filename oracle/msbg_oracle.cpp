@@ -252,13 +252,22 @@ struct Oracle {
       }
   }
 
-  /* src/msbg3.cpp:2472-2583 */
-  void prepareLevel0(const Callbacks &cb)
+  /* src/msbg3.cpp:2472-2583.  Blocks are allocated serially, then filled
+   * block-parallel when the callbacks are pure functions (scene / dense). */
+  void prepareLevel0(const Callbacks &cb, bool parallel)
   {
+    for (int bid = 0; bid < nBlocks; bid++) {
+      const int l = binfo[0][bid].level;
+      flags[l][0].alloc(bid);
+      for (int dir = 0; dir < 3; dir++) fch[CH_W0 + dir][l][0].alloc(bid);
+      if (l == 0) fch[CH_B][0][0].alloc(bid);
+      fch[CH_X][l][0].alloc(bid);
+    }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nThreads) if (parallel)
     for (int bid = 0; bid < nBlocks; bid++) {
       BlockInfo &bi = binfo[0][bid];
       const int l = bi.level, b = bsxL(l);
-      CellFlags *dst = flags[l][0].alloc(bid);
+      CellFlags *dst = flags[l][0].get(bid);
       int bx, by, bz;
       blockCoords(bid, bx, by, bz);
       bool hasLiquid = false;
@@ -274,20 +283,23 @@ struct Oracle {
       if (hasLiquid) bi.flags &= ~BLK_NO_FLUID; else bi.flags |= BLK_NO_FLUID;
       bi.flags &= ~BLK_FIXED;
     }
-    for (int dir = 0; dir < 3; dir++)
+    for (int dir = 0; dir < 3; dir++) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nThreads) if (parallel)
       for (int bid = 0; bid < nBlocks; bid++) {
         const int l = binfo[0][bid].level, b = bsxL(l);
-        float *w = fch[CH_W0 + dir][l][0].alloc(bid);
+        float *w = fch[CH_W0 + dir][l][0].get(bid);
         int bx, by, bz;
         blockCoords(bid, bx, by, bz);
         for (int vz = 0, vid = 0; vz < b; vz++)
           for (int vy = 0; vy < b; vy++)
             for (int vx = 0; vx < b; vx++, vid++) w[vid] = cb.faceCoeff(cb.user, dir, bx * b + vx, by * b + vy, bz * b + vz);
       }
+    }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nThreads) if (parallel)
     for (int bid = 0; bid < nBlocks; bid++) {
       if (binfo[0][bid].level != 0) continue;
       const int b = bsx0;
-      float *r = fch[CH_B][0][0].alloc(bid);
+      float *r = fch[CH_B][0][0].get(bid);
       int bx, by, bz;
       blockCoords(bid, bx, by, bz);
       for (int vz = 0, vid = 0; vz < b; vz++)
@@ -295,7 +307,7 @@ struct Oracle {
           for (int vx = 0; vx < b; vx++, vid++) r[vid] = cb.rhs(cb.user, bx * b + vx, by * b + vy, bz * b + vz);
     }
     for (int bid = 0; bid < nBlocks; bid++) {  // setChannel(0, CH_PRESSURE)
-      float *x = fch[CH_X][binfo[0][bid].level][0].alloc(bid);
+      float *x = fch[CH_X][binfo[0][bid].level][0].get(bid);
       std::fill(x, x + fch[CH_X][binfo[0][bid].level][0].nvox(), 0.0f);
     }
   }
@@ -306,11 +318,14 @@ struct Oracle {
     for (int level = L; level < nLevels; level++) {
       Grid<CellFlags> &sgF = flags[level][L];
       Grid<float> &sgD = fch[CH_DIAG][level][L];
+      for (int bid = 0; bid < nBlocks; bid++)
+        if (binfo[L][bid].level == level && (binfo[L][bid].flags & BLK_EXISTS)) sgD.alloc(bid);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nThreads)
       for (int bid = 0; bid < nBlocks; bid++) {
         BlockInfo &bi = binfo[L][bid];
         if (bi.level != level || !(bi.flags & BLK_EXISTS)) continue;
         CellFlags *f = sgF.get(bid);
-        float *d = sgD.alloc(bid);
+        float *d = sgD.get(bid);
         if (!f) continue;
         const int b = bsxL(level);
         bi.flags &= ~BLK_CUTS_SOLID;
@@ -328,6 +343,7 @@ struct Oracle {
     if (L == 0) {
       for (int level = 0; level < nLevels; level++) {
         Grid<float> &U = fch[CH_W0][level][0], &V = fch[CH_W1][level][0], &W = fch[CH_W2][level][0];
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nThreads)
         for (int bid = 0; bid < nBlocks; bid++) {
           BlockInfo &bi = binfo[0][bid];
           if (bi.level != level || !(bi.flags & BLK_EXISTS)) continue;
@@ -354,6 +370,7 @@ struct Oracle {
       }
     } else {
       for (int level = L; level < nLevels; level++) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nThreads)
         for (int bid = 0; bid < nBlocks; bid++) {
           BlockInfo &bi = binfo[L][bid];
           if (bi.level != level || !(bi.flags & BLK_EXISTS)) continue;
@@ -382,10 +399,13 @@ struct Oracle {
       if (level < Lc) continue;   // no grid below the MG level
       Grid<CellFlags> &C = flags[level][Lc];
       Grid<CellFlags> &F = flags[level][Lf];
+      for (int bid = 0; bid < nBlocks; bid++)
+        if (binfo[Lc][bid].level == level) C.alloc(bid);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nThreads)
       for (int bid = 0; bid < nBlocks; bid++) {
         BlockInfo &bi = binfo[Lc][bid];
         if (bi.level != level) continue;
-        CellFlags *dst = C.alloc(bid);
+        CellFlags *dst = C.get(bid);
         int bx, by, bz;
         blockCoords(bid, bx, by, bz);
         const int b = bsxL(level);
@@ -415,9 +435,12 @@ struct Oracle {
       for (int level = Lc; level < nLevels; level++) {
         Grid<float> &C = fch[CH_W0 + dir][level][Lc];
         Grid<float> &F = fch[CH_W0 + dir][level][Lf];
+        for (int bid = 0; bid < nBlocks; bid++)
+          if (binfo[Lc][bid].level == level) C.alloc(bid);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nThreads)
         for (int bid = 0; bid < nBlocks; bid++) {
           if (binfo[Lc][bid].level != level) continue;
-          float *w = C.alloc(bid);
+          float *w = C.get(bid);
           int bx, by, bz;
           blockCoords(bid, bx, by, bz);
           const int b = bsxL(level);
@@ -442,10 +465,10 @@ struct Oracle {
     buildRelaxationBlocks(Lc);
   }
 
-  void prepare(const Callbacks &cb)
+  void prepare(const Callbacks &cb, bool parallel)
   {
     allocPressureChannels();
-    prepareLevel0(cb);
+    prepareLevel0(cb, parallel);
     computeDiagonal(0);
     buildRelaxationBlocks(0);
     for (int L = 1; L < nLevels; L++) restrictPressureMeta(L);
@@ -1017,14 +1040,14 @@ void orc_level_map_from_scene(const msbgx_scene *s, int bsx0, int nLevels, int m
 void orc_prepare_scene(void *h, const msbgx_scene *s)
 {
   Callbacks cb{sceneCellType, sceneFaceCoeff, sceneRhs, (void *)s};
-  ((Oracle *)h)->prepare(cb);
+  ((Oracle *)h)->prepare(cb, true);
 }
 void orc_prepare_dense(void *h, int nx, int ny, int nz, const uint16_t *flags, const float *wu, const float *wv,
                        const float *ww, const float *rhs)
 {
   DenseInput d{nx, ny, nz, flags, {wu, wv, ww}, rhs};
   Callbacks cb{denseCellType, denseFaceCoeff, denseRhs, &d};
-  ((Oracle *)h)->prepare(cb);
+  ((Oracle *)h)->prepare(cb, true);
 }
 float orc_omega(void *h) { return ((Oracle *)h)->omega; }
 

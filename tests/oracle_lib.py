@@ -8,7 +8,7 @@ import numpy as np
 from msbgx import Scene
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_PATH = os.path.join(ROOT, "oracle", "libmsbg_oracle.so")
+ORACLE_PATH = os.environ.get("MSBG_ORACLE_SO") or os.path.join(ROOT, "oracle", "libmsbg_oracle.so")
 
 _lib = C.CDLL(ORACLE_PATH)
 P = C.c_void_p
