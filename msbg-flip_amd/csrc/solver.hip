@@ -1075,7 +1075,7 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
     if (useGraph && n == chunk) {
       if (!d.graph) {
         hipGraph_t graph;
-        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeRelaxed));
         for (int i = 0; i < chunk; i++) pcgIteration(g, p);
         HIPCHK(hipStreamEndCapture(d.stream, &graph));
         HIPCHK(hipGraphInstantiate(&d.graph, graph, nullptr, nullptr, 0));
