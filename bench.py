@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=3, help="PCG iterations in the CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP16/BF16 precision-form runs")
+    ap.add_argument("--no-fp64-oracle", action="store_true", help="skip the CPU FP64 reference solve")
     ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
     ap.add_argument("--no-graph", action="store_true", help="launch PCG iterations eagerly (no hipGraph)")
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -99,6 +101,54 @@ def pmc_calibration(msbgx, cfg, log):
     copy_cells = int(sum((cfg.bsx >> int(l)) ** 3 for l, f in zip(lv, fl) if not f & msbgx.BLK_NO_FLUID))
     log(f"pmc calibration: copy of {copy_cells} cells ({copy_cells * 4} B read, {copy_cells * 4} B written); "
         f"smoother pass: {cells} cells, {by:.0f} algorithmic bytes, {ms:.4f} ms")
+
+
+def precision_runs(msbgx, g, cfg, args, log):
+    """Precision form: FP16/BF16 smoother operands (FP32 compute, FP32 Krylov
+    operator, FP64 dots) -- time to 1e-6 and smoother GB/s -- and FP64
+    iterative refinement to 1e-10 for every variant, with the error against
+    the CPU FP64 solution of the same level-0 system."""
+    out = {}
+    x_ref = None
+    if not args.no_fp64_oracle and not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        o = oracle_lib.build_oracle(cfg, threads=min(args.cpu_threads, os.cpu_count() or 1))
+        o.prepare_scene(cfg.scene)
+        t0 = time.time()
+        it, rel, x_ref = o.solve_fp64(tol=1e-12, max_iter=20000)
+        log(f"fp64 oracle: {it} Jacobi-CG iterations, true rel residual {rel:.2e}, {time.time() - t0:.1f} s")
+        den = sum(float((v ** 2).sum()) for v in x_ref.values()) ** 0.5
+    for name, prec, bpc in (("fp32", msbgx.PREC_FP32, 30), ("fp16", msbgx.PREC_FP16, 22),
+                            ("bf16", msbgx.PREC_BF16, 22)):
+        g.set_precision(prec)
+        times = []
+        for i in range(2):
+            g.zero_channel(msbgx.CH_PRESSURE, 0)
+            it, rr = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+            times.append(g.last_solve_ms())
+        ms_k, _, cells = g.time_kernel(0, reps=20)
+        t0 = time.time()
+        outer, inner, rel64, xb = g.solve_pressure_refined(tol=1e-10, max_outer=30, inner_tol=1e-5,
+                                                          want_x=x_ref is not None)
+        t_ref = (time.time() - t0) * 1e3
+        rec = {"solve_ms_to_1e-6": round(min(times), 3), "iterations": it, "rel_residual": rr,
+               "smoother_GBs": round(bpc * cells / (ms_k * 1e-3) / 1e9, 1), "smoother_bytes_per_cell": bpc,
+               "refined_to_1e-10": {"outer": outer, "inner_iterations": inner, "fp64_rel_residual": rel64,
+                                    "wall_ms": round(t_ref, 1)}}
+        if x_ref is not None:
+            num = sum(float(((xb.get(b, 0.0) - v) ** 2).sum()) for b, v in x_ref.items()) ** 0.5
+            g.zero_channel(msbgx.CH_PRESSURE, 0)
+            g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+            num1 = 0.0
+            for b, v in x_ref.items():
+                a = g.get_block(msbgx.CH_PRESSURE, 0, b, cfg.bsx)
+                num1 += float(((a.astype(float) - v) ** 2).sum())
+            rec["error_vs_fp64_oracle"] = {"plain_to_1e-6": num1 ** 0.5 / den, "refined_to_1e-10": num / den}
+        log(f"precision {name}: {rec}")
+        out[name] = rec
+    g.set_precision(msbgx.PREC_FP32)
+    return out
 
 
 def read_traffic(kernel, workload):
@@ -218,6 +268,8 @@ def main():
             "pcg_update_GBs": round(r["update"][1] / (r["update"][0] * 1e-3) / 1e9, 1),
         },
     }
+    if not args.no_precision:
+        line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)

@@ -62,6 +62,14 @@ typedef struct msbgx_solve_params {
   float tol;        /* 5e-4 */
 } msbgx_solve_params;
 
+/* Storage precision of the smoother's operands (precision form, a build
+ * addition without reference counterpart): the block Gauss-Seidel smoother
+ * reads 16-bit copies of the face coefficients and the inverse diagonal and
+ * computes in FP32; the Krylov operator q = A d keeps the FP32 coefficients. */
+#define MSBGX_PREC_FP32 0
+#define MSBGX_PREC_FP16 1
+#define MSBGX_PREC_BF16 2
+
 /* Solver option bits for msbgx_set_options(). */
 #define MSBGX_OPT_LITERAL_VCYCLE 0x1u  /* run residual/restriction/prolongation even when every
                                           coarser level has no relaxation block             */
@@ -106,6 +114,24 @@ int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene);
  * Returns the iteration count in *iters (max_iter when not converged) and
  * |r|/|b| in *rel_residual. */
 int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters, float *rel_residual);
+/* Precision form (no reference counterpart): selects the smoother operand
+ * storage (MSBGX_PREC_*); takes effect immediately, also after prepare. */
+int msbgx_set_precision(msbgx_grid *g, int prec);
+
+/* Mixed-precision iterative refinement around the MG-PCG (no reference
+ * counterpart).  x64 = 0; repeat: r = b - A x64 in FP64 over the level-0
+ * unknowns (coarse-block unknowns held at 0, as in the reference's PCG);
+ * stop when |r| <= tol |b|; solve A e = r with the MG-PCG to inner->tol;
+ * x64 += e.  The pressure channel receives (float) x64; the FP64 solution
+ * of the level-0 blocks is copied to x64_out (in PCG-block order, see
+ * msbgx_level0_blocks) when non-NULL.  Returns outer iterations, the summed
+ * inner iterations and the final FP64 relative residual. */
+int msbgx_solve_pressure_refined(msbgx_grid *g, const msbgx_solve_params *inner, double tol, int max_outer,
+                                 int *outer_iters, int *inner_iters, double *rel_residual, double *x64_out);
+/* The level-0 blocks the PCG vector operations cover, in their pool order
+ * (the order of msbgx_solve_pressure_refined's x64_out); returns the count. */
+int msbgx_level0_blocks(const msbgx_grid *g, int *out, int cap);
+
 /* Relative residual after each PCG iteration of the last solve (up to cap). */
 int msbgx_residual_history(const msbgx_grid *g, double *out, int cap);
 /* Device time of the last solve in milliseconds (HIP events around the PCG). */

@@ -223,6 +223,37 @@ int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters,
   return deviceSolve(*g, *p, iters, rel);
 }
 
+int msbgx_set_precision(msbgx_grid *g, int prec)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (prec != MSBGX_PREC_FP32 && prec != MSBGX_PREC_FP16 && prec != MSBGX_PREC_BF16) {
+    setError("unknown precision");
+    return MSBGX_EINVAL;
+  }
+  g->prec = prec;
+  if (g->prepared) return deviceSetPrecision(*g, prec);
+  return MSBGX_OK;
+}
+
+int msbgx_solve_pressure_refined(msbgx_grid *g, const msbgx_solve_params *p, double tol, int max_outer,
+                                 int *outer_iters, int *inner_iters, double *rel, double *x64_out)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (!p || !outer_iters || !inner_iters || !rel || !(tol > 0.0) || max_outer < 1) {
+    setError("bad argument");
+    return MSBGX_EINVAL;
+  }
+  if (!g->prepared) { setError("solve before prepare"); return MSBGX_ESTATE; }
+  return deviceSolveRefined(*g, *p, tol, max_outer, outer_iters, inner_iters, rel, x64_out);
+}
+
+int msbgx_level0_blocks(const msbgx_grid *g, int *out, int cap)
+{
+  if (!g) return 0;
+  for (int i = 0; i < (int)g->active0.size() && i < cap; i++) out[i] = g->active0[i];
+  return (int)g->active0.size();
+}
+
 int msbgx_residual_history(const msbgx_grid *g, double *out, int cap)
 {
   if (!g || !out) return 0;

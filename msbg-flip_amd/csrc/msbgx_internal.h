@@ -57,6 +57,7 @@ struct MgLevel {
   uint16_t *d_flags = nullptr;
   float *d_W[3] = {nullptr, nullptr, nullptr};
   float *d_D = nullptr;
+  uint16_t *d_H[4] = {nullptr, nullptr, nullptr, nullptr};  // 16-bit W_x, W_y, W_z, D (precision form)
   float *d_ch[N_VEC] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int *d_lists = nullptr;
   DevList dAct[kMaxLevels];
@@ -74,6 +75,8 @@ struct Grid {
   bool prepared = false;
   bool aliasEstablished[kMaxLevels] = {false, false, false, false, false};
   int nActive0 = 0;             // blocks at the head of the MG0 pool touched by PCG vector ops
+  std::vector<int> active0;     // those blocks, in pool order
+  int prec = MSBGX_PREC_FP32;   // smoother operand storage
   /* device state */
   void *dev = nullptr;          // solver state (device buffers, stream, graph)
   std::vector<double> history;
@@ -125,6 +128,9 @@ int deviceZero(Grid &g, int ch, int L);
 int deviceCopy(Grid &g, int src, int dst);
 int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse);
 int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
+int deviceSetPrecision(Grid &g, int prec);
+int deviceSolveRefined(Grid &g, const msbgx_solve_params &p, double tol, int maxOuter, int *outer, int *inner,
+                       double *rel, double *x64out);
 
 void setError(const std::string &msg);
 

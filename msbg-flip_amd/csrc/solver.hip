@@ -50,7 +50,25 @@ struct LevelArgs {
   const float *W0, *W1, *W2, *D;
   int nbx, nby, nbz, bsx0;
   float bcW[3];  // face coefficient at the +side domain boundary per direction
+  /* precision form: 16-bit copies of W_x, W_y, W_z, D read by the smoother */
+  const uint16_t *H[4];
+  int prec;      // MSBGX_PREC_FP32 / FP16 / BF16
 };
+
+__device__ __forceinline__ float fromBits16(uint16_t v, int prec)
+{
+  if (prec == MSBGX_PREC_FP16) return (float)__builtin_bit_cast(_Float16, v);
+  return __uint_as_float((uint32_t)v << 16);
+}
+/* smoother coefficient k (0..2 = W_x, W_y, W_z; 3 = D) at cell q */
+__device__ __forceinline__ float smCoef(const LevelArgs &a, int k, int64_t q)
+{
+  if (a.prec == MSBGX_PREC_FP32) {
+    const float *p = k == 0 ? a.W0 : (k == 1 ? a.W1 : (k == 2 ? a.W2 : a.D));
+    return p[q];
+  }
+  return fromBits16(a.H[k][q], a.prec);
+}
 
 struct PcgState {
   double bnorm2, rho, rhoNew, dq, rr, tolSq;
@@ -311,16 +329,16 @@ __device__ __forceinline__ void loadCoeffs(const ColCtx &c, int z, float &D, flo
 {
   const LevelArgs &a = *c.a;
   const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
-  D = a.D[q];
+  D = smCoef(a, 3, q);
   Bb = c.Bv[q];
-  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : a.W0[q];
-  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : a.W1[q];
-  w5 = (z == 0 && c.mixed4) ? 1.0f : a.W2[q];
-  if (c.tx < B - 1) w2 = a.W0[q + 1];
-  else w2 = c.fk1 == FK_SAME ? a.W0[c.fo1 + B * c.ty + B * B * z] : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
-  if (c.ty < B - 1) w4 = a.W1[q + B];
-  else w4 = c.fk3 == FK_SAME ? a.W1[c.fo3 + c.tx + B * B * z] : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
-  w6 = z < B - 1 ? a.W2[q + B * B] : c.w6top;
+  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : smCoef(a, 0, q);
+  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : smCoef(a, 1, q);
+  w5 = (z == 0 && c.mixed4) ? 1.0f : smCoef(a, 2, q);
+  if (c.tx < B - 1) w2 = smCoef(a, 0, q + 1);
+  else w2 = c.fk1 == FK_SAME ? smCoef(a, 0, c.fo1 + B * c.ty + B * B * z) : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
+  if (c.ty < B - 1) w4 = smCoef(a, 1, q + B);
+  else w4 = c.fk3 == FK_SAME ? smCoef(a, 1, c.fo3 + c.tx + B * B * z) : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
+  w6 = z < B - 1 ? smCoef(a, 2, q + B * B) : c.w6top;
 }
 
 template <int B, bool REGC, int DIR, bool UPDATE, int I>
@@ -433,7 +451,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
   c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
   c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
-  c.w6top = fkind[5] == FK_SAME ? a.W2[foff[5] + c.tx + B * c.ty] : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
   c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
 
   ColRegs<B, REGC> r;
@@ -740,6 +758,12 @@ struct Dev {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
   int graphChunk = 0;
+  /* iterative refinement (precision form) */
+  double *x64 = nullptr;
+  float *b0 = nullptr;
+  int *dActive0 = nullptr;
+  int64_t refCap = 0;
+  bool b0Valid = false;
 };
 
 static Dev &dev(Grid &g) { return *(Dev *)g.dev; }
@@ -757,6 +781,7 @@ static void freeLevel(MgLevel &m)
   auto f = [](void *p) { if (p) (void)hipFree(p); };
   f(m.d_level); f(m.d_off); f(m.d_flags); f(m.d_D); f(m.d_lists);
   for (int d = 0; d < 3; d++) { f(m.d_W[d]); m.d_W[d] = nullptr; }
+  for (int k = 0; k < 4; k++) { f(m.d_H[k]); m.d_H[k] = nullptr; }
   for (int c = 0; c < N_VEC; c++) { f(m.d_ch[c]); m.d_ch[c] = nullptr; }
   m.d_level = nullptr; m.d_off = nullptr; m.d_flags = nullptr; m.d_D = nullptr; m.d_lists = nullptr;
 }
@@ -771,6 +796,9 @@ void deviceFree(Grid &g)
     if (d.partials) (void)hipFree(d.partials);
     if (d.hist) (void)hipFree(d.hist);
     if (d.zeroFlag) (void)hipFree(d.zeroFlag);
+    if (d.x64) (void)hipFree(d.x64);
+    if (d.b0) (void)hipFree(d.b0);
+    if (d.dActive0) (void)hipFree(d.dActive0);
     if (d.ev0) (void)hipEventDestroy(d.ev0);
     if (d.ev1) (void)hipEventDestroy(d.ev1);
     if (d.stream) (void)hipStreamDestroy(d.stream);
@@ -787,6 +815,8 @@ static LevelArgs levelArgs(const Grid &g, int L)
   a.W0 = m.d_W[0]; a.W1 = m.d_W[1]; a.W2 = m.d_W[2]; a.D = m.d_D;
   a.nbx = g.nbx; a.nby = g.nby; a.nbz = g.nbz; a.bsx0 = g.bsx0;
   for (int d = 0; d < 3; d++) a.bcW[d] = g.domainBc[d][1] != MSBGX_DBC_OPEN ? 0.0f : 1.0f;
+  for (int k = 0; k < 4; k++) a.H[k] = m.d_H[k];
+  a.prec = (g.prec != MSBGX_PREC_FP32 && m.d_H[0]) ? g.prec : MSBGX_PREC_FP32;
   return a;
 }
 
@@ -858,6 +888,13 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
     for (int c = 0; c < 2; c++)
       for (int l = 0; l < kMaxLevels; l++, si++) m.dRelax[c][l] = DevList{m.d_lists + spans[si].first, spans[si].second};
   }
+  d.b0Valid = false;
+  if (d.dActive0) { (void)hipFree(d.dActive0); d.dActive0 = nullptr; }
+  if (int rc = dalloc(&d.dActive0, g.active0.size())) return rc;
+  if (!g.active0.empty())
+    HIPCHK(hipMemcpy(d.dActive0, g.active0.data(), g.active0.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (g.prec != MSBGX_PREC_FP32)
+    if (int rc = deviceSetPrecision(g, g.prec)) return rc;
   MgLevel &m0 = g.mg[0];
   if (scene) {
     /* level-0 blocks holding liquid are the head of the pool (act[0]) */
@@ -1297,6 +1334,170 @@ int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, in
   else { c = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0; bpc = 24.0; }
   *cells = (int)c;
   *bytes = bpc * (double)c;
+  return MSBGX_OK;
+}
+
+/* -------------------------------------------------------- precision form */
+
+/* Round-to-nearest-even conversion of FP32 coefficients to FP16 or BF16 bits. */
+__global__ void k_to16(const float *__restrict__ src, uint16_t *__restrict__ dst, int64_t n, int prec)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = src[i];
+    uint16_t o;
+    if (prec == MSBGX_PREC_FP16) {
+      o = __builtin_bit_cast(uint16_t, (_Float16)v);   // IEEE round-to-nearest-even
+    } else {
+      uint32_t b = __float_as_uint(v);
+      if ((b & 0x7fffffffu) > 0x7f800000u) o = (uint16_t)((b >> 16) | 0x40u);   // NaN stays NaN
+      else o = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+    }
+    dst[i] = o;
+  }
+}
+
+int deviceSetPrecision(Grid &g, int prec)
+{
+  if (!g.dev) return MSBGX_OK;
+  Dev &d = dev(g);
+  for (int L = 0; L < g.nLevels; L++) {
+    MgLevel &m = g.mg[L];
+    if (prec == MSBGX_PREC_FP32) continue;
+    const float *src[4] = {m.d_W[0], m.d_W[1], m.d_W[2], m.d_D};
+    for (int k = 0; k < 4; k++) {
+      if (!m.d_H[k])
+        if (int rc = dalloc(&m.d_H[k], (size_t)m.ncells)) return rc;
+      hipLaunchKernelGGL(k_to16, dim3(1024), dim3(256), 0, d.stream, src[k], m.d_H[k], m.ncells, prec);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }   // kernel args changed
+  return MSBGX_OK;
+}
+
+/* r = b0 - A x64 in FP64 over the level-0 unknowns (coarse-block and
+ * non-PCG blocks hold 0), written as float into the right-hand-side channel,
+ * with FP64 partials of |r|^2.  A x = (1/D) x - sum W x_nb over same-level
+ * face neighbours (src/msbg3.cpp:1282-1291 evaluated in FP64). */
+__global__ void __launch_bounds__(256) k_residual64(LevelArgs a, const int *__restrict__ blocks, int64_t n0,
+                                                   const double *__restrict__ x64, const float *__restrict__ b0,
+                                                   float *__restrict__ rout, double *partials)
+{
+  __shared__ double red[4];
+  const int bid = blocks[blockIdx.x];
+  const int B = a.bsx0, B2 = B * B, n = B2 * B;
+  const int64_t off = a.off[bid];
+  const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  auto xAt = [&](int nb, int v) -> double {
+    const int64_t o = a.off[nb];
+    return (a.level[nb] == 0 && o < n0) ? x64[o + v] : 0.0;
+  };
+  double acc = 0.0;
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    const int vx = v % B, vy = (v / B) % B, vz = v / B2;
+    const int64_t c = off + v;
+    const float D = a.D[c];
+    double y = 0.0;
+    if (D != 0.0f) {
+      y = x64[c] / (double)D;
+      /* -x, +x, -y, +y, -z, +z */
+      double s = 0.0;
+      if (vx > 0) s += (double)a.W0[c] * x64[c - 1];
+      else if (bx > 0) s += (double)a.W0[c] * xAt(bid - 1, v + B - 1);
+      if (vx < B - 1) s += (double)a.W0[c + 1] * x64[c + 1];
+      else if (bx < a.nbx - 1 && a.level[bid + 1] == 0) s += (double)a.W0[a.off[bid + 1] + v - (B - 1)] * xAt(bid + 1, v - (B - 1));
+      if (vy > 0) s += (double)a.W1[c] * x64[c - B];
+      else if (by > 0) s += (double)a.W1[c] * xAt(bid - a.nbx, v + B2 - B);
+      if (vy < B - 1) s += (double)a.W1[c + B] * x64[c + B];
+      else if (by < a.nby - 1 && a.level[bid + a.nbx] == 0) s += (double)a.W1[a.off[bid + a.nbx] + v - (B2 - B)] * xAt(bid + a.nbx, v - (B2 - B));
+      const int nxy = a.nbx * a.nby;
+      if (vz > 0) s += (double)a.W2[c] * x64[c - B2];
+      else if (bz > 0) s += (double)a.W2[c] * xAt(bid - nxy, v + n - B2);
+      if (vz < B - 1) s += (double)a.W2[c + B2] * x64[c + B2];
+      else if (bz < a.nbz - 1 && a.level[bid + nxy] == 0) s += (double)a.W2[a.off[bid + nxy] + v - (n - B2)] * xAt(bid + nxy, v - (n - B2));
+      y -= s;
+    }
+    const double r = (double)b0[c] - y;
+    rout[c] = (float)r;
+    acc += r * r;
+  }
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+/* x64 += x (and x = (float) x64 when store) over [0, n) */
+__global__ void k_accum64(double *__restrict__ x64, float *__restrict__ x, int64_t n, int store)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (store) x[i] = (float)x64[i];
+    else x64[i] += (double)x[i];
+  }
+}
+
+int deviceSolveRefined(Grid &g, const msbgx_solve_params &p, double tol, int maxOuter, int *outer, int *inner,
+                       double *rel, double *x64out)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+  if (d.refCap < n0) {
+    if (d.x64) (void)hipFree(d.x64);
+    if (d.b0) (void)hipFree(d.b0);
+    d.x64 = nullptr; d.b0 = nullptr;
+    if (int rc = dalloc(&d.x64, n0)) return rc;
+    if (int rc = dalloc(&d.b0, n0)) return rc;
+    d.refCap = n0;
+    d.b0Valid = false;
+  }
+  if (!d.b0Valid) {
+    HIPCHK(hipMemcpyAsync(d.b0, m0.d_ch[C_B], n0 * sizeof(float), hipMemcpyDeviceToDevice, d.stream));
+    d.b0Valid = true;
+  }
+  HIPCHK(hipMemsetAsync(d.x64, 0, n0 * sizeof(double), d.stream));
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, d.b0, d.b0, n0, d.partials, d.zeroFlag);
+  if (d.histCap < 2) {
+    if (d.hist) (void)hipFree(d.hist);
+    d.hist = nullptr;
+    if (int rc = dalloc(&d.hist, 2)) return rc;
+    d.histCap = 2;
+  }
+  finalize(d, kVecBlocks, FIN_FINAL, d.hist);
+  double bb = 0.0;
+  HIPCHK(hipMemcpyAsync(&bb, d.hist, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  const LevelArgs a = levelArgs(g, 0);
+  int nOuter = 0, nInner = 0;
+  double relNow = 0.0;
+  std::vector<double> outerHist;
+  for (;;) {
+    if (g.nActive0 > 0)
+      hipLaunchKernelGGL(k_residual64, dim3(g.nActive0), dim3(256), 0, d.stream, a, d.dActive0, n0, d.x64, d.b0,
+                         m0.d_ch[C_B], d.partials);
+    finalize(d, std::max(g.nActive0, 1), FIN_FINAL, d.hist);
+    double rr = 0.0;
+    HIPCHK(hipMemcpyAsync(&rr, d.hist, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    relNow = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    outerHist.push_back(relNow);
+    if (relNow <= tol || nOuter >= maxOuter || bb <= 1e-30) break;
+    zeroActive(g, 0, m0.d_ch[C_X], d.zeroFlag, d.stream);
+    int it = 0;
+    float rIn = 0.f;
+    if (int rc = deviceSolve(g, p, &it, &rIn)) return rc;
+    nInner += it;
+    nOuter++;
+    hipLaunchKernelGGL(k_accum64, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, d.x64, m0.d_ch[C_X], n0, 0);
+  }
+  hipLaunchKernelGGL(k_accum64, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, d.x64, m0.d_ch[C_X], n0, 1);
+  HIPCHK(hipMemcpyAsync(m0.d_ch[C_B], d.b0, n0 * sizeof(float), hipMemcpyDeviceToDevice, d.stream));
+  if (x64out) HIPCHK(hipMemcpyAsync(x64out, d.x64, n0 * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  HIPCHK(hipGetLastError());
+  g.history = outerHist;
+  *outer = nOuter;
+  *inner = nInner;
+  *rel = relNow;
   return MSBGX_OK;
 }
 

@@ -312,6 +312,7 @@ int buildMetadata(Grid &g, Sampler &s, std::vector<float> &rhs)
   for (int bid = 0; bid < g.nBlocks; bid++)
     if (m0.level[bid] == 0 && (blockFluid[bid] || blockRhs[bid])) order.push_back(bid);
   g.nActive0 = (int)order.size();
+  g.active0 = order;
   for (int bid = 0; bid < g.nBlocks; bid++)
     if (!(m0.level[bid] == 0 && (blockFluid[bid] || blockRhs[bid]))) order.push_back(bid);
   std::vector<int64_t> offNat = m0.off;

@@ -27,6 +27,7 @@ CH_PRESSURE, CH_DIVERGENCE, CH_R, CH_Z, CH_D, CH_Q = 0, 1, 2, 3, 4, 5
 CH_FACE_U, CH_FACE_V, CH_FACE_W, CH_DIAGONAL, CH_FLAGS = 7, 8, 9, 10, 11
 DBC_OPEN, DBC_SOLID = 0, 1
 OPT_LITERAL_VCYCLE, OPT_NO_GRAPH = 0x1, 0x2
+PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
 CELL_SOLID, CELL_VOID = 0x0001, 0x1000
 BLK_COARSE_FINE, BLK_FINE_COARSE, BLK_NO_FLUID, BLK_EXISTS = 1 << 2, 1 << 3, 1 << 5, 1 << 12
 
@@ -150,6 +151,10 @@ def _load():
         "msbgx_time_kernel": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int)]),
         "msbgx_level_map_from_scene": (C.c_int, [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, P]),
+        "msbgx_set_precision": (C.c_int, [P, C.c_int]),
+        "msbgx_solve_pressure_refined": (C.c_int, [P, C.POINTER(SolveParams), C.c_double, C.c_int,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), P]),
+        "msbgx_level0_blocks": (C.c_int, [P, P, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -167,7 +172,8 @@ EXPORTED = [
     "msbgx_solve_pressure", "msbgx_residual_history", "msbgx_last_solve_ms", "msbgx_get_channel_block",
     "msbgx_set_channel_block", "msbgx_block_info", "msbgx_relax_list", "msbgx_omega", "msbgx_relax",
     "msbgx_multiply", "msbgx_downsample", "msbgx_prolongate", "msbgx_zero_channel", "msbgx_copy_channel",
-    "msbgx_vcycle", "msbgx_time_kernel", "msbgx_level_map_from_scene",
+    "msbgx_vcycle", "msbgx_time_kernel", "msbgx_level_map_from_scene", "msbgx_set_precision",
+    "msbgx_solve_pressure_refined", "msbgx_level0_blocks",
 ]
 
 
@@ -254,6 +260,29 @@ class MultiresSparseGrid:
         it, rr = C.c_int(), C.c_float()
         _ck(lib.msbgx_solve_pressure(self._h, C.byref(p), C.byref(it), C.byref(rr)), "solve")
         return it.value, rr.value
+
+    def set_precision(self, prec):
+        _ck(lib.msbgx_set_precision(self._h, prec), "set_precision")
+
+    def level0_blocks(self):
+        out = np.zeros(self.n_blocks, np.int32)
+        n = lib.msbgx_level0_blocks(self._h, _ptr(out), self.n_blocks)
+        return out[:n].copy()
+
+    def solve_pressure_refined(self, tol=1e-10, max_outer=20, inner_tol=1e-6, n_pre=3, n_post=3, n_coarse=24,
+                               inner_max_iter=500, want_x=False):
+        """FP64 iterative refinement around the MG-PCG.  Returns (outer, inner
+        iterations, FP64 relative residual, x64 per level-0 block or None)."""
+        p = SolveParams(n_pre, n_post, n_coarse, inner_max_iter, inner_tol)
+        o, i, r = C.c_int(), C.c_int(), C.c_double()
+        blocks = self.level0_blocks()
+        x = np.zeros(len(blocks) * self.bsx ** 3) if want_x else None
+        _ck(lib.msbgx_solve_pressure_refined(self._h, C.byref(p), tol, max_outer, C.byref(o), C.byref(i),
+                                             C.byref(r), _ptr(x) if want_x else None), "solve_refined")
+        xb = None
+        if want_x:
+            xb = {int(b): x[k * self.bsx ** 3:(k + 1) * self.bsx ** 3] for k, b in enumerate(blocks)}
+        return o.value, i.value, r.value, xb
 
     def residual_history(self):
         n = lib.msbgx_residual_history(self._h, None, 0)

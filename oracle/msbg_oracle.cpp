@@ -953,6 +953,115 @@ struct Oracle {
   }
 };
 
+/* FP64 reference for the precision form: the same level-0 operator as the
+ * PCG of the reference applies (Q4: coarse-block unknowns held at 0), in
+ * double -- (A x)_i = x_i / D_i - sum_nb W x_nb over same-level level-0
+ * face neighbours (src/msbg3.cpp:1282-1291) -- solved with textbook
+ * Jacobi-preconditioned CG (M^-1 = D, zero on non-fluid cells). */
+struct Fp64Solver {
+  Oracle &o;
+  int b, n;
+  std::vector<int> lvl0;        // level-0 block ids
+  std::vector<int> slot;        // bid -> index in lvl0 or -1
+  explicit Fp64Solver(Oracle &o_) : o(o_), b(o_.bsx0), n(o_.bsx0 * o_.bsx0 * o_.bsx0)
+  {
+    slot.assign(o.nBlocks, -1);
+    for (int bid = 0; bid < o.nBlocks; bid++)
+      if (o.binfo[0][bid].level == 0) { slot[bid] = (int)lvl0.size(); lvl0.push_back(bid); }
+  }
+  void apply(const std::vector<double> &x, std::vector<double> &y)
+  {
+    const int b2 = b * b;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(o.nThreads)
+    for (int i = 0; i < (int)lvl0.size(); i++) {
+      const int bid = lvl0[i];
+      int bx, by, bz;
+      o.blockCoords(bid, bx, by, bz);
+      const float *D = o.fch[CH_DIAG][0][0].get(bid);
+      const float *W[3] = {o.fch[CH_W0][0][0].get(bid), o.fch[CH_W1][0][0].get(bid), o.fch[CH_W2][0][0].get(bid)};
+      const double *xb = &x[(size_t)i * n];
+      for (int v = 0; v < n; v++) {
+        double yv = 0.0;
+        if (D[v] != 0.0f) {
+          const int vx = v % b, vy = (v / b) % b, vz = v / b2;
+          const int c[3] = {vx, vy, vz}, st[3] = {1, b, b2};
+          const int nbOff[3] = {1, o.nbx, o.nbx * o.nby};
+          const int bc[3] = {bx, by, bz}, nbMax[3] = {o.nbx, o.nby, o.nbz};
+          double s = 0.0;
+          for (int d = 0; d < 3; d++) {
+            /* -d face: own W */
+            if (c[d] > 0) s += (double)W[d][v] * xb[v - st[d]];
+            else if (bc[d] > 0 && slot[bid - nbOff[d]] >= 0)
+              s += (double)W[d][v] * x[(size_t)slot[bid - nbOff[d]] * n + v + (b - 1) * st[d]];
+            /* +d face: W of the next cell */
+            if (c[d] < b - 1) s += (double)W[d][v + st[d]] * xb[v + st[d]];
+            else if (bc[d] < nbMax[d] - 1 && slot[bid + nbOff[d]] >= 0) {
+              const int nb = bid + nbOff[d];
+              const float *Wn = o.fch[CH_W0 + d][0][0].get(nb);
+              s += (double)Wn[v - (b - 1) * st[d]] * x[(size_t)slot[nb] * n + v - (b - 1) * st[d]];
+            }
+          }
+          yv = xb[v] / (double)D[v] - s;
+        }
+        y[(size_t)i * n + v] = yv;
+      }
+    }
+  }
+  int solve(double tol, int maxit, std::vector<double> &x, double *relOut)
+  {
+    const size_t N = lvl0.size() * (size_t)n;
+    std::vector<double> r(N), z(N), p(N), q(N), dinv(N, 0.0);
+    x.assign(N, 0.0);
+    for (size_t i = 0; i < lvl0.size(); i++) {
+      const float *B = o.fch[CH_B][0][0].get(lvl0[i]);
+      const float *D = o.fch[CH_DIAG][0][0].get(lvl0[i]);
+      for (int v = 0; v < n; v++) {
+        r[i * n + v] = B ? (double)B[v] : 0.0;
+        dinv[i * n + v] = (double)D[v];
+      }
+    }
+    double bb = 0;
+    for (size_t k = 0; k < N; k++) bb += r[k] * r[k];
+    if (bb <= 0) { *relOut = 0; return 0; }
+    for (size_t k = 0; k < N; k++) z[k] = dinv[k] * r[k];
+    p = z;
+    double rz = 0;
+    for (size_t k = 0; k < N; k++) rz += r[k] * z[k];
+    int it = 0;
+    double rr = bb;
+    for (; it < maxit && rr > tol * tol * bb; it++) {
+      apply(p, q);
+      double pq = 0;
+      for (size_t k = 0; k < N; k++) pq += p[k] * q[k];
+      const double al = rz / pq;
+      rr = 0;
+      for (size_t k = 0; k < N; k++) {
+        x[k] += al * p[k];
+        r[k] -= al * q[k];
+        rr += r[k] * r[k];
+      }
+      for (size_t k = 0; k < N; k++) z[k] = dinv[k] * r[k];
+      double rzn = 0;
+      for (size_t k = 0; k < N; k++) rzn += r[k] * z[k];
+      const double be = rzn / rz;
+      rz = rzn;
+      for (size_t k = 0; k < N; k++) p[k] = z[k] + be * p[k];
+    }
+    /* true residual */
+    apply(x, q);
+    double tr = 0;
+    for (size_t i = 0; i < lvl0.size(); i++) {
+      const float *B = o.fch[CH_B][0][0].get(lvl0[i]);
+      for (int v = 0; v < n; v++) {
+        const double e = (B ? (double)B[v] : 0.0) - q[i * n + v];
+        tr += e * e;
+      }
+    }
+    *relOut = sqrt(tr / bb);
+    return it;
+  }
+};
+
 /* Caller-side level policy (src/flip_sim.cpp:480-548): 26-neighbourhood BFS
  * block distance from seed blocks; d <= d0 -> 0, d == d0 + 1 -> 1, else the
  * coarsest level.  Seeds: blocks holding a liquid cell (mode 0) or a liquid
@@ -1112,6 +1221,21 @@ int orc_solve(void *h, int nPre, int nPost, int nCoarse, int maxIter, float tol,
   o->solvePCG(sp, &it, &rr, &hv);
   if (relRes) *relRes = rr;
   if (hist) for (int i = 0; i < (int)hv.size() && i < histCap; i++) hist[i] = hv[i];
+  return it;
+}
+
+/* FP64 solution of the level-0 system (see Fp64Solver); out holds
+ * nBlocks * bsx0^3 doubles, block bid at bid * bsx0^3, zero for blocks that
+ * are not level 0.  Returns CG iterations; *rel = FP64 true residual. */
+int orc_solve_fp64(void *h, double tol, int maxit, double *out, double *rel)
+{
+  Oracle *o = (Oracle *)h;
+  Fp64Solver s(*o);
+  std::vector<double> x;
+  const int it = s.solve(tol, maxit, x, rel);
+  const size_t n = (size_t)o->bsx0 * o->bsx0 * o->bsx0;
+  memset(out, 0, sizeof(double) * n * o->nBlocks);
+  for (size_t i = 0; i < s.lvl0.size(); i++) memcpy(out + (size_t)s.lvl0[i] * n, &x[i * n], sizeof(double) * n);
   return it;
 }
 

@@ -37,6 +37,7 @@ for name, res, args in [
     ("orc_copy_channel", None, [P, C.c_int, C.c_int]),
     ("orc_solve", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float), P, C.c_int]),
     ("orc_export_level0", None, [P, C.c_int, P]),
+    ("orc_solve_fp64", C.c_int, [P, C.c_double, C.c_int, P, C.POINTER(C.c_double)]),
 ]:
     f = getattr(_lib, name)
     f.restype, f.argtypes = res, args
@@ -139,6 +140,16 @@ class Oracle:
         hist = np.zeros(max_iter + 1)
         it = _lib.orc_solve(self._h, n_pre, n_post, n_coarse, max_iter, tol, C.byref(rr), _p(hist), max_iter + 1)
         return it, rr.value, hist
+
+    def solve_fp64(self, tol=1e-12, max_iter=5000):
+        """FP64 Jacobi-PCG solution of the level-0 system; returns (iters, true
+        rel. residual, {bid: x block})."""
+        n = self.bsx ** 3
+        out = np.zeros(self.n_blocks * n)
+        rel = C.c_double()
+        it = _lib.orc_solve_fp64(self._h, tol, max_iter, _p(out), C.byref(rel))
+        lv, _ = self.block_info(0)
+        return it, rel.value, {b: out[b * n:(b + 1) * n] for b in range(self.n_blocks) if lv[b] == 0}
 
     def export_level0(self, ch, shape):
         out = np.zeros(shape[0] * shape[1] * shape[2], np.float32)

@@ -389,3 +389,45 @@ def test_zero_rhs_returns_immediately():
     o.prepare_scene(s)
     it, rr, _ = o.solve(max_iter=10, tol=1e-6)
     assert it == 0 and rr == 0.0
+
+
+def test_fp64_reference_solver_matches_dense_solve():
+    """The oracle's FP64 solve (precision-form reference) on a tiny grid equals
+    numpy.linalg.solve of the operator assembled from the FP32 oracle's apply
+    (exact on this grid: the apply of a unit vector has no rounding)."""
+    n, bsx = 8, 4
+    rng = np.random.default_rng(13)
+    void = np.zeros((n, n, n), bool)
+    void[:, -2:, :] = True
+    w = [np.ones((n, n, n), np.float32) for _ in range(3)]
+    for k, ax in enumerate((2, 1, 0)):
+        sl = [slice(None)] * 3
+        sl[ax] = 0
+        w[k][tuple(sl)] = 0.0
+    rhs = np.where(void, 0, rng.standard_normal((n, n, n))).astype(np.float32)
+    o = dense_problem(n, bsx, levels=2, void=void, w=w, rhs=rhs)
+    it, rel, xb = o.solve_fp64(tol=1e-14, max_iter=1000)
+    assert rel <= 1e-13
+    nb = n // bsx
+    x = np.zeros((n, n, n))
+    for bid, blk in xb.items():
+        bx, by, bz = bid % nb, (bid // nb) % nb, bid // (nb * nb)
+        x[bz * bsx:(bz + 1) * bsx, by * bsx:(by + 1) * bsx, bx * bsx:(bx + 1) * bsx] = blk.reshape(bsx, bsx, bsx)
+    # independent FP64 assembly: A_ii = 1/D_i, A_ij = -W(face ij) between fluid cells
+    D = blocks_to_field(o, n, bsx, 10).astype(np.float64)
+    fluid = ~void
+    idx = np.flatnonzero(fluid.reshape(-1))
+    pos = -np.ones(n ** 3, int)
+    pos[idx] = np.arange(idx.size)
+    A = np.zeros((idx.size, idx.size))
+    strides = (1, n, n * n)
+    for k, c in enumerate(idx):
+        z, y, xx = np.unravel_index(c, (n, n, n))
+        A[k, k] = 1.0 / D.reshape(-1)[c]
+        for d, (coord, wk) in enumerate(((xx, w[0]), (y, w[1]), (z, w[2]))):
+            if coord > 0 and pos[c - strides[d]] >= 0:
+                A[k, pos[c - strides[d]]] -= float(wk.reshape(-1)[c])
+            if coord < n - 1 and pos[c + strides[d]] >= 0:
+                A[k, pos[c + strides[d]]] -= float(wk.reshape(-1)[c + strides[d]])
+    xs = np.linalg.solve(A, rhs.reshape(-1)[idx].astype(np.float64))
+    assert np.linalg.norm(x.reshape(-1)[idx] - xs) <= 1e-10 * np.linalg.norm(xs)
