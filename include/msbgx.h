@@ -132,7 +132,39 @@ int msbgx_solve_pressure_refined(msbgx_grid *g, const msbgx_solve_params *inner,
  * (the order of msbgx_solve_pressure_refined's x64_out); returns the count. */
 int msbgx_level0_blocks(const msbgx_grid *g, int *out, int cap);
 
-/* Relative residual after each PCG iteration of the last solve (up to cap). */
+/* Batched form (no reference counterpart; the reference solves one grid per
+ * call, src/msbg3.cpp:3262-3271).  Merges n prepared problems that share the
+ * block size, level count and domain boundary spec into one block set; each
+ * smoother, apply and transfer launch then covers the blocks of every
+ * problem, and each problem keeps its own PCG scalars, smoother weight and
+ * convergence test.  The problems are copied (their handles stay usable and
+ * may be destroyed).  Block `b` of problem `q` is block
+ * msbgx_batch_block_base(batch, q) + b of the batch handle, so the channel
+ * inspection functions and the per-kernel entry points work on the batch.
+ * Each problem's iterates equal those of msbgx_solve_pressure on its own
+ * handle.  Domain spec, refinement map, prepare, msbgx_solve_pressure and the
+ * refined solve are rejected on a batch handle. */
+int msbgx_batch_create(msbgx_grid **out, msbgx_grid *const *problems, int n);
+int msbgx_batch_size(const msbgx_grid *batch);
+int msbgx_batch_block_base(const msbgx_grid *batch, int problem);
+/* Per-problem iteration counts and |r|/|b| into iters[n], rel_residual[n]. */
+int msbgx_batch_solve(msbgx_grid *batch, const msbgx_solve_params *p, int *iters, float *rel_residual);
+/* Per-problem relative residual history of the last batched solve. */
+int msbgx_batch_residual_history(const msbgx_grid *batch, int problem, double *out, int cap);
+
+/* Autotuner (no reference counterpart): measures, on the handle's own
+ * prepared problem (single or batch), the smoother kernel per block size
+ * (bit k of *smoother_mask set = the LDS-tile kernel for blocks of
+ * bsx0 >> k; FP32 operands, blocks up to 16^3) and the PCG iterations per
+ * graph replay (*graph_chunk), and keeps the fastest.  The choices change
+ * launch shapes only: results stay bit-identical.  Runs trial solves with
+ * `p`; the pressure channel is zero on return. */
+int msbgx_autotune(msbgx_grid *g, const msbgx_solve_params *p, int *graph_chunk, int *smoother_mask);
+/* Explicit settings for the same two knobs (graph_chunk in [1,64]). */
+int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask);
+
+/* Relative residual after each PCG iteration of the last solve (up to cap);
+ * returns the history length (out may be NULL to query it). */
 int msbgx_residual_history(const msbgx_grid *g, double *out, int cap);
 /* Device time of the last solve in milliseconds (HIP events around the PCG). */
 double msbgx_last_solve_ms(const msbgx_grid *g);

@@ -58,11 +58,15 @@ int checkGrid(const msbgx_grid *g)
   return MSBGX_OK;
 }
 
+
 int prepareWith(Grid &g, Sampler &s, const msbgx_scene *scene)
 {
   std::vector<float> rhs;
   int rc = buildMetadata(g, s, rhs);
   if (rc) return rc;
+  g.hasScene = scene != nullptr;
+  if (scene) g.scene = *scene;
+  g.hostRhs = rhs;
   rc = deviceUpload(g, rhs, scene);
   if (rc) return rc;
   g.prepared = true;
@@ -75,6 +79,17 @@ int prepareWith(Grid &g, Sampler &s, const msbgx_scene *scene)
 using namespace msbgx;
 
 struct msbgx_grid : Grid {};
+
+/* Single-grid operations that have no meaning on a batch handle. */
+static int checkSingle(const msbgx_grid *g)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (g->batched) {
+    setError("operation not available on a batch handle");
+    return MSBGX_EINVAL;
+  }
+  return MSBGX_OK;
+}
 
 extern "C" {
 
@@ -133,7 +148,7 @@ int msbgx_set_options(msbgx_grid *g, unsigned options)
 
 int msbgx_set_domain_bc(msbgx_grid *g, int l, int r, int b, int t, int f, int k)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   const int v[6] = {l, r, b, t, f, k};
   for (int i = 0; i < 6; i++)
     if (v[i] != MSBGX_DBC_OPEN && v[i] != MSBGX_DBC_SOLID) { setError("unknown boundary kind"); return MSBGX_EINVAL; }
@@ -146,7 +161,7 @@ int msbgx_set_domain_bc(msbgx_grid *g, int l, int r, int b, int t, int f, int k)
 
 int msbgx_regularize_refinement_map(const msbgx_grid *g, const int *in, int *out)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!in || !out) { setError("null map"); return MSBGX_EINVAL; }
   if (g->nLevels < 2) { std::copy(in, in + g->nBlocks, out); return MSBGX_OK; }
   regularize(*g, in, out);
@@ -155,7 +170,7 @@ int msbgx_regularize_refinement_map(const msbgx_grid *g, const int *in, int *out
 
 int msbgx_set_refinement_map(msbgx_grid *g, const int *levels)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!levels) { setError("null map"); return MSBGX_EINVAL; }
   for (int i = 0; i < g->nBlocks; i++)
     if (levels[i] < 0 || levels[i] >= g->nLevels) { setError("block level out of range"); return MSBGX_EINVAL; }
@@ -177,7 +192,7 @@ int msbgx_set_refinement_map(msbgx_grid *g, const int *levels)
 
 int msbgx_prepare_pressure_solve(msbgx_grid *g, const msbgx_callbacks *cb)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!cb || !cb->sample_cell_type || !cb->sample_face_coeff || !cb->sample_rhs) { setError("incomplete callbacks"); return MSBGX_EINVAL; }
   CallbackSampler s;
   s.cb = *cb;
@@ -187,7 +202,7 @@ int msbgx_prepare_pressure_solve(msbgx_grid *g, const msbgx_callbacks *cb)
 int msbgx_prepare_pressure_solve_dense(msbgx_grid *g, const uint16_t *flags, const float *fu, const float *fv,
                                        const float *fw, const float *rhs)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!flags || !fu || !fv || !fw) { setError("null input array"); return MSBGX_EINVAL; }
   DenseSampler s;
   s.nx = g->sx0; s.ny = g->sy0; s.flags = flags; s.w[0] = fu; s.w[1] = fv; s.w[2] = fw; s.r = rhs;
@@ -196,7 +211,7 @@ int msbgx_prepare_pressure_solve_dense(msbgx_grid *g, const uint16_t *flags, con
 
 int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!scene || scene->nx != g->sx0 || scene->ny != g->sy0 || scene->nz != g->sz0) {
     setError("scene size must equal the grid size");
     return MSBGX_EINVAL;
@@ -213,7 +228,7 @@ int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene)
 
 int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters, float *rel)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!p || !iters || !rel) { setError("null argument"); return MSBGX_EINVAL; }
   if (!g->prepared) { setError("solve before prepare"); return MSBGX_ESTATE; }
   if (p->n_pre < 0 || p->n_post < 0 || p->n_coarse < 0 || p->max_iter < 0 || !(p->tol >= 0.0f)) {
@@ -238,7 +253,7 @@ int msbgx_set_precision(msbgx_grid *g, int prec)
 int msbgx_solve_pressure_refined(msbgx_grid *g, const msbgx_solve_params *p, double tol, int max_outer,
                                  int *outer_iters, int *inner_iters, double *rel, double *x64_out)
 {
-  if (int rc = checkGrid(g)) return rc;
+  if (int rc = checkSingle(g)) return rc;
   if (!p || !outer_iters || !inner_iters || !rel || !(tol > 0.0) || max_outer < 1) {
     setError("bad argument");
     return MSBGX_EINVAL;
@@ -256,8 +271,8 @@ int msbgx_level0_blocks(const msbgx_grid *g, int *out, int cap)
 
 int msbgx_residual_history(const msbgx_grid *g, double *out, int cap)
 {
-  if (!g || !out) return 0;
-  const int n = std::min<int>(cap, (int)g->history.size());
+  if (!g) return 0;
+  const int n = out ? std::min<int>(cap, (int)g->history.size()) : 0;
   for (int i = 0; i < n; i++) out[i] = g->history[i];
   return (int)g->history.size();
 }
@@ -356,6 +371,86 @@ int msbgx_time_kernel(msbgx_grid *g, int which, int reps, double *ms, double *by
   MSBGX_PREPARED_OR_FAIL(g);
   if (which < 0 || which > 2 || reps < 1 || !ms || !bytes || !cells) { setError("bad argument"); return MSBGX_EINVAL; }
   return deviceTimeKernel(*g, which, reps, ms, bytes, cells);
+}
+
+static int checkParams(const msbgx_solve_params *p)
+{
+  if (!p || p->n_pre < 0 || p->n_post < 0 || p->n_coarse < 0 || p->max_iter < 0 || !(p->tol >= 0.0f)) {
+    setError("invalid solve parameters");
+    return MSBGX_EINVAL;
+  }
+  return MSBGX_OK;
+}
+
+int msbgx_batch_create(msbgx_grid **out, msbgx_grid *const *problems, int n)
+{
+  if (!out) { setError("out is null"); return MSBGX_EINVAL; }
+  *out = nullptr;
+  if (!problems || n < 1) { setError("empty batch"); return MSBGX_EINVAL; }
+  std::vector<const Grid *> probs;
+  for (int i = 0; i < n; i++) {
+    if (!problems[i] || problems[i]->batched) { setError("batch members must be single prepared grids"); return MSBGX_EINVAL; }
+    probs.push_back(problems[i]);
+  }
+  msbgx_grid *m = new (std::nothrow) msbgx_grid();
+  if (!m) { setError("out of host memory"); return MSBGX_ENOMEM; }
+  int rc = buildBatchGrid(*m, probs);
+  if (!rc) rc = deviceUploadBatch(*m, probs);
+  if (rc) {
+    deviceFree(*m);
+    delete m;
+    return rc;
+  }
+  m->prepared = true;
+  *out = m;
+  return MSBGX_OK;
+}
+
+int msbgx_batch_size(const msbgx_grid *b) { return b ? (b->batched ? b->nProb : 1) : 0; }
+
+int msbgx_batch_block_base(const msbgx_grid *b, int problem)
+{
+  if (int rc = checkGrid(b)) return rc;
+  if (!b->batched) return problem == 0 ? 0 : (setError("problem index out of range"), MSBGX_EINVAL);
+  if (problem < 0 || problem >= b->nProb) { setError("problem index out of range"); return MSBGX_EINVAL; }
+  return b->probBlock0[problem];
+}
+
+int msbgx_batch_solve(msbgx_grid *b, const msbgx_solve_params *p, int *iters, float *rel)
+{
+  if (int rc = checkGrid(b)) return rc;
+  if (!b->batched) { setError("not a batch handle"); return MSBGX_EINVAL; }
+  if (!iters || !rel) { setError("null argument"); return MSBGX_EINVAL; }
+  if (int rc = checkParams(p)) return rc;
+  return deviceSolveBatch(*b, *p, iters, rel);
+}
+
+int msbgx_batch_residual_history(const msbgx_grid *b, int problem, double *out, int cap)
+{
+  if (!b || !b->batched || problem < 0 || problem >= (int)b->probHistory.size()) return 0;
+  const std::vector<double> &h = b->probHistory[problem];
+  for (int i = 0; out && i < (int)h.size() && i < cap; i++) out[i] = h[i];
+  return (int)h.size();
+}
+
+int msbgx_autotune(msbgx_grid *g, const msbgx_solve_params *p, int *graph_chunk, int *smoother_mask)
+{
+  MSBGX_PREPARED_OR_FAIL(g);
+  if (!graph_chunk || !smoother_mask) { setError("null argument"); return MSBGX_EINVAL; }
+  if (int rc = checkParams(p)) return rc;
+  return deviceAutotune(*g, *p, graph_chunk, smoother_mask);
+}
+
+int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask)
+{
+  if (int rc = checkGrid(g)) return rc;
+  if (graph_chunk < 1 || graph_chunk > 64 || smoother_mask < 0 || smoother_mask >= (1 << kMaxLevels)) {
+    setError("bad launch configuration");
+    return MSBGX_EINVAL;
+  }
+  g->graphChunk = graph_chunk;
+  for (int l = 0; l < kMaxLevels; l++) g->smootherVariant[l] = (smoother_mask >> l) & 1;
+  return MSBGX_OK;
 }
 
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)

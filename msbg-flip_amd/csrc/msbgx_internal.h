@@ -81,6 +81,22 @@ struct Grid {
   void *dev = nullptr;          // solver state (device buffers, stream, graph)
   std::vector<double> history;
   double lastSolveMs = 0.0;
+  /* inputs kept for building batched block sets */
+  bool hasScene = false;
+  msbgx_scene scene{};
+  std::vector<float> hostRhs;   // MG0 pool layout (callback / dense prepare paths)
+  /* batched block sets: several independent problems merged into one */
+  bool batched = false;
+  int nProb = 1;
+  std::vector<std::vector<double>> probHistory;  // per-problem relative residuals of the last batched solve
+  std::vector<int> probBlock0;  // first block of each problem
+  std::vector<int> probOf;      // problem of each block
+  std::vector<float> probOmega; // smoother weight per problem
+  std::vector<int> nbrTable;    // 6 face neighbours per block (-1 = domain boundary)
+  std::vector<int> probActive0; // PCG-active level-0 blocks per problem
+  std::vector<std::vector<int>> probRaster;  // nbx, nby, nbz per problem
+  int smootherVariant[kMaxLevels] = {0, 0, 0, 0, 0};  // per block level: 0 column-resident, 1 LDS tile
+  int graphChunk = 4;           // PCG iterations per hipGraph replay
 
   int bsxL(int l) const { return bsx0 >> l; }
   void blockCoords(int bid, int &bx, int &by, int &bz) const
@@ -129,6 +145,11 @@ int deviceCopy(Grid &g, int src, int dst);
 int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse);
 int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
 int deviceSetPrecision(Grid &g, int prec);
+/* batched block sets (batch.cpp / solver.hip) */
+int buildBatchGrid(Grid &merged, const std::vector<const Grid *> &probs);
+int deviceUploadBatch(Grid &merged, const std::vector<const Grid *> &probs);
+int deviceSolveBatch(Grid &merged, const msbgx_solve_params &p, int *iters, float *rel);
+int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *variantMask);
 int deviceSolveRefined(Grid &g, const msbgx_solve_params &p, double tol, int maxOuter, int *outer, int *inner,
                        double *rel, double *x64out);
 

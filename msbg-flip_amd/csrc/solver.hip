@@ -53,6 +53,10 @@ struct LevelArgs {
   /* precision form: 16-bit copies of W_x, W_y, W_z, D read by the smoother */
   const uint16_t *H[4];
   int prec;      // MSBGX_PREC_FP32 / FP16 / BF16
+  /* batched block sets (several independent problems in one block set) */
+  const int *nbr;        // 6 face neighbours per block, -1 = domain boundary; null: raster arithmetic
+  const int *probOf;     // problem of each block
+  const float *omegaOf;  // smoother weight of each problem
 };
 
 __device__ __forceinline__ float fromBits16(uint16_t v, int prec)
@@ -110,7 +114,6 @@ __device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int
 {
   constexpr int NF = Cfg<B>::NF;
   const int tx = threadIdx.x % B, ty = threadIdx.x / B;
-  int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
   const float a13 = (float)(1.0 * (1. / 3.));      // DPMR_GRAD_COEFF_CF (src/msbg.h:85-87)
   const float a23 = (float)(1.0 * 2 * (1. / 3.));  // 2 * DPMR_GRAD_COEFF_CF
   const float *Wd[3] = {a.W0, a.W1, a.W2};
@@ -118,17 +121,23 @@ __device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int
 #pragma unroll
   for (int face = 0; face < 6; face++) {
     const int dir = face >> 1, side = face & 1;
-    int nbxx = bx, nbyy = by, nbzz = bz;
-    if (dir == 0) nbxx += side ? 1 : -1;
-    else if (dir == 1) nbyy += side ? 1 : -1;
-    else nbzz += side ? 1 : -1;
+    int nb;
+    if (a.nbr) {
+      nb = a.nbr[6 * bid + face];   // batched block sets: explicit face-neighbour table
+    } else {
+      int nbxx = bid % a.nbx, nbyy = (bid / a.nbx) % a.nby, nbzz = bid / (a.nbx * a.nby);
+      if (dir == 0) nbxx += side ? 1 : -1;
+      else if (dir == 1) nbyy += side ? 1 : -1;
+      else nbzz += side ? 1 : -1;
+      nb = (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz)
+               ? -1 : nbxx + a.nbx * (nbyy + a.nby * nbzz);
+    }
     float val = 0.0f;
     int kind;
     int64_t off2 = 0;
-    if (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz) {
+    if (nb < 0) {
       kind = FK_DOMAIN;
     } else {
-      const int nb = nbxx + a.nbx * (nbyy + a.nby * nbzz);
       const int lev2 = a.level[nb];
       off2 = a.off[nb];
       if (lev2 == lev) {
@@ -256,6 +265,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
 
   const int x = threadIdx.x % B, y = threadIdx.x / B;
   const int o = x / S + y;
+  if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
   const float om = omega, omm = 1.0f - omega;
   for (int k = 0; k < nIter - 1; k++) {
     const bool rev = (reverse != 0) == ((k & 1) != 0);
@@ -452,6 +462,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
   c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
   c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
   c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
 
   ColRegs<B, REGC> r;
@@ -689,12 +700,31 @@ __device__ double sumPartials(const double *p, int n)
 
 enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL };
 
+__device__ void applyFinalize(PcgState *st, double s, int op, double *out);
+
 __global__ void __launch_bounds__(256) k_finalize(PcgState *st, const double *partials, int n, int op, double *out)
 {
   const bool skipIfDone = (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA);
   if (skipIfDone && st->done) return;
   const double s = sumPartials(partials, n);
   if (threadIdx.x != 0) return;
+  applyFinalize(st, s, op, out);
+}
+
+/* Batched form: workgroup p finalizes problem p from its chunk partials. */
+__global__ void __launch_bounds__(256) k_finalize_b(PcgState *st, const double *partials, const int *cb, const int *ce,
+                                                   int op, double *out, int outStride)
+{
+  const int p = blockIdx.x;
+  const bool skipIfDone = (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA);
+  if (skipIfDone && st[p].done) return;
+  const double s = sumPartials(partials + cb[p], ce[p] - cb[p]);
+  if (threadIdx.x != 0) return;
+  applyFinalize(st + p, s, op, out ? out + (size_t)p * outStride : nullptr);
+}
+
+__device__ void applyFinalize(PcgState *st, double s, int op, double *out)
+{
   switch (op) {
     case FIN_BNORM:
       st->bnorm2 = s;
@@ -733,10 +763,11 @@ __global__ void __launch_bounds__(256) k_finalize(PcgState *st, const double *pa
 /* Divergence right-hand side of level-0 blocks from the scene's MAC velocity
  * (flipRhsCB, src/flip_sim.cpp:329-338; sampled as in src/msbg3.cpp:2551-2579). */
 __global__ void k_scene_rhs(msbgx_scene s, const int *__restrict__ blocks, const int64_t *__restrict__ off, int nbx,
-                            int nby, int bsx0, float *__restrict__ Bout)
+                            int nby, int bsx0, float *__restrict__ Bout, int bidShift)
 {
   const int bid = blocks[blockIdx.x];
-  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  const int lb = bid - bidShift;   // block id inside its own raster
+  const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
   const int n = bsx0 * bsx0 * bsx0;
   const int64_t o = off[bid];
   for (int v = threadIdx.x; v < n; v += blockDim.x) {
@@ -746,6 +777,21 @@ __global__ void k_scene_rhs(msbgx_scene s, const int *__restrict__ blocks, const
 }
 
 /* -------------------------------------------------------------------- host */
+
+/* Device state of a batched block set: the chunk table of the PCG vector
+ * kernels (chunk c covers cells [cBeg[c], cEnd[c]) of problem cProb[c];
+ * problem p owns chunks [cb[p], ce[p])) and one PCG state per problem. */
+struct BatchDev {
+  int nChunks = 0;
+  int *cProb = nullptr, *cb = nullptr, *ce = nullptr;
+  int64_t *cBeg = nullptr, *cEnd = nullptr;
+  PcgState *states = nullptr;
+  int *allDone = nullptr;   // 1 once every problem has converged or stopped
+  double *hist = nullptr;
+  int histStride = 0;
+  hipGraphExec_t graph = nullptr;
+  int graphChunk = 0;
+};
 
 struct Dev {
   hipStream_t stream = nullptr;
@@ -764,6 +810,10 @@ struct Dev {
   int *dActive0 = nullptr;
   int64_t refCap = 0;
   bool b0Valid = false;
+  /* batched block sets */
+  int *nbr = nullptr, *probOf = nullptr;
+  float *omegaOf = nullptr;
+  BatchDev *batch = nullptr;
 };
 
 static Dev &dev(Grid &g) { return *(Dev *)g.dev; }
@@ -799,6 +849,13 @@ void deviceFree(Grid &g)
     if (d.x64) (void)hipFree(d.x64);
     if (d.b0) (void)hipFree(d.b0);
     if (d.dActive0) (void)hipFree(d.dActive0);
+    auto f = [](void *p) { if (p) (void)hipFree(p); };
+    f(d.nbr); f(d.probOf); f(d.omegaOf);
+    if (BatchDev *b = d.batch) {
+      if (b->graph) (void)hipGraphExecDestroy(b->graph);
+      f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
+      delete b;
+    }
     if (d.ev0) (void)hipEventDestroy(d.ev0);
     if (d.ev1) (void)hipEventDestroy(d.ev1);
     if (d.stream) (void)hipStreamDestroy(d.stream);
@@ -817,6 +874,10 @@ static LevelArgs levelArgs(const Grid &g, int L)
   for (int d = 0; d < 3; d++) a.bcW[d] = g.domainBc[d][1] != MSBGX_DBC_OPEN ? 0.0f : 1.0f;
   for (int k = 0; k < 4; k++) a.H[k] = m.d_H[k];
   a.prec = (g.prec != MSBGX_PREC_FP32 && m.d_H[0]) ? g.prec : MSBGX_PREC_FP32;
+  const Dev *dd = (const Dev *)g.dev;
+  a.nbr = dd ? dd->nbr : nullptr;
+  a.probOf = dd ? dd->probOf : nullptr;
+  a.omegaOf = dd ? dd->omegaOf : nullptr;
   return a;
 }
 
@@ -900,7 +961,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
     /* level-0 blocks holding liquid are the head of the pool (act[0]) */
     if (m0.dAct[0].n > 0)
       hipLaunchKernelGGL(k_scene_rhs, dim3(m0.dAct[0].n), dim3(256), 0, d.stream, *scene, m0.dAct[0].ptr, m0.d_off,
-                         g.nbx, g.nby, g.bsx0, m0.d_ch[C_B]);
+                         g.nbx, g.nby, g.bsx0, m0.d_ch[C_B], 0);
     HIPCHK(hipGetLastError());
   } else if (!rhs.empty()) {
     HIPCHK(hipMemcpyAsync(m0.d_ch[C_B], rhs.data(), rhs.size() * sizeof(float), hipMemcpyHostToDevice, d.stream));
@@ -916,8 +977,15 @@ static void launchRelaxB(const Grid &g, int L, const DevList &l, float *X, const
                          hipStream_t s)
 {
   if (l.n == 0) return;
-  hipLaunchKernelGGL(k_relax_col<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, g.omega,
-                     reverse, done);
+  const LevelArgs a = levelArgs(g, L);
+  if constexpr (B <= 16) {
+    /* LDS-tile variant (FP32 operands only); the autotuner may pick it per block size */
+    if (g.smootherVariant[__builtin_ctz(g.bsx0 / B)] == 1 && a.prec == MSBGX_PREC_FP32) {
+      hipLaunchKernelGGL(k_relax<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, a, l.ptr, X, Bv, g.omega, reverse, 5, done);
+      return;
+    }
+  }
+  hipLaunchKernelGGL(k_relax_col<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, a, l.ptr, X, Bv, g.omega, reverse, done);
 }
 
 static void launchRelaxLev(const Grid &g, int L, int lev, const DevList &l, float *X, const float *Bv, int reverse,
@@ -1105,7 +1173,8 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
   HIPCHK(hipGetLastError());
 
   const bool useGraph = !(g.options & MSBGX_OPT_NO_GRAPH);
-  const int chunk = 4;
+  const int chunk = std::max(1, g.graphChunk);
+  if (d.graph && d.graphChunk != chunk) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
   PcgState hs{};
   for (int done_it = 0; done_it < p.max_iter;) {
     const int n = std::min(chunk, p.max_iter - done_it);
@@ -1117,6 +1186,7 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
         HIPCHK(hipStreamEndCapture(d.stream, &graph));
         HIPCHK(hipGraphInstantiate(&d.graph, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
+        d.graphChunk = chunk;
       }
       HIPCHK(hipGraphLaunch(d.graph, d.stream));
     } else {
@@ -1498,6 +1568,365 @@ int deviceSolveRefined(Grid &g, const msbgx_solve_params &p, double tol, int max
   *outer = nOuter;
   *inner = nInner;
   *rel = relNow;
+  return MSBGX_OK;
+}
+
+/* ------------------------------------------------------ batched block sets */
+
+/* Chunked PCG vector kernels: workgroup c covers cells [beg[c], end[c]) of
+ * problem prob[c]; one FP64 partial per chunk (fixed order per problem). */
+__global__ void __launch_bounds__(kVecThreads) k_copy_norm_c(const float *__restrict__ b, float *__restrict__ r,
+                                                             const int64_t *beg, const int64_t *end, double *partials)
+{
+  __shared__ double red[kVecThreads / 64];
+  double acc = 0.0;
+  for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x) {
+    const float v = b[i];
+    r[i] = v;
+    acc += (double)v * (double)v;
+  }
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_dot_c(const float *__restrict__ a, const float *__restrict__ b,
+                                                       const int64_t *beg, const int64_t *end, double *partials)
+{
+  __shared__ double red[kVecThreads / 64];
+  double acc = 0.0;
+  for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_update_xr_c(float *__restrict__ x, float *__restrict__ r,
+                                                             const float *__restrict__ d, const float *__restrict__ q,
+                                                             const int *prob, const int64_t *beg, const int64_t *end,
+                                                             const PcgState *st, double *partials)
+{
+  __shared__ double red[kVecThreads / 64];
+  const PcgState &s = st[prob[blockIdx.x]];
+  double acc = 0.0;
+  if (!s.done) {
+    const float alpha = s.alpha;
+    for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x) {
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_axpy_c(float *__restrict__ d, const float *__restrict__ z,
+                                                        const int *prob, const int64_t *beg, const int64_t *end,
+                                                        const PcgState *st)
+{
+  const PcgState &s = st[prob[blockIdx.x]];
+  if (s.done) return;
+  const float beta = s.beta;
+  for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x) d[i] = fmaf(beta, d[i], z[i]);
+}
+
+static BatchDev *batchDev(Grid &g) { return ((Dev *)g.dev)->batch; }
+
+int deviceUploadBatch(Grid &m, const std::vector<const Grid *> &probs)
+{
+  std::vector<float> noRhs;
+  if (int rc = deviceUpload(m, noRhs, nullptr)) return rc;
+  Dev &d = dev(m);
+  const int nb = m.nBlocks;
+  if (int rc = dalloc(&d.nbr, (size_t)nb * 6)) return rc;
+  if (int rc = dalloc(&d.probOf, nb)) return rc;
+  if (int rc = dalloc(&d.omegaOf, m.nProb)) return rc;
+  HIPCHK(hipMemcpy(d.nbr, m.nbrTable.data(), (size_t)nb * 6 * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.probOf, m.probOf.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d.omegaOf, m.probOmega.data(), m.nProb * sizeof(float), hipMemcpyHostToDevice));
+  /* right-hand sides: scene problems on the device, others from their host copy */
+  MgLevel &m0 = m.mg[0];
+  std::vector<float> hostB;
+  bool anyHost = false;
+  for (int q = 0; q < m.nProb; q++) anyHost |= !probs[q]->hasScene;
+  if (anyHost) hostB.assign(m0.ncells, 0.0f);
+  int *dl = nullptr;
+  if (int rc = dalloc(&dl, nb)) return rc;
+  for (int q = 0; q < m.nProb; q++) {
+    const Grid &p = *probs[q];
+    const int s = m.probBlock0[q];
+    if (p.hasScene) {
+      std::vector<int> l;
+      for (int b : p.mg[0].act[0]) l.push_back(s + b);
+      if (l.empty()) continue;
+      HIPCHK(hipMemcpy(dl, l.data(), l.size() * sizeof(int), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_scene_rhs, dim3((int)l.size()), dim3(256), 0, d.stream, p.scene, dl, m0.d_off, p.nbx, p.nby,
+                         m.bsx0, m0.d_ch[C_B], s);
+      HIPCHK(hipStreamSynchronize(d.stream));
+    } else if (!p.hostRhs.empty()) {
+      const int64_t n = (int64_t)m.bsx0 * m.bsx0 * m.bsx0;
+      for (int b : p.active0)
+        std::copy(p.hostRhs.begin() + p.mg[0].off[b], p.hostRhs.begin() + p.mg[0].off[b] + n,
+                  hostB.begin() + m0.off[s + b]);
+    }
+  }
+  (void)hipFree(dl);
+  if (anyHost) {
+    /* merge with the device-computed scene blocks: upload only host-problem blocks */
+    const int64_t n = (int64_t)m.bsx0 * m.bsx0 * m.bsx0;
+    for (int q = 0; q < m.nProb; q++) {
+      if (probs[q]->hasScene) continue;
+      for (int b : probs[q]->active0) {
+        const int64_t o = m0.off[m.probBlock0[q] + b];
+        HIPCHK(hipMemcpy(m0.d_ch[C_B] + o, hostB.data() + o, n * sizeof(float), hipMemcpyHostToDevice));
+      }
+    }
+  }
+  /* chunk table: each problem's PCG range split into chunks of <= 64K cells */
+  BatchDev *bd = new BatchDev();
+  d.batch = bd;
+  std::vector<int> cp, cbv, cev;
+  std::vector<int64_t> beg, end;
+  const int64_t bcells = (int64_t)m.bsx0 * m.bsx0 * m.bsx0, CH = 65536;
+  int64_t pos = 0;
+  for (int q = 0; q < m.nProb; q++) {
+    cbv.push_back((int)cp.size());
+    const int64_t len = (int64_t)m.probActive0[q] * bcells;
+    for (int64_t o = 0; o < len; o += CH) {
+      cp.push_back(q);
+      beg.push_back(pos + o);
+      end.push_back(pos + std::min(len, o + CH));
+    }
+    if (len == 0) {  // keep one empty chunk so the problem has a partial
+      cp.push_back(q);
+      beg.push_back(pos);
+      end.push_back(pos);
+    }
+    cev.push_back((int)cp.size());
+    pos += len;
+  }
+  bd->nChunks = (int)cp.size();
+  if (int rc = dalloc(&bd->cProb, cp.size())) return rc;
+  if (int rc = dalloc(&bd->cBeg, beg.size())) return rc;
+  if (int rc = dalloc(&bd->cEnd, end.size())) return rc;
+  if (int rc = dalloc(&bd->cb, m.nProb)) return rc;
+  if (int rc = dalloc(&bd->ce, m.nProb)) return rc;
+  if (int rc = dalloc(&bd->states, m.nProb)) return rc;
+  if (int rc = dalloc(&bd->allDone, 1)) return rc;
+  HIPCHK(hipMemcpy(bd->cProb, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bd->cBeg, beg.data(), beg.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bd->cEnd, end.data(), end.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bd->cb, cbv.data(), cbv.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bd->ce, cev.data(), cev.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (d.partialsCap < bd->nChunks) {
+    (void)hipFree(d.partials);
+    d.partials = nullptr;
+    if (int rc = dalloc(&d.partials, bd->nChunks)) return rc;
+    d.partialsCap = bd->nChunks;
+  }
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+/* allDone = every problem's PCG has stopped (one workgroup). */
+__global__ void __launch_bounds__(256) k_all_done(const PcgState *st, int n, int *allDone)
+{
+  __shared__ int anyLive;
+  if (threadIdx.x == 0) anyLive = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < n; p += blockDim.x)
+    if (!st[p].done) anyLive = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) allDone[0] = anyLive ? 0 : 1;
+}
+
+static void finB(Grid &m, int op, double *out, int stride)
+{
+  Dev &d = dev(m);
+  BatchDev &b = *batchDev(m);
+  hipLaunchKernelGGL(k_finalize_b, dim3(m.nProb), dim3(256), 0, d.stream, b.states, d.partials, b.cb, b.ce, op, out,
+                     stride);
+}
+
+/* One batched PCG iteration: every launch covers all problems. */
+static void pcgIterationBatch(Grid &m, const msbgx_solve_params &p)
+{
+  Dev &d = dev(m);
+  BatchDev &b = *batchDev(m);
+  MgLevel &m0 = m.mg[0];
+  const int *nd = b.allDone;
+  multiplyLaunch(m, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], nullptr, nd, d.stream);
+  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Q], b.cBeg, b.cEnd,
+                     d.partials);
+  finB(m, FIN_ALPHA, nullptr, 0);
+  hipLaunchKernelGGL(k_update_xr_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], b.cProb, b.cBeg, b.cEnd, b.states, d.partials);
+  finB(m, FIN_CONV, b.hist, b.histStride);
+  hipLaunchKernelGGL(k_all_done, dim3(1), dim3(256), 0, d.stream, b.states, m.nProb, b.allDone);
+  zeroActive(m, 0, m0.d_ch[C_Z], nd, d.stream);
+  vcycleLaunch(m, 0, p.n_pre, p.n_post, p.n_coarse, nd, d.stream);
+  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], b.cBeg, b.cEnd,
+                     d.partials);
+  finB(m, FIN_BETA, nullptr, 0);
+  hipLaunchKernelGGL(k_axpy_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], b.cProb,
+                     b.cBeg, b.cEnd, b.states);
+}
+
+int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *rel)
+{
+  Dev &d = dev(m);
+  BatchDev &b = *batchDev(m);
+  MgLevel &m0 = m.mg[0];
+  const int P = m.nProb;
+  if (b.histStride < p.max_iter + 1) {
+    if (b.hist) (void)hipFree(b.hist);
+    b.hist = nullptr;
+    if (int rc = dalloc(&b.hist, (size_t)P * (p.max_iter + 1))) return rc;
+    b.histStride = p.max_iter + 1;
+    if (b.graph) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
+  }
+  std::vector<PcgState> init(P);
+  for (auto &s : init) {
+    s = PcgState{};
+    s.tolSq = (double)p.tol * (double)p.tol;
+    s.maxIter = p.max_iter;
+  }
+  HIPCHK(hipMemcpyAsync(b.states, init.data(), P * sizeof(PcgState), hipMemcpyHostToDevice, d.stream));
+  HIPCHK(hipMemcpyAsync(b.allDone, d.zeroFlag, sizeof(int), hipMemcpyDeviceToDevice, d.stream));
+  HIPCHK(hipEventRecord(d.ev0, d.stream));
+  const int *nd = d.zeroFlag;
+  hipLaunchKernelGGL(k_copy_norm_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_B], m0.d_ch[C_R], b.cBeg,
+                     b.cEnd, d.partials);
+  finB(m, FIN_BNORM, nullptr, 0);
+  zeroActive(m, 0, m0.d_ch[C_Z], nd, d.stream);
+  vcycleLaunch(m, 0, p.n_pre, p.n_post, p.n_coarse, nd, d.stream);
+  for (int lev = 0; lev < m.nLevels; lev++)
+    if (m0.dAct[lev].n)
+      hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
+                         m0.d_off, m.bsx0, m0.d_ch[C_Z], m0.d_ch[C_D], nd);
+  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], b.cBeg, b.cEnd,
+                     d.partials);
+  finB(m, FIN_RHO0, nullptr, 0);
+  hipLaunchKernelGGL(k_all_done, dim3(1), dim3(256), 0, d.stream, b.states, P, b.allDone);
+  HIPCHK(hipGetLastError());
+  const int chunk = std::max(1, m.graphChunk);
+  const bool useGraph = !(m.options & MSBGX_OPT_NO_GRAPH);
+  if (b.graph && b.graphChunk != chunk) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
+  std::vector<PcgState> hs(P);
+  for (int it = 0; it < p.max_iter;) {
+    const int n = std::min(chunk, p.max_iter - it);
+    if (useGraph && n == chunk) {
+      if (!b.graph) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < chunk; i++) pcgIterationBatch(m, p);
+        HIPCHK(hipStreamEndCapture(d.stream, &graph));
+        HIPCHK(hipGraphInstantiate(&b.graph, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        b.graphChunk = chunk;
+      }
+      HIPCHK(hipGraphLaunch(b.graph, d.stream));
+    } else {
+      for (int i = 0; i < n; i++) pcgIterationBatch(m, p);
+    }
+    it += n;
+    HIPCHK(hipMemcpyAsync(hs.data(), b.states, P * sizeof(PcgState), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    bool all = true;
+    for (auto &s : hs) all &= s.done != 0;
+    if (all) break;
+  }
+  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_R], b.cBeg, b.cEnd,
+                     d.partials);
+  finB(m, FIN_FINAL, b.hist + p.max_iter, b.histStride);
+  HIPCHK(hipEventRecord(d.ev1, d.stream));
+  HIPCHK(hipMemcpyAsync(hs.data(), b.states, P * sizeof(PcgState), hipMemcpyDeviceToHost, d.stream));
+  std::vector<double> fin(P);
+  for (int q = 0; q < P; q++)
+    HIPCHK(hipMemcpyAsync(&fin[q], b.hist + (size_t)q * b.histStride + p.max_iter, sizeof(double),
+                          hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+  m.lastSolveMs = ms;
+  std::vector<double> hh((size_t)P * b.histStride);
+  HIPCHK(hipMemcpy(hh.data(), b.hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  m.probHistory.assign(P, {});
+  for (int q = 0; q < P; q++) {
+    for (int i = 0; i < hs[q].iter; i++)
+      m.probHistory[q].push_back(std::sqrt(hh[(size_t)q * b.histStride + i] / hs[q].bnorm2));
+    if (hs[q].brk == 2) { iters[q] = 0; rel[q] = 0.0f; continue; }
+    rel[q] = (float)std::sqrt(fin[q] / hs[q].bnorm2);
+    iters[q] = hs[q].conv > 0 ? hs[q].conv : p.max_iter;
+  }
+  return MSBGX_OK;
+}
+
+/* ------------------------------------------------------------- autotuner */
+
+/* Picks, by measurement on this handle's own problem:
+ *  - the smoother kernel per block size (column-resident or LDS tile; the
+ *    tile form exists for FP32 operands and blocks up to 16^3), timing the
+ *    two colour passes over every MG level's relaxation lists;
+ *  - the number of PCG iterations per hipGraph replay (1, 2, 4, 8, 16),
+ *    timing a whole solve (its second run, so graph capture is excluded).
+ * Both choices reorder nothing inside a kernel, so the solver's results are
+ * bit-identical for every choice.  The pressure channel is zero afterwards. */
+int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *variantMask)
+{
+  Dev &d = dev(g);
+  const int reps = 10;
+  int mask = 0;
+  for (int lev = 0; lev < g.nLevels; lev++) {
+    g.smootherVariant[lev] = 0;
+    const int B = g.bsx0 >> lev;
+    if (B > 16 || g.prec != MSBGX_PREC_FP32) continue;
+    int nBlocks = 0;
+    for (int L = 0; L <= lev; L++) nBlocks += g.mg[L].dRelax[0][lev].n + g.mg[L].dRelax[1][lev].n;
+    if (nBlocks == 0) continue;
+    float t[2] = {0.f, 0.f};
+    for (int v = 0; v < 2; v++) {
+      g.smootherVariant[lev] = v;
+      for (int rep = -2; rep < reps; rep++) {
+        if (rep == 0) HIPCHK(hipEventRecord(d.ev0, d.stream));
+        for (int L = 0; L <= lev; L++)
+          for (int c = 0; c < 2; c++)
+            launchRelaxLev(g, L, lev, g.mg[L].dRelax[c][lev], g.mg[L].d_ch[C_Z], g.mg[L].d_ch[C_R], 0, d.zeroFlag,
+                           d.stream);
+      }
+      HIPCHK(hipEventRecord(d.ev1, d.stream));
+      HIPCHK(hipStreamSynchronize(d.stream));
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventElapsedTime(&t[v], d.ev0, d.ev1));
+    }
+    g.smootherVariant[lev] = t[1] < 0.95f * t[0] ? 1 : 0;
+    if (g.smootherVariant[lev]) mask |= 1 << lev;
+  }
+  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  if (d.batch && d.batch->graph) { (void)hipGraphExecDestroy(d.batch->graph); d.batch->graph = nullptr; }
+
+  const int P = g.batched ? g.nProb : 1;
+  std::vector<int> it(P);
+  std::vector<float> rel(P);
+  static const int cands[] = {1, 2, 4, 8, 16};
+  int best = g.graphChunk;
+  double bestMs = 1e30;
+  for (int c : cands) {
+    g.graphChunk = c;
+    double ms = 0.0;
+    for (int run = 0; run < 2; run++) {
+      if (int rc = deviceZero(g, C_X, 0)) return rc;
+      const int rc = g.batched ? deviceSolveBatch(g, p, it.data(), rel.data()) : deviceSolve(g, p, it.data(), rel.data());
+      if (rc) return rc;
+      ms = g.lastSolveMs;
+    }
+    if (ms < bestMs) { bestMs = ms; best = c; }
+  }
+  g.graphChunk = best;
+  if (int rc = deviceZero(g, C_X, 0)) return rc;
+  *chunkOut = best;
+  *variantMask = mask;
   return MSBGX_OK;
 }
 

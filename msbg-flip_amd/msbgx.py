@@ -101,6 +101,23 @@ def config_c2(n=512, bsx=32, seed=1234):
     return Config("c2_slab_spheres_multires", n, bsx, 3, s, level_mode=1, level_d0=2)
 
 
+def config_c3(seed=1234, n_problems=64):
+    """Batched form: n_problems independent scenes, sizes from {64,128,192,256}^3,
+    16^3 blocks, 2 MG levels, each problem either single-resolution (every block
+    level 0) or two-resolution (level 0 within one block of liquid); 1-8 spheres
+    with radius ~U[0.1,0.3] and centre ~U[0.2,0.8]^3; velocity seed per problem."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_problems):
+        n = int(rng.choice([64, 128, 192, 256]))
+        k = int(rng.integers(1, 9))
+        sph = [(*rng.uniform(0.2, 0.8, 3), rng.uniform(0.1, 0.3)) for _ in range(k)]
+        mode = -1 if rng.integers(0, 2) == 0 else 0
+        s = make_scene(n, spheres=sph, seed=seed * 1000 + i)
+        out.append(Config(f"c3_problem{i}_{n}", n, 16, 2, s, level_mode=mode, level_d0=1))
+    return out
+
+
 def config_c4(n=256, seed=1234):
     """Dam break: box x<0.4, y<0.6 plus a sphere r~U[0.1,0.2]."""
     rng = np.random.default_rng(seed)
@@ -155,6 +172,13 @@ def _load():
         "msbgx_solve_pressure_refined": (C.c_int, [P, C.POINTER(SolveParams), C.c_double, C.c_int,
                                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), P]),
         "msbgx_level0_blocks": (C.c_int, [P, P, C.c_int]),
+        "msbgx_batch_create": (C.c_int, [C.POINTER(P), P, C.c_int]),
+        "msbgx_batch_size": (C.c_int, [P]),
+        "msbgx_batch_block_base": (C.c_int, [P, C.c_int]),
+        "msbgx_batch_solve": (C.c_int, [P, C.POINTER(SolveParams), P, P]),
+        "msbgx_batch_residual_history": (C.c_int, [P, C.c_int, P, C.c_int]),
+        "msbgx_autotune": (C.c_int, [P, C.POINTER(SolveParams), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "msbgx_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -173,7 +197,9 @@ EXPORTED = [
     "msbgx_set_channel_block", "msbgx_block_info", "msbgx_relax_list", "msbgx_omega", "msbgx_relax",
     "msbgx_multiply", "msbgx_downsample", "msbgx_prolongate", "msbgx_zero_channel", "msbgx_copy_channel",
     "msbgx_vcycle", "msbgx_time_kernel", "msbgx_level_map_from_scene", "msbgx_set_precision",
-    "msbgx_solve_pressure_refined", "msbgx_level0_blocks",
+    "msbgx_solve_pressure_refined", "msbgx_level0_blocks", "msbgx_batch_create", "msbgx_batch_size",
+    "msbgx_batch_block_base", "msbgx_batch_solve", "msbgx_batch_residual_history", "msbgx_autotune",
+    "msbgx_set_launch_config",
 ]
 
 
@@ -343,6 +369,47 @@ class MultiresSparseGrid:
 
     def vcycle(self, n_pre=3, n_post=3, n_coarse=24):
         _ck(lib.msbgx_vcycle(self._h, n_pre, n_post, n_coarse), "vcycle")
+
+    def autotune(self, n_pre=3, n_post=3, n_coarse=24, max_iter=500, tol=1e-6):
+        """Measure and keep the fastest graph chunk and smoother kernel per block
+        size; returns (graph_chunk, smoother_mask).  The pressure is zero after."""
+        p = SolveParams(n_pre, n_post, n_coarse, max_iter, tol)
+        c, m = C.c_int(), C.c_int()
+        _ck(lib.msbgx_autotune(self._h, C.byref(p), C.byref(c), C.byref(m)), "autotune")
+        return c.value, m.value
+
+    def set_launch_config(self, graph_chunk, smoother_mask):
+        _ck(lib.msbgx_set_launch_config(self._h, graph_chunk, smoother_mask), "set_launch_config")
+
+
+class BatchGrid(MultiresSparseGrid):
+    """Batched block set over several prepared grids (``msbgx_batch_*``):
+    one launch per kernel covers every problem; per-problem PCG state."""
+
+    @classmethod
+    def create(cls, problems):
+        arr = (C.c_void_p * len(problems))(*[g._h for g in problems])
+        h = C.c_void_p()
+        _ck(lib.msbgx_batch_create(C.byref(h), arr, len(problems)), "batch_create")
+        b = cls(h)
+        b.bsx, b.n_levels = problems[0].bsx, problems[0].n_levels
+        b.n_blocks = lib.msbgx_num_blocks(h)
+        b.n_problems = len(problems)
+        b.bases = [lib.msbgx_batch_block_base(h, q) for q in range(len(problems))]
+        return b
+
+    def solve(self, n_pre=3, n_post=3, n_coarse=24, max_iter=500, tol=5e-4):
+        p = SolveParams(n_pre, n_post, n_coarse, max_iter, tol)
+        it = np.zeros(self.n_problems, np.int32)
+        rr = np.zeros(self.n_problems, np.float32)
+        _ck(lib.msbgx_batch_solve(self._h, C.byref(p), _ptr(it), _ptr(rr)), "batch_solve")
+        return it, rr
+
+    def problem_history(self, q):
+        n = lib.msbgx_batch_residual_history(self._h, q, None, 0)
+        out = np.zeros(max(n, 1))
+        lib.msbgx_batch_residual_history(self._h, q, _ptr(out), n)
+        return out[:n]
 
 
 def build_grid(cfg: Config, walls=DBC_SOLID):
