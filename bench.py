@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-precision", action="store_true", help="skip the FP16/BF16 precision-form runs")
     ap.add_argument("--no-fp64-oracle", action="store_true", help="skip the CPU FP64 reference solve")
     ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form (64 problems) runs")
+    ap.add_argument("--batch-max-iter", type=int, default=400)
     ap.add_argument("--no-graph", action="store_true", help="launch PCG iterations eagerly (no hipGraph)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="only run a block copy of known size plus a few smoother passes (for rocprofv3 --pmc)")
@@ -149,6 +151,52 @@ def precision_runs(msbgx, g, cfg, args, log):
         out[name] = rec
     g.set_precision(msbgx.PREC_FP32)
     return out
+
+
+def batched_runs(msbgx, args, log):
+    """Batched form (C3 workload): 64 independent problems, sizes from
+    {64,128,192,256}^3, 16^3 blocks, 2 MG levels, each solved to 1e-6 (at most
+    --batch-max-iter iterations) -- one after the other on their own handles,
+    then all together as one batched block set after the autotuner has picked
+    the launch shape from 40-iteration trial solves."""
+    t0 = time.time()
+    cfgs = msbgx.config_c3(seed=1234, n_problems=64)
+    mi = args.batch_max_iter
+    singles = []
+    for c in cfgs:
+        g = msbgx.build_grid(c)
+        g.prepare_pressure_solve_flip(scene=c.scene)
+        singles.append(g)
+    log(f"batched: prepared {len(cfgs)} problems in {time.time() - t0:.1f} s")
+    seq_ms, seq_it = 0.0, []
+    for k, g in enumerate(singles):
+        g.solve_pressure_flip(max_iter=mi, tol=args.tol)   # warm-up (graph capture)
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, _ = g.solve_pressure_flip(max_iter=mi, tol=args.tol)
+        seq_ms += g.last_solve_ms()
+        seq_it.append(it)
+        if k % 8 == 7:
+            log(f"batched: sequential solves {k + 1}/{len(singles)}, {seq_ms:.1f} ms so far")
+    b = msbgx.BatchGrid.create(singles)
+    for g in singles:
+        g.destroy()
+    chunk, mask = b.autotune(max_iter=40, tol=0.0)
+    log(f"batched: autotune chunk {chunk} mask {mask}")
+    b.zero_channel(msbgx.CH_PRESSURE, 0)
+    it, rr = b.solve(max_iter=mi, tol=args.tol)
+    ms = b.last_solve_ms()
+    cells = sum(int(c.n) ** 3 for c in cfgs)
+    conv = int((it < mi).sum())
+    rec = {"problems": len(cfgs), "sizes": [c.n for c in cfgs], "effective_cells": cells,
+           "max_iter": mi, "converged_problems": conv,
+           "batched_ms": round(ms, 3), "sequential_ms": round(seq_ms, 3),
+           "speedup_vs_sequential": round(seq_ms / ms, 2),
+           "problems_per_s": round(len(cfgs) / (ms * 1e-3), 1),
+           "iterations": [int(v) for v in it], "iterations_equal_to_individual": bool((it == seq_it).all()),
+           "autotune": {"graph_chunk": chunk, "smoother_tile_mask": mask}}
+    log(f"batched: {ms:.1f} ms batched vs {seq_ms:.1f} ms sequential, {conv}/{len(cfgs)} converged")
+    b.destroy()
+    return rec
 
 
 def read_traffic(kernel, workload):
@@ -270,6 +318,8 @@ def main():
     }
     if not args.no_precision:
         line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
+    if not args.no_batch:
+        line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)

@@ -75,6 +75,8 @@ typedef struct msbgx_solve_params {
                                           coarser level has no relaxation block             */
 #define MSBGX_OPT_NO_GRAPH 0x2u        /* launch PCG iterations eagerly instead of replaying a
                                           captured hipGraph                                  */
+#define MSBGX_OPT_NO_COMPACT 0x4u      /* batched solves: keep converged problems' blocks in the
+                                          launches instead of dropping them                  */
 
 const char *msbgx_last_error(void);
 void msbgx_default_solve_params(msbgx_solve_params *p);

@@ -785,6 +785,14 @@ struct BatchDev {
   int nChunks = 0;
   int *cProb = nullptr, *cb = nullptr, *ce = nullptr;
   int64_t *cBeg = nullptr, *cEnd = nullptr;
+  /* the same table restricted to problems still iterating (live), rebuilt
+   * with the block lists when enough problems have converged */
+  int nChunksL = 0;
+  int *cProbL = nullptr, *cbL = nullptr, *ceL = nullptr;
+  int64_t *cBegL = nullptr, *cEndL = nullptr;
+  std::vector<int> hProb, hCb, hCe;
+  std::vector<int64_t> hBeg, hEnd;
+  int listBlocks = 0;       // blocks in the current (possibly compacted) lists
   PcgState *states = nullptr;
   int *allDone = nullptr;   // 1 once every problem has converged or stopped
   double *hist = nullptr;
@@ -854,6 +862,7 @@ void deviceFree(Grid &g)
     if (BatchDev *b = d.batch) {
       if (b->graph) (void)hipGraphExecDestroy(b->graph);
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
+      f(b->cProbL); f(b->cbL); f(b->ceL); f(b->cBegL); f(b->cEndL);
       delete b;
     }
     if (d.ev0) (void)hipEventDestroy(d.ev0);
@@ -886,6 +895,36 @@ static float *chanPtr(const Grid &g, int ch, int L)
 {
   if (ch < 0 || ch >= N_VEC) return nullptr;
   return g.mg[L].d_ch[ch];
+}
+
+/* Device block lists of MG level L: act[lev] ..., relaxCL[c][lev] ... in one
+ * buffer.  With `live` (batched block sets), only blocks of problems whose
+ * live flag is set are kept; the buffer is sized for the full lists. */
+static int uploadLists(Grid &g, int L, const std::vector<char> *live)
+{
+  MgLevel &m = g.mg[L];
+  auto keep = [&](int b) { return !live || (*live)[g.probOf[b]]; };
+  std::vector<int> all;
+  std::vector<std::pair<int, int>> spans;
+  size_t full = 0;
+  auto add = [&](const std::vector<int> &v) {
+    const int s0 = (int)all.size();
+    for (int b : v)
+      if (keep(b)) all.push_back(b);
+    spans.push_back({s0, (int)all.size() - s0});
+    full += v.size();
+  };
+  for (int l = 0; l < kMaxLevels; l++) add(m.act[l]);
+  for (int c = 0; c < 2; c++)
+    for (int l = 0; l < kMaxLevels; l++) add(m.relaxCL[c][l]);
+  if (!m.d_lists)
+    if (int rc = dalloc(&m.d_lists, full)) return rc;
+  if (!all.empty()) HIPCHK(hipMemcpy(m.d_lists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  int si = 0;
+  for (int l = 0; l < kMaxLevels; l++, si++) m.dAct[l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+  for (int c = 0; c < 2; c++)
+    for (int l = 0; l < kMaxLevels; l++, si++) m.dRelax[c][l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+  return MSBGX_OK;
 }
 
 int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scene)
@@ -933,21 +972,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
       if (int rc = dalloc(&m.d_ch[cl[i]], nc)) return rc;
       HIPCHK(hipMemsetAsync(m.d_ch[cl[i]], 0, nc * sizeof(float), d.stream));
     }
-    /* lists: act[lev] ..., relaxCL[c][lev] ..., actAll */
-    std::vector<int> all;
-    std::vector<std::pair<int, int>> spans;
-    for (int l = 0; l < kMaxLevels; l++) { spans.push_back({(int)all.size(), (int)m.act[l].size()}); all.insert(all.end(), m.act[l].begin(), m.act[l].end()); }
-    for (int c = 0; c < 2; c++)
-      for (int l = 0; l < kMaxLevels; l++) {
-        spans.push_back({(int)all.size(), (int)m.relaxCL[c][l].size()});
-        all.insert(all.end(), m.relaxCL[c][l].begin(), m.relaxCL[c][l].end());
-      }
-    if (int rc = dalloc(&m.d_lists, all.size())) return rc;
-    if (!all.empty()) HIPCHK(hipMemcpy(m.d_lists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
-    int si = 0;
-    for (int l = 0; l < kMaxLevels; l++, si++) m.dAct[l] = DevList{m.d_lists + spans[si].first, spans[si].second};
-    for (int c = 0; c < 2; c++)
-      for (int l = 0; l < kMaxLevels; l++, si++) m.dRelax[c][l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+    if (int rc = uploadLists(g, L, nullptr)) return rc;
   }
   d.b0Valid = false;
   if (d.dActive0) { (void)hipFree(d.dActive0); d.dActive0 = nullptr; }
@@ -1712,6 +1737,12 @@ int deviceUploadBatch(Grid &m, const std::vector<const Grid *> &probs)
   if (int rc = dalloc(&bd->cEnd, end.size())) return rc;
   if (int rc = dalloc(&bd->cb, m.nProb)) return rc;
   if (int rc = dalloc(&bd->ce, m.nProb)) return rc;
+  if (int rc = dalloc(&bd->cProbL, cp.size())) return rc;
+  if (int rc = dalloc(&bd->cBegL, beg.size())) return rc;
+  if (int rc = dalloc(&bd->cEndL, end.size())) return rc;
+  if (int rc = dalloc(&bd->cbL, m.nProb)) return rc;
+  if (int rc = dalloc(&bd->ceL, m.nProb)) return rc;
+  bd->hProb = cp; bd->hBeg = beg; bd->hEnd = end; bd->hCb = cbv; bd->hCe = cev;
   if (int rc = dalloc(&bd->states, m.nProb)) return rc;
   if (int rc = dalloc(&bd->allDone, 1)) return rc;
   HIPCHK(hipMemcpy(bd->cProb, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1719,6 +1750,7 @@ int deviceUploadBatch(Grid &m, const std::vector<const Grid *> &probs)
   HIPCHK(hipMemcpy(bd->cEnd, end.data(), end.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(bd->cb, cbv.data(), cbv.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(bd->ce, cev.data(), cev.size() * sizeof(int), hipMemcpyHostToDevice));
+  bd->listBlocks = -1;   // forces setLive() to upload the full live table on the first solve
   if (d.partialsCap < bd->nChunks) {
     (void)hipFree(d.partials);
     d.partials = nullptr;
@@ -1741,12 +1773,51 @@ __global__ void __launch_bounds__(256) k_all_done(const PcgState *st, int n, int
   if (threadIdx.x == 0) allDone[0] = anyLive ? 0 : 1;
 }
 
-static void finB(Grid &m, int op, double *out, int stride)
+static void finB(Grid &m, int op, double *out, int stride, bool live = false)
 {
   Dev &d = dev(m);
   BatchDev &b = *batchDev(m);
-  hipLaunchKernelGGL(k_finalize_b, dim3(m.nProb), dim3(256), 0, d.stream, b.states, d.partials, b.cb, b.ce, op, out,
-                     stride);
+  hipLaunchKernelGGL(k_finalize_b, dim3(m.nProb), dim3(256), 0, d.stream, b.states, d.partials, live ? b.cbL : b.cb,
+                     live ? b.ceL : b.ce, op, out, stride);
+}
+
+/* Restrict the block lists of every MG level and the live chunk table to
+ * the problems with live[q] set (all problems when live is empty).  The
+ * captured graph holds the old list lengths, so it is dropped. */
+static int setLive(Grid &m, const std::vector<char> &live)
+{
+  Dev &d = dev(m);
+  BatchDev &b = *batchDev(m);
+  HIPCHK(hipStreamSynchronize(d.stream));
+  std::vector<char> all(m.nProb, 1);
+  const std::vector<char> &lv = live.empty() ? all : live;
+  for (int L = 0; L < m.nLevels; L++)
+    if (int rc = uploadLists(m, L, &lv)) return rc;
+  std::vector<int> cp, cbv(m.nProb, 0), cev(m.nProb, 0);
+  std::vector<int64_t> beg, end;
+  int blocks = 0;
+  for (int q = 0; q < m.nProb; q++) {
+    if (!lv[q]) continue;
+    cbv[q] = (int)cp.size();
+    for (int c = b.hCb[q]; c < b.hCe[q]; c++) {
+      cp.push_back(q);
+      beg.push_back(b.hBeg[c]);
+      end.push_back(b.hEnd[c]);
+    }
+    cev[q] = (int)cp.size();
+    blocks += (q + 1 < m.nProb ? m.probBlock0[q + 1] : m.nBlocks) - m.probBlock0[q];
+  }
+  b.nChunksL = (int)cp.size();
+  if (!cp.empty()) {
+    HIPCHK(hipMemcpy(b.cProbL, cp.data(), cp.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.cBegL, beg.data(), beg.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.cEndL, end.data(), end.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(b.cbL, cbv.data(), cbv.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.ceL, cev.data(), cev.size() * sizeof(int), hipMemcpyHostToDevice));
+  b.listBlocks = blocks;
+  if (b.graph) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
+  return MSBGX_OK;
 }
 
 /* One batched PCG iteration: every launch covers all problems. */
@@ -1756,21 +1827,22 @@ static void pcgIterationBatch(Grid &m, const msbgx_solve_params &p)
   BatchDev &b = *batchDev(m);
   MgLevel &m0 = m.mg[0];
   const int *nd = b.allDone;
+  const int nc = b.nChunksL;
   multiplyLaunch(m, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], nullptr, nd, d.stream);
-  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Q], b.cBeg, b.cEnd,
+  hipLaunchKernelGGL(k_dot_c, dim3(nc), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Q], b.cBegL, b.cEndL,
                      d.partials);
-  finB(m, FIN_ALPHA, nullptr, 0);
-  hipLaunchKernelGGL(k_update_xr_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
-                     m0.d_ch[C_D], m0.d_ch[C_Q], b.cProb, b.cBeg, b.cEnd, b.states, d.partials);
-  finB(m, FIN_CONV, b.hist, b.histStride);
+  finB(m, FIN_ALPHA, nullptr, 0, true);
+  hipLaunchKernelGGL(k_update_xr_c, dim3(nc), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], b.cProbL, b.cBegL, b.cEndL, b.states, d.partials);
+  finB(m, FIN_CONV, b.hist, b.histStride, true);
   hipLaunchKernelGGL(k_all_done, dim3(1), dim3(256), 0, d.stream, b.states, m.nProb, b.allDone);
   zeroActive(m, 0, m0.d_ch[C_Z], nd, d.stream);
   vcycleLaunch(m, 0, p.n_pre, p.n_post, p.n_coarse, nd, d.stream);
-  hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], b.cBeg, b.cEnd,
+  hipLaunchKernelGGL(k_dot_c, dim3(nc), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], b.cBegL, b.cEndL,
                      d.partials);
-  finB(m, FIN_BETA, nullptr, 0);
-  hipLaunchKernelGGL(k_axpy_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], b.cProb,
-                     b.cBeg, b.cEnd, b.states);
+  finB(m, FIN_BETA, nullptr, 0, true);
+  hipLaunchKernelGGL(k_axpy_c, dim3(nc), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], b.cProbL,
+                     b.cBegL, b.cEndL, b.states);
 }
 
 int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *rel)
@@ -1786,6 +1858,8 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
     b.histStride = p.max_iter + 1;
     if (b.graph) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
   }
+  if (b.listBlocks != m.nBlocks)
+    if (int rc = setLive(m, {})) return rc;   // every problem is live at the start
   std::vector<PcgState> init(P);
   for (auto &s : init) {
     s = PcgState{};
@@ -1836,6 +1910,16 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
     bool all = true;
     for (auto &s : hs) all &= s.done != 0;
     if (all) break;
+    /* drop converged problems from the launches once they hold a quarter of
+     * the listed blocks (their x and r are frozen already) */
+    std::vector<char> live(P);
+    int liveBlocks = 0;
+    for (int q = 0; q < P; q++) {
+      live[q] = hs[q].done ? 0 : 1;
+      if (live[q]) liveBlocks += (q + 1 < P ? m.probBlock0[q + 1] : m.nBlocks) - m.probBlock0[q];
+    }
+    if (!(m.options & MSBGX_OPT_NO_COMPACT) && 4 * liveBlocks <= 3 * b.listBlocks)
+      if (int rc = setLive(m, live)) return rc;
   }
   hipLaunchKernelGGL(k_dot_c, dim3(b.nChunks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_R], b.cBeg, b.cEnd,
                      d.partials);
@@ -1850,6 +1934,8 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
   m.lastSolveMs = ms;
+  if (b.listBlocks != m.nBlocks)
+    if (int rc = setLive(m, {})) return rc;   // leave the full lists for the other entry points
   std::vector<double> hh((size_t)P * b.histStride);
   HIPCHK(hipMemcpy(hh.data(), b.hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
   m.probHistory.assign(P, {});

@@ -45,15 +45,15 @@ def small_batch(n_problems, seed):
     return cfgs
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batched_problems_equal_individual_solves(use_graph):
-    print(f"batch vs individual, graph={use_graph}", flush=True)
+@pytest.mark.parametrize("opts", [0, msbgx.OPT_NO_GRAPH, msbgx.OPT_NO_COMPACT])
+def test_batched_problems_equal_individual_solves(opts):
+    """Default: converged problems leave the launches (list compaction)."""
+    print(f"batch vs individual, options={opts}", flush=True)
     cfgs = small_batch(6, seed=7)
     assert len(cfgs) == 6
     singles = [prepared(c) for c in cfgs]
     batch = msbgx.BatchGrid.create(singles)
-    if not use_graph:
-        batch.set_options(msbgx.OPT_NO_GRAPH)
+    batch.set_options(opts)
     it_b, rr_b = batch.solve(max_iter=200, tol=1e-6)
     for q, (c, g) in enumerate(zip(cfgs, singles)):
         it, rr = g.solve_pressure_flip(max_iter=200, tol=1e-6)
