@@ -165,6 +165,52 @@ int msbgx_autotune(msbgx_grid *g, const msbgx_solve_params *p, int *graph_chunk,
 /* Explicit settings for the same two knobs (graph_chunk in [1,64]). */
 int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask);
 
+/* Learned correction of the preconditioner (no reference counterpart).
+ * A 4-layer 3x3x3 convolutional network without biases (1 -> 16 -> 16 -> 16
+ * -> 1 channels, ReLU after the first three; BF16 weights and activations,
+ * FP32 accumulation) maps the residual the V-cycle leaves on a level-0 16^3
+ * block plus its one-voxel halo to an error estimate:
+ *   z = V(r) + N(r - A V(r)).
+ * Parameters, FP32, in this order: W1[27][16] (tap, out), W2[27][16][16],
+ * W3[27][16][16] (tap, in, out), W4[27][16] (tap, in); tap = 9 (dz+1) +
+ * 3 (dy+1) + (dx+1).  Grids with 16^3 base blocks only. */
+#define MSBGX_NET_PARAMS 14688
+typedef struct msbgx_net msbgx_net;
+/* He-normal initial weights from `seed`. */
+int msbgx_net_create(msbgx_net **out, uint64_t seed);
+void msbgx_net_destroy(msbgx_net *n);
+int msbgx_net_get_weights(msbgx_net *n, float *w);        /* MSBGX_NET_PARAMS floats */
+int msbgx_net_set_weights(msbgx_net *n, const float *w);  /* also resets the optimiser */
+/* Replace the training set by `count` host samples: x [count][18^3] (block
+ * plus halo, x fastest), target and mask [count][16^3]. */
+int msbgx_net_set_samples(msbgx_net *n, int count, const float *x, const float *target, const uint8_t *mask);
+/* Masked mean-square loss over the training set, its gradient (NULL to skip,
+ * MSBGX_NET_PARAMS floats) and the network outputs ([count][16^3], NULL to
+ * skip), computed by the GPU kernels used in training. */
+int msbgx_net_loss_grad(msbgx_net *n, double *loss, float *grad, float *out);
+/* Append training samples from the plain MG-PCG on a prepared grid: after
+ * every `every`-th iteration of a solve (params p), `blocks_per_pair` random
+ * level-0 PCG blocks give (input r - A V(r), target x* - x_k - V(r)) divided
+ * by the target's RMS on the block; x* is a prior solve to p->tol.  Returns the number of
+ * iterates sampled in *pairs.  The grid's pressure is zero afterwards. */
+int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, int every, int max_pairs,
+                      int blocks_per_pair, uint64_t seed, int *pairs);
+int msbgx_net_num_samples(const msbgx_net *n);
+/* Adam on random minibatches of `batch` samples; first-step and smoothed
+ * last-step losses; msbgx_net_last_train_ms gives the GPU time. */
+int msbgx_net_train(msbgx_net *n, int steps, int batch, float lr, uint64_t seed, double *loss_first,
+                    double *loss_last);
+double msbgx_net_last_train_ms(const msbgx_net *n);
+/* Per-kernel entry point: ch_out += N(ch_in) on the level-0 PCG blocks
+ * (MG level 0 channels; the halo is read from level-0 neighbours). */
+int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out);
+/* MG-PCG with the learned correction (flexible beta).  A non-finite network
+ * output or a residual that drops by less than 10% over 5 iterations turns
+ * the correction off for the rest of the solve and restarts the plain
+ * MG-PCG; *fallback_iter is the iteration where that happened, -1 if never. */
+int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_params *p, int *iters,
+                                 float *rel_residual, int *fallback_iter);
+
 /* Relative residual after each PCG iteration of the last solve (up to cap);
  * returns the history length (out may be NULL to query it). */
 int msbgx_residual_history(const msbgx_grid *g, double *out, int cap);

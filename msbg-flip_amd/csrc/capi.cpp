@@ -450,7 +450,96 @@ int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask)
   }
   g->graphChunk = graph_chunk;
   for (int l = 0; l < kMaxLevels; l++) g->smootherVariant[l] = (smoother_mask >> l) & 1;
+  deviceInvalidateGraphs(*g);
   return MSBGX_OK;
+}
+
+int msbgx_net_create(msbgx_net **out, uint64_t seed)
+{
+  if (!out) { setError("out is null"); return MSBGX_EINVAL; }
+  *out = nullptr;
+  Net *n = nullptr;
+  const int rc = netCreate(&n, seed);
+  if (rc) { netDestroy(n); return rc; }
+  *out = reinterpret_cast<msbgx_net *>(n);
+  return MSBGX_OK;
+}
+
+void msbgx_net_destroy(msbgx_net *n) { netDestroy(reinterpret_cast<Net *>(n)); }
+
+static Net *netOf(msbgx_net *n) { return reinterpret_cast<Net *>(n); }
+
+int msbgx_net_get_weights(msbgx_net *n, float *w)
+{
+  if (!n || !w) { setError("null argument"); return MSBGX_EINVAL; }
+  return netGetWeights(*netOf(n), w);
+}
+
+int msbgx_net_set_weights(msbgx_net *n, const float *w)
+{
+  if (!n || !w) { setError("null argument"); return MSBGX_EINVAL; }
+  return netSetWeights(*netOf(n), w);
+}
+
+int msbgx_net_set_samples(msbgx_net *n, int count, const float *x, const float *target, const uint8_t *mask)
+{
+  if (!n || count < 1 || !x || !target || !mask) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netSetSamples(*netOf(n), count, x, target, mask);
+}
+
+int msbgx_net_loss_grad(msbgx_net *n, double *loss, float *grad, float *out)
+{
+  if (!n || !loss) { setError("null argument"); return MSBGX_EINVAL; }
+  return netLossGradAll(*netOf(n), loss, grad, out);
+}
+
+static int checkLearnedGrid(msbgx_grid *g)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!g->prepared) { setError("not prepared"); return MSBGX_ESTATE; }
+  if (g->bsx0 != 16) { setError("the learned correction needs 16^3 base blocks"); return MSBGX_EINVAL; }
+  return MSBGX_OK;
+}
+
+int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, int every, int max_pairs,
+                      int blocks_per_pair, uint64_t seed, int *pairs)
+{
+  if (!n || !pairs) { setError("null argument"); return MSBGX_EINVAL; }
+  if (int rc = checkLearnedGrid(g)) return rc;
+  if (int rc = checkParams(p)) return rc;
+  if (every < 1 || max_pairs < 0 || blocks_per_pair < 1) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netCollect(*netOf(n), *g, *p, every, max_pairs, blocks_per_pair, seed, pairs);
+}
+
+int msbgx_net_num_samples(const msbgx_net *n) { return n ? netNumSamples(*reinterpret_cast<const Net *>(n)) : 0; }
+
+int msbgx_net_train(msbgx_net *n, int steps, int batch, float lr, uint64_t seed, double *loss_first,
+                    double *loss_last)
+{
+  if (!n || steps < 0 || batch < 1 || !(lr > 0.0f)) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netTrain(*netOf(n), steps, batch, lr, seed, loss_first, loss_last);
+}
+
+double msbgx_net_last_train_ms(const msbgx_net *n) { return n ? netLastTrainMs(*reinterpret_cast<const Net *>(n)) : 0.0; }
+
+int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out)
+{
+  if (!n) { setError("null argument"); return MSBGX_EINVAL; }
+  if (int rc = checkLearnedGrid(g)) return rc;
+  if (ch_in < 0 || ch_in > MSBGX_CH_Q || ch_out < 0 || ch_out > MSBGX_CH_Q || ch_in == ch_out) {
+    setError("bad channel");
+    return MSBGX_EINVAL;
+  }
+  return netApply(*netOf(n), *g, ch_in, ch_out);
+}
+
+int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_params *p, int *iters, float *rel,
+                                 int *fallback_iter)
+{
+  if (!n || !iters || !rel || !fallback_iter) { setError("null argument"); return MSBGX_EINVAL; }
+  if (int rc = checkLearnedGrid(g)) return rc;
+  if (int rc = checkParams(p)) return rc;
+  return deviceSolveLearned(*g, *netOf(n), *p, iters, rel, fallback_iter);
 }
 
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)

@@ -1,0 +1,1135 @@
+/*
+ * Learned correction of the MG-PCG preconditioner (build addition; the
+ * reference has no learned component).  Compiled as part of solver.hip,
+ * whose device state, PCG kernels and launch helpers it uses.
+ *
+ * Network: four 3x3x3 convolutions without biases, channels 1 -> 16 -> 16 ->
+ * 16 -> 1, ReLU after the first three, BF16 weights and activations, FP32
+ * accumulation.  It runs per level-0 16^3 block on an 18^3 patch: the block
+ * plus a one-voxel halo from the 26 neighbouring blocks (zero where the
+ * neighbour is not a level-0 block or outside the raster); every layer pads
+ * the patch with zeros ("same" convolutions) and the last layer's output on
+ * the 16^3 block cells is the correction.  Without biases the network is
+ * positively homogeneous, N(a r) = a N(r) for a > 0, so residual magnitudes
+ * need no normalisation at inference.
+ *
+ * Use in PCG: z = V(r) + N(r - A V(r)), V the plain V-cycle, i.e. the network
+ * estimates the error the V-cycle leaves.  While the correction is on, beta
+ * is the flexible (Polak-Ribiere) one; a non-finite network output (masked to
+ * zero where it occurs) or a stall of the residual (less than 10% reduction
+ * over 5 iterations) switches the correction off for the rest of the solve
+ * and restarts the directions: the plain MG-PCG is the fallback.
+ *
+ * Training pairs come from the plain MG-PCG itself: after iteration k the
+ * error is e = x* - x_k (x* the converged solve), the network input is
+ * rho = r_k - A V(r_k) and its target e - V(r_k).  Each sampled block's
+ * patch and target are divided by the RMS of the target over the block's
+ * unknowns (harmless by the homogeneity; it weights the samples equally).
+ *
+ * Storage.  A patch buffer holds 20^3 positions (the 18^3 patch plus a zero
+ * ring) between 32 zero slack positions on either side, so that every tap
+ * of every computed position reads inside the buffer.  Activations are
+ * channels-last BF16 (16 channels = 32 bytes per position).  Positions of
+ * the ring and of the first and last plane are never written non-zero.
+ *
+ * Kernels (all positions of planes 1..18; ring positions write zeros):
+ *   k_nn_gather     rho of a block + halo -> BF16 patch (inference)
+ *   k_nn_conv1      1 -> 16, VALU (27 MACs x 16 per position), ReLU
+ *   k_nn_conv16     16 -> 16 on MFMA (v_mfma_f32_16x16x32_bf16: 16 positions
+ *                   x 16 output channels x 2 taps of 16 input channels),
+ *                   input planes staged in LDS; forward (ReLU) and backward
+ *                   data gradient (tap-flipped, channel-transposed weights,
+ *                   times the ReLU mask of the layer input)
+ *   k_nn_out        16 -> 1 on the block cells, VALU from LDS; inference adds
+ *                   the masked, finite correction to z
+ *   training: k_nn_loss (masked MSE gradient), k_nn_bwd4, k_nn_dw16,
+ *   k_nn_dw_small, k_nn_reduce (fixed-order FP64 sum of per-workgroup
+ *   partials), k_nn_adam, k_nn_pack
+ */
+
+namespace msbgx {
+
+namespace nn {
+constexpr int C = 16;                       // channels of the hidden layers
+constexpr int E = 20;                       // padded patch edge
+constexpr int PL = E * E;                   // positions per plane
+constexpr int NPOS = E * E * E;
+constexpr int SL = 32;                      // slack positions on either side
+constexpr int PS = NPOS + 2 * SL;           // positions per patch buffer
+constexpr int T = 27;                       // taps
+constexpr int NJ = 14;                      // MFMA K-steps (two taps each)
+constexpr int B = 16;                       // block edge
+constexpr int NCELL = B * B * B;
+constexpr int NW1 = T * C, NW2 = T * C * C, NW4 = T * C;
+constexpr int OFF_W1 = 0, OFF_W2 = NW1, OFF_W3 = NW1 + NW2, OFF_W4 = NW1 + 2 * NW2;
+constexpr int NPARAM = 2 * NW1 + 2 * NW2;   // 14688
+constexpr int ZP = 2;                       // output planes per workgroup
+constexpr int NZG = 18 / ZP;                // workgroups per block along z (conv layers)
+constexpr int LDS_POS = (ZP + 2) * PL + 64; // staged positions per workgroup
+// tap t = 9 (dz+1) + 3 (dy+1) + (dx+1)
+__host__ __device__ constexpr int tapOff(int t) { return (t % 3 - 1) + E * ((t / 3) % 3 - 1) + PL * (t / 9 - 1); }
+}  // namespace nn
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t nnF2bf(float v)
+{
+  const uint32_t b = __float_as_uint(v);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+  return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float nnBf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bool nnRing(int pos)
+{
+  const int u = pos % nn::E, v = (pos / nn::E) % nn::E;
+  return u == 0 || u == nn::E - 1 || v == 0 || v == nn::E - 1;
+}
+
+/* Stage the positions [S0, S0 + LDS_POS) of a 16-channel patch into LDS;
+ * S0 = SL + (z0 - 1) PL - 32 for output planes starting at patch plane z0. */
+__device__ __forceinline__ void nnStage16(uint4 *lds, const uint16_t *__restrict__ src, int z0)
+{
+  const int64_t S0 = nn::SL + (int64_t)(z0 - 1) * nn::PL - 32;
+  const uint4 *g = reinterpret_cast<const uint4 *>(src + S0 * nn::C);
+  for (int c = threadIdx.x; c < nn::LDS_POS * 2; c += blockDim.x) lds[c] = g[c];
+}
+
+/* Patch of rho around level-0 block blocks[i] (inference). */
+__global__ void __launch_bounds__(256) k_nn_gather(const float *__restrict__ src, const int *__restrict__ blocks,
+                                                   const int *__restrict__ level, const int64_t *__restrict__ off,
+                                                   int nbx, int nby, int nbz, uint16_t *__restrict__ X0,
+                                                   const int *skip)
+{
+  if (skip && *skip) return;
+  const int i = blockIdx.x, bid = blocks[i];
+  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  uint16_t *dst = X0 + (int64_t)i * nn::PS + nn::SL;
+  for (int pos = threadIdx.x; pos < nn::NPOS; pos += blockDim.x) {
+    const int u = pos % nn::E, v = (pos / nn::E) % nn::E, w = pos / nn::PL;
+    float val = 0.0f;
+    if (u >= 1 && u <= 18 && v >= 1 && v <= 18 && w >= 1 && w <= 18) {
+      const int cx = u - 2, cy = v - 2, cz = w - 2;   // cell coordinates, -1..16
+      const int ox = cx < 0 ? -1 : (cx > 15 ? 1 : 0), oy = cy < 0 ? -1 : (cy > 15 ? 1 : 0),
+                oz = cz < 0 ? -1 : (cz > 15 ? 1 : 0);
+      const int nx = bx + ox, ny = by + oy, nz = bz + oz;
+      if (nx >= 0 && ny >= 0 && nz >= 0 && nx < nbx && ny < nby && nz < nbz) {
+        const int nb = nx + nbx * (ny + nby * nz);
+        if (level[nb] == 0) {
+          const int lx = cx - 16 * ox, ly = cy - 16 * oy, lz = cz - 16 * oz;
+          val = src[off[nb] + lx + 16 * ly + 256 * lz];
+        }
+      }
+    }
+    dst[pos] = nnF2bf(val);
+  }
+}
+
+/* Layer 1 (1 -> 16), ReLU.  X0 patch of sample idx[i] (or i). */
+__global__ void __launch_bounds__(256) k_nn_conv1(const uint16_t *__restrict__ X0, const int *__restrict__ idx,
+                                                  const float *__restrict__ w1, uint16_t *__restrict__ A1,
+                                                  const int *skip)
+{
+  if (skip && *skip) return;
+  __shared__ float w[nn::T * nn::C];
+  for (int k = threadIdx.x; k < nn::T * nn::C; k += blockDim.x) w[k] = w1[k];
+  __syncthreads();
+  const int i = blockIdx.x, s = idx ? idx[i] : i;
+  const uint16_t *x = X0 + (int64_t)s * nn::PS + nn::SL;
+  uint16_t *out = A1 + ((int64_t)i * nn::PS + nn::SL) * nn::C;
+  const int z0 = 1 + nn::ZP * blockIdx.y;
+  for (int q = threadIdx.x; q < nn::ZP * nn::PL; q += blockDim.x) {
+    const int pos = z0 * nn::PL + q;
+    float acc[nn::C];
+#pragma unroll
+    for (int c = 0; c < nn::C; c++) acc[c] = 0.0f;
+    const bool ring = nnRing(pos);
+    if (!ring) {
+#pragma unroll
+      for (int t = 0; t < nn::T; t++) {
+        const float xv = nnBf2f(x[pos + nn::tapOff(t)]);
+#pragma unroll
+        for (int c = 0; c < nn::C; c++) acc[c] = fmaf(w[t * nn::C + c], xv, acc[c]);
+      }
+    }
+    uint32_t packed[nn::C / 2];
+#pragma unroll
+    for (int c = 0; c < nn::C; c += 2)
+      packed[c / 2] = (uint32_t)nnF2bf(fmaxf(acc[c], 0.0f)) | ((uint32_t)nnF2bf(fmaxf(acc[c + 1], 0.0f)) << 16);
+    uint4 *o = reinterpret_cast<uint4 *>(out + (int64_t)pos * nn::C);
+    o[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    o[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+  }
+}
+
+/* 16 -> 16 convolution on MFMA.  BWD = false: dst = ReLU(conv(src, W)).
+ * BWD = true: dst = conv(src, Wb) * (mask > 0) (data gradient through the
+ * ReLU of the layer input `mask`).  wpk: per K-step j and lane the 8 BF16
+ * B-operand values (see k_nn_pack). */
+template <bool BWD>
+__global__ void __launch_bounds__(256) k_nn_conv16(const uint16_t *__restrict__ src, const uint4 *__restrict__ wpk,
+                                                   const uint16_t *__restrict__ mask, uint16_t *__restrict__ dst,
+                                                   const int *skip)
+{
+  if (skip && *skip) return;
+  __shared__ uint4 lds[nn::LDS_POS * 2];
+  const int i = blockIdx.x;
+  const int z0 = 1 + nn::ZP * blockIdx.y;
+  const int64_t base = (int64_t)i * nn::PS * nn::C;
+  nnStage16(lds, src + base, z0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bf16x8_t wb[nn::NJ];
+#pragma unroll
+  for (int j = 0; j < nn::NJ; j++) wb[j] = __builtin_bit_cast(bf16x8_t, wpk[j * 64 + lane]);
+  __syncthreads();
+  const int m = lane & 15, kg = lane >> 4;
+  const int ldsOrigin = (z0 - 1) * nn::PL - 32;   // patch position of lds[0]
+  for (int tile = wave; tile < nn::ZP * 25; tile += 4) {
+    const int pb = (z0 + tile / 25) * nn::PL + (tile % 25) * 16;
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < nn::NJ; j++) {
+      const int tap = 2 * j + (kg >> 1);
+      bf16x8_t a;
+      if (tap < nn::T) {
+        const int lp = pb + m + nn::tapOff(tap) - ldsOrigin;
+        a = __builtin_bit_cast(bf16x8_t, lds[lp * 2 + (kg & 1)]);
+      } else {
+        a = __builtin_bit_cast(bf16x8_t, make_uint4(0u, 0u, 0u, 0u));
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb[j], acc, 0, 0, 0);
+    }
+    const int co = m;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int pos = pb + 4 * kg + r;
+      float v = acc[r];
+      if (BWD) {
+        const uint16_t mk = mask[base + (int64_t)(nn::SL + pos) * nn::C + co];
+        v = (mk & 0x7fffu) && !(mk & 0x8000u) ? v : 0.0f;
+      } else {
+        v = fmaxf(v, 0.0f);
+      }
+      if (nnRing(pos)) v = 0.0f;
+      dst[base + (int64_t)(nn::SL + pos) * nn::C + co] = nnF2bf(v);
+    }
+  }
+}
+
+/* Layer 4 (16 -> 1) on the 16^3 block cells of two cell planes per
+ * workgroup.  Inference (out == nullptr): z[off[bid] + cell] += n on cells
+ * that are unknowns (not SOLID/VOID, D != 0); a non-finite n is dropped and
+ * flagged in st->bad.  Training: out[i][cell] = n (0 off the mask). */
+__global__ void __launch_bounds__(256) k_nn_out(const uint16_t *__restrict__ A3, const float *__restrict__ w4,
+                                                const int *__restrict__ blocks, const int64_t *__restrict__ off,
+                                                const uint16_t *__restrict__ flags, const float *__restrict__ D,
+                                                float *__restrict__ z, PcgState *st,
+                                                const uint8_t *__restrict__ tmask, const int *__restrict__ idx,
+                                                float *__restrict__ out)
+{
+  if (st && st->skipNet) return;
+  __shared__ uint4 lds[nn::LDS_POS * 2];
+  __shared__ float w[nn::T * nn::C];
+  const int i = blockIdx.x;
+  const int cz0 = 2 * blockIdx.y;       // first cell plane
+  const int z0 = cz0 + 2;               // its patch plane
+  const int64_t base = (int64_t)i * nn::PS * nn::C;
+  nnStage16(lds, A3 + base, z0);
+  for (int k = threadIdx.x; k < nn::T * nn::C; k += blockDim.x) w[k] = w4[k];
+  __syncthreads();
+  const int ldsOrigin = (z0 - 1) * nn::PL - 32;
+  for (int q = threadIdx.x; q < 2 * 256; q += blockDim.x) {
+    const int cx = q & 15, cy = (q >> 4) & 15, cz = cz0 + (q >> 8);
+    const int pos = (cz + 2) * nn::PL + (cy + 2) * nn::E + (cx + 2);
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < nn::T; t++) {
+      const int lp = pos + nn::tapOff(t) - ldsOrigin;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint4 v = lds[lp * 2 + h];
+        const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          acc = fmaf(w[t * nn::C + 8 * h + 2 * e], nnBf2f((uint16_t)(wv[e] & 0xffffu)), acc);
+          acc = fmaf(w[t * nn::C + 8 * h + 2 * e + 1], nnBf2f((uint16_t)(wv[e] >> 16)), acc);
+        }
+      }
+    }
+    const int cell = cx + 16 * cy + 256 * cz;
+    if (out) {
+      const int s = idx ? idx[i] : i;
+      out[(int64_t)i * nn::NCELL + cell] = tmask[(int64_t)s * nn::NCELL + cell] ? acc : 0.0f;
+    } else {
+      const int64_t c = off[blocks[i]] + cell;
+      const bool unknown = !(flags[c] & (CELL_SOLID | CELL_VOID)) && D[c] != 0.0f;
+      if (unknown) {
+        if (isfinite(acc)) z[c] += acc;
+        else if (st) st->bad = 1;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------ training */
+
+/* dY = 2 (out - target) mask * invN and per-sample partial losses. */
+__global__ void __launch_bounds__(256) k_nn_loss(const float *__restrict__ out, const float *__restrict__ target,
+                                                 const uint8_t *__restrict__ tmask, const int *__restrict__ idx,
+                                                 float invN, float *__restrict__ dY, double *__restrict__ lossPart)
+{
+  __shared__ double red[4];
+  const int i = blockIdx.x, s = idx ? idx[i] : i;
+  double acc = 0.0;
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    const int64_t o = (int64_t)i * nn::NCELL + c, so = (int64_t)s * nn::NCELL + c;
+    const float d = tmask[so] ? out[o] - target[so] : 0.0f;
+    dY[o] = 2.0f * d * invN;
+    acc += (double)d * (double)d;
+  }
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) lossPart[i] = t * (double)invN;
+}
+
+/* dZ3 = (sum_t W4[t][ci] dY[cell(pos - off_t)]) * (A3 > 0) on planes 1..18. */
+__global__ void __launch_bounds__(256) k_nn_bwd4(const float *__restrict__ dY, const float *__restrict__ w4,
+                                                 const uint16_t *__restrict__ A3, uint16_t *__restrict__ G3)
+{
+  __shared__ float w[nn::T * nn::C];
+  for (int k = threadIdx.x; k < nn::T * nn::C; k += blockDim.x) w[k] = w4[k];
+  __syncthreads();
+  const int i = blockIdx.x;
+  const int z0 = 1 + nn::ZP * blockIdx.y;
+  const int64_t base = (int64_t)i * nn::PS * nn::C;
+  const float *dy = dY + (int64_t)i * nn::NCELL;
+  for (int q = threadIdx.x; q < nn::ZP * nn::PL; q += blockDim.x) {
+    const int pos = z0 * nn::PL + q;
+    float acc[nn::C];
+#pragma unroll
+    for (int c = 0; c < nn::C; c++) acc[c] = 0.0f;
+    if (!nnRing(pos)) {
+      const int u = pos % nn::E, v = (pos / nn::E) % nn::E, ww = pos / nn::PL;
+#pragma unroll
+      for (int t = 0; t < nn::T; t++) {
+        const int cx = u - (t % 3 - 1) - 2, cy = v - ((t / 3) % 3 - 1) - 2, cz = ww - (t / 9 - 1) - 2;
+        if (cx < 0 || cy < 0 || cz < 0 || cx > 15 || cy > 15 || cz > 15) continue;
+        const float g = dy[cx + 16 * cy + 256 * cz];
+#pragma unroll
+        for (int c = 0; c < nn::C; c++) acc[c] = fmaf(w[t * nn::C + c], g, acc[c]);
+      }
+    }
+    const uint16_t *mk = A3 + base + (int64_t)(nn::SL + pos) * nn::C;
+    uint16_t *o = G3 + base + (int64_t)(nn::SL + pos) * nn::C;
+#pragma unroll
+    for (int c = 0; c < nn::C; c++) {
+      const uint16_t m = mk[c];
+      o[c] = nnF2bf(((m & 0x7fffu) && !(m & 0x8000u)) ? acc[c] : 0.0f);
+    }
+  }
+}
+
+/* Per-workgroup partial dW[t][ci][co] = sum_pos A[pos + off_t][ci] dZ[pos][co]
+ * over the workgroup's two output planes (A staged in LDS). */
+__global__ void __launch_bounds__(256) k_nn_dw16(const uint16_t *__restrict__ A, const uint16_t *__restrict__ dZ,
+                                                 float *__restrict__ partial)
+{
+  __shared__ uint4 lds[nn::LDS_POS * 2];
+  const int i = blockIdx.x;
+  const int z0 = 1 + nn::ZP * blockIdx.y;
+  const int64_t base = (int64_t)i * nn::PS * nn::C;
+  nnStage16(lds, A + base, z0);
+  __syncthreads();
+  const uint16_t *L = reinterpret_cast<const uint16_t *>(lds);
+  const int co = threadIdx.x & 15, ci = threadIdx.x >> 4;
+  const int ldsOrigin = (z0 - 1) * nn::PL - 32;
+  float acc[nn::T];
+#pragma unroll
+  for (int t = 0; t < nn::T; t++) acc[t] = 0.0f;
+  for (int q = 0; q < nn::ZP * nn::PL; q++) {
+    const int pos = z0 * nn::PL + q;
+    if (nnRing(pos)) continue;
+    const float g = nnBf2f(dZ[base + (int64_t)(nn::SL + pos) * nn::C + co]);
+#pragma unroll
+    for (int t = 0; t < nn::T; t++) acc[t] = fmaf(nnBf2f(L[(pos + nn::tapOff(t) - ldsOrigin) * nn::C + ci]), g, acc[t]);
+  }
+  float *o = partial + ((int64_t)i * gridDim.y + blockIdx.y) * nn::NW2;
+#pragma unroll
+  for (int t = 0; t < nn::T; t++) o[(t * nn::C + ci) * nn::C + co] = acc[t];
+}
+
+/* Per-workgroup partials of the two 432-entry gradients:
+ *   FIRST: dW1[t][co] = sum_pos X0[pos + off_t] dZ1[pos][co]   (planes 1..18)
+ *   LAST:  dW4[t][ci] = sum_cell A3[pos(cell) + off_t][ci] dY[cell] */
+template <bool LAST>
+__global__ void __launch_bounds__(256) k_nn_dw_small(const uint16_t *__restrict__ X0, const int *__restrict__ idx,
+                                                     const uint16_t *__restrict__ G1, const uint16_t *__restrict__ A3,
+                                                     const float *__restrict__ dY, float *__restrict__ partial)
+{
+  const int i = blockIdx.x;
+  float acc[2] = {0.0f, 0.0f};
+  int ent[2];
+  for (int k = 0; k < 2; k++) ent[k] = threadIdx.x + 256 * k;
+  if (!LAST) {
+    const int s = idx ? idx[i] : i;
+    const uint16_t *x = X0 + (int64_t)s * nn::PS + nn::SL;
+    const uint16_t *g = G1 + ((int64_t)i * nn::PS + nn::SL) * nn::C;
+    const int z0 = 1 + nn::ZP * blockIdx.y;
+    for (int q = 0; q < nn::ZP * nn::PL; q++) {
+      const int pos = z0 * nn::PL + q;
+      if (nnRing(pos)) continue;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (ent[k] >= nn::NW1) continue;
+        const int t = ent[k] / nn::C, co = ent[k] % nn::C;
+        acc[k] = fmaf(nnBf2f(x[pos + nn::tapOff(t)]), nnBf2f(g[(int64_t)pos * nn::C + co]), acc[k]);
+      }
+    }
+  } else {
+    const uint16_t *a = A3 + ((int64_t)i * nn::PS + nn::SL) * nn::C;
+    const float *dy = dY + (int64_t)i * nn::NCELL;
+    const int cz0 = 2 * blockIdx.y;
+    for (int q = 0; q < 512; q++) {
+      const int cx = q & 15, cy = (q >> 4) & 15, cz = cz0 + (q >> 8);
+      const float g = dy[cx + 16 * cy + 256 * cz];
+      const int pos = (cz + 2) * nn::PL + (cy + 2) * nn::E + (cx + 2);
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (ent[k] >= nn::NW4) continue;
+        const int t = ent[k] / nn::C, ci = ent[k] % nn::C;
+        acc[k] = fmaf(nnBf2f(a[(int64_t)(pos + nn::tapOff(t)) * nn::C + ci]), g, acc[k]);
+      }
+    }
+  }
+  float *o = partial + ((int64_t)i * gridDim.y + blockIdx.y) * nn::NW1;
+  for (int k = 0; k < 2; k++)
+    if (ent[k] < nn::NW1) o[ent[k]] = acc[k];
+}
+
+/* grad[e] = sum over nPart partial rows (fixed order, FP64). */
+__global__ void k_nn_reduce(const float *__restrict__ partial, int nPart, int nEnt, float *__restrict__ grad)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEnt) return;
+  double s = 0.0;
+  for (int p = 0; p < nPart; p++) s += (double)partial[(int64_t)p * nEnt + e];
+  grad[e] = (float)s;
+}
+
+__global__ void k_nn_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                          float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gi = g[e];
+  const float mi = b1 * m[e] + (1.0f - b1) * gi;
+  const float vi = b2 * v[e] + (1.0f - b2) * gi * gi;
+  m[e] = mi;
+  v[e] = vi;
+  w[e] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+/* BF16 forms of the FP32 parameters: w1, w4 as BF16-rounded floats, and the
+ * MFMA B operands of layers 2 and 3, forward and backward.  Forward: lane l
+ * of K-step j holds W[tap][ci0 + e][co], co = l % 16, tap = 2j + (l / 32),
+ * ci0 = 8 ((l / 16) & 1).  Backward (data gradient = conv of dZ with the
+ * tap-flipped, channel-transposed weights): W[26 - tap][l % 16][ci0 + e]. */
+__global__ void k_nn_pack(const float *__restrict__ w, float *__restrict__ w1f, float *__restrict__ w4f,
+                          uint16_t *__restrict__ p2, uint16_t *__restrict__ p3, uint16_t *__restrict__ b2,
+                          uint16_t *__restrict__ b3)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nn::NW1) {
+    w1f[e] = nnBf2f(nnF2bf(w[nn::OFF_W1 + e]));
+    w4f[e] = nnBf2f(nnF2bf(w[nn::OFF_W4 + e]));
+  }
+  if (e < nn::NJ * 64 * 8) {
+    const int j = e / 512, lane = (e / 8) % 64, k = e % 8;
+    const int tap = 2 * j + (lane >> 5), ci0 = 8 * ((lane >> 4) & 1), n = lane & 15;
+    float f2 = 0.0f, f3 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+    if (tap < nn::T) {
+      f2 = w[nn::OFF_W2 + (tap * nn::C + ci0 + k) * nn::C + n];
+      f3 = w[nn::OFF_W3 + (tap * nn::C + ci0 + k) * nn::C + n];
+      g2 = w[nn::OFF_W2 + ((26 - tap) * nn::C + n) * nn::C + ci0 + k];
+      g3 = w[nn::OFF_W3 + ((26 - tap) * nn::C + n) * nn::C + ci0 + k];
+    }
+    p2[e] = nnF2bf(f2); p3[e] = nnF2bf(f3); b2[e] = nnF2bf(g2); b3[e] = nnF2bf(g3);
+  }
+}
+
+/* Training sample extraction: patch of rho (from src) and target
+ * eps = err - zv on the block cells, both divided by the patch RMS of rho;
+ * mask = cell is an unknown.  One workgroup per sampled block. */
+__global__ void __launch_bounds__(256) k_nn_sample(const float *__restrict__ rho, const float *__restrict__ err,
+                                                   const float *__restrict__ zv, const int *__restrict__ blocks,
+                                                   const int *__restrict__ level, const int64_t *__restrict__ off,
+                                                   const uint16_t *__restrict__ flags, const float *__restrict__ D,
+                                                   int nbx, int nby, int nbz, uint16_t *__restrict__ X0,
+                                                   float *__restrict__ tgt, uint8_t *__restrict__ tmask)
+{
+  __shared__ double red[4];
+  __shared__ float scale;
+  const int i = blockIdx.x, bid = blocks[i];
+  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  auto fetch = [&](int pos) -> float {
+    const int u = pos % nn::E, v = (pos / nn::E) % nn::E, w = pos / nn::PL;
+    if (u < 1 || u > 18 || v < 1 || v > 18 || w < 1 || w > 18) return 0.0f;
+    const int cx = u - 2, cy = v - 2, cz = w - 2;
+    const int ox = cx < 0 ? -1 : (cx > 15 ? 1 : 0), oy = cy < 0 ? -1 : (cy > 15 ? 1 : 0),
+              oz = cz < 0 ? -1 : (cz > 15 ? 1 : 0);
+    const int nx = bx + ox, ny = by + oy, nz = bz + oz;
+    if (nx < 0 || ny < 0 || nz < 0 || nx >= nbx || ny >= nby || nz >= nbz) return 0.0f;
+    const int nb = nx + nbx * (ny + nby * nz);
+    if (level[nb] != 0) return 0.0f;
+    return rho[off[nb] + (cx - 16 * ox) + 16 * (cy - 16 * oy) + 256 * (cz - 16 * oz)];
+  };
+  /* scale = 1 / RMS of the target over the block's unknowns */
+  double acc = 0.0, cnt = 0.0;
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    const int64_t o = off[bid] + c;
+    if ((flags[o] & (CELL_SOLID | CELL_VOID)) || D[o] == 0.0f) continue;
+    const double e = (double)err[o] - (double)zv[o];
+    acc += e * e;
+    cnt += 1.0;
+  }
+  const double ss = blockReduceSum(acc, red);
+  const double nn_ = blockReduceSum(cnt, red);
+  if (threadIdx.x == 0) scale = (ss > 0.0 && nn_ > 0.0) ? (float)(1.0 / sqrt(ss / nn_)) : 0.0f;
+  __syncthreads();
+  const float sc = scale;
+  uint16_t *x = X0 + (int64_t)i * nn::PS + nn::SL;
+  for (int pos = threadIdx.x; pos < nn::NPOS; pos += blockDim.x) x[pos] = nnF2bf(fetch(pos) * sc);
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    const int64_t o = off[bid] + c;
+    const bool unknown = sc > 0.0f && !(flags[o] & (CELL_SOLID | CELL_VOID)) && D[o] != 0.0f;
+    tgt[(int64_t)i * nn::NCELL + c] = unknown ? (err[o] - zv[o]) * sc : 0.0f;
+    tmask[(int64_t)i * nn::NCELL + c] = unknown ? 1 : 0;
+  }
+}
+
+/* a - b over [0, n) */
+__global__ void k_nn_sub(const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ out, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+
+/* dst = src over [0, n) when *flag == want */
+__global__ void k_nn_copy(const float *__restrict__ src, float *__restrict__ dst, int64_t n, const int *flag, int want)
+{
+  if (*flag != want) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+/* d = z + beta d, z = the plain V-cycle output when st->usePlain */
+__global__ void __launch_bounds__(kVecThreads) k_nn_axpy(float *__restrict__ d, const float *__restrict__ z,
+                                                         const float *__restrict__ zplain, int64_t n,
+                                                         const PcgState *st)
+{
+  if (st->done) return;
+  const float beta = st->beta;
+  const float *zz = st->usePlain ? zplain : z;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = fmaf(beta, d[i], zz[i]);
+}
+
+/* ------------------------------------------------------------------ host */
+
+struct NetBuffers {
+  int cap = 0;                                   // blocks
+  uint16_t *X0 = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr;
+  uint16_t *G1 = nullptr, *G2 = nullptr, *G3 = nullptr;  // training only
+  float *out = nullptr, *dY = nullptr;
+  float *part = nullptr;                         // dW partials
+  double *lossPart = nullptr;
+  int *idx = nullptr;
+};
+
+struct Net {
+  hipStream_t stream = nullptr;
+  std::vector<float> host;                       // FP32 parameters (host mirror)
+  float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  float *w1f = nullptr, *w4f = nullptr;
+  uint16_t *p2 = nullptr, *p3 = nullptr, *b2 = nullptr, *b3 = nullptr;
+  int adamStep = 0;
+  /* training set */
+  int nS = 0, capS = 0;
+  uint16_t *setX = nullptr;
+  float *setT = nullptr;
+  uint8_t *setM = nullptr;
+  std::vector<int> maskCount;
+  NetBuffers tb;                                 // training work buffers
+  double lastTrainMs = 0.0;
+};
+
+static void freeBuffers(NetBuffers &b)
+{
+  auto f = [](void *p) { if (p) (void)hipFree(p); };
+  f(b.X0); f(b.A1); f(b.A2); f(b.A3); f(b.G1); f(b.G2); f(b.G3); f(b.out); f(b.dY); f(b.part); f(b.lossPart); f(b.idx);
+  b = NetBuffers{};
+}
+
+/* Patch buffers for `n` blocks, zeroed (ring, first/last plane and slack stay zero). */
+static int allocBuffers(NetBuffers &b, int n, bool training, hipStream_t s)
+{
+  if (b.cap >= n && (!training || b.G1)) return MSBGX_OK;
+  freeBuffers(b);
+  const size_t act = (size_t)n * nn::PS * nn::C;
+  if (int rc = dalloc(&b.X0, (size_t)n * nn::PS)) return rc;
+  if (int rc = dalloc(&b.A1, act)) return rc;
+  if (int rc = dalloc(&b.A2, act)) return rc;
+  if (int rc = dalloc(&b.A3, act)) return rc;
+  HIPCHK(hipMemsetAsync(b.X0, 0, (size_t)n * nn::PS * 2, s));
+  HIPCHK(hipMemsetAsync(b.A1, 0, act * 2, s));
+  HIPCHK(hipMemsetAsync(b.A2, 0, act * 2, s));
+  HIPCHK(hipMemsetAsync(b.A3, 0, act * 2, s));
+  if (training) {
+    if (int rc = dalloc(&b.G1, act)) return rc;
+    if (int rc = dalloc(&b.G2, act)) return rc;
+    if (int rc = dalloc(&b.G3, act)) return rc;
+    HIPCHK(hipMemsetAsync(b.G1, 0, act * 2, s));
+    HIPCHK(hipMemsetAsync(b.G2, 0, act * 2, s));
+    HIPCHK(hipMemsetAsync(b.G3, 0, act * 2, s));
+    if (int rc = dalloc(&b.out, (size_t)n * nn::NCELL)) return rc;
+    if (int rc = dalloc(&b.dY, (size_t)n * nn::NCELL)) return rc;
+    if (int rc = dalloc(&b.part, (size_t)n * nn::NZG * nn::NW2)) return rc;
+    if (int rc = dalloc(&b.lossPart, n)) return rc;
+    if (int rc = dalloc(&b.idx, n)) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  b.cap = n;
+  return MSBGX_OK;
+}
+
+static int netPack(Net &n)
+{
+  hipLaunchKernelGGL(k_nn_pack, dim3((nn::NJ * 64 * 8 + 255) / 256), dim3(256), 0, n.stream, n.w, n.w1f, n.w4f, n.p2,
+                     n.p3, n.b2, n.b3);
+  HIPCHK(hipGetLastError());
+  return MSBGX_OK;
+}
+
+int netCreate(Net **out, uint64_t seed)
+{
+  Net *n = new Net();
+  *out = n;
+  HIPCHK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+  if (int rc = dalloc(&n->w, nn::NPARAM)) return rc;
+  if (int rc = dalloc(&n->m, nn::NPARAM)) return rc;
+  if (int rc = dalloc(&n->v, nn::NPARAM)) return rc;
+  if (int rc = dalloc(&n->grad, nn::NPARAM)) return rc;
+  if (int rc = dalloc(&n->w1f, nn::NW1)) return rc;
+  if (int rc = dalloc(&n->w4f, nn::NW4)) return rc;
+  const size_t npk = nn::NJ * 64 * 8;
+  if (int rc = dalloc(&n->p2, npk)) return rc;
+  if (int rc = dalloc(&n->p3, npk)) return rc;
+  if (int rc = dalloc(&n->b2, npk)) return rc;
+  if (int rc = dalloc(&n->b3, npk)) return rc;
+  /* He-normal initialisation from a splitmix64 stream (Box-Muller) */
+  n->host.resize(nn::NPARAM);
+  uint64_t st = seed;
+  auto uni = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto normal = [&]() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); };
+  const int fan[4] = {nn::T, nn::T * nn::C, nn::T * nn::C, nn::T * nn::C};
+  const int beg[4] = {nn::OFF_W1, nn::OFF_W2, nn::OFF_W3, nn::OFF_W4};
+  const int len[4] = {nn::NW1, nn::NW2, nn::NW2, nn::NW4};
+  for (int l = 0; l < 4; l++) {
+    const double sd = std::sqrt((l == 3 ? 1.0 : 2.0) / fan[l]);
+    for (int e = 0; e < len[l]; e++) n->host[beg[l] + e] = (float)(sd * normal());
+  }
+  HIPCHK(hipMemcpy(n->w, n->host.data(), nn::NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(n->m, 0, nn::NPARAM * sizeof(float)));
+  HIPCHK(hipMemset(n->v, 0, nn::NPARAM * sizeof(float)));
+  if (int rc = netPack(*n)) return rc;
+  HIPCHK(hipStreamSynchronize(n->stream));
+  return MSBGX_OK;
+}
+
+void netDestroy(Net *n)
+{
+  if (!n) return;
+  auto f = [](void *p) { if (p) (void)hipFree(p); };
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  f(n->w); f(n->m); f(n->v); f(n->grad); f(n->w1f); f(n->w4f); f(n->p2); f(n->p3); f(n->b2); f(n->b3);
+  f(n->setX); f(n->setT); f(n->setM);
+  freeBuffers(n->tb);
+  if (n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+}
+
+int netSetWeights(Net &n, const float *w)
+{
+  n.host.assign(w, w + nn::NPARAM);
+  HIPCHK(hipMemcpy(n.w, w, nn::NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(n.m, 0, nn::NPARAM * sizeof(float)));
+  HIPCHK(hipMemset(n.v, 0, nn::NPARAM * sizeof(float)));
+  n.adamStep = 0;
+  if (int rc = netPack(n)) return rc;
+  HIPCHK(hipStreamSynchronize(n.stream));
+  return MSBGX_OK;
+}
+
+int netGetWeights(Net &n, float *w)
+{
+  HIPCHK(hipMemcpy(w, n.w, nn::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return MSBGX_OK;
+}
+
+/* Forward of layers 1-3 for `nb` patches (X0 of sample idx[i] or i). */
+static void netForward123(Net &n, const uint16_t *X0, const int *idx, NetBuffers &b, int nb, const int *skip,
+                          hipStream_t s)
+{
+  hipLaunchKernelGGL(k_nn_conv1, dim3(nb, nn::NZG), dim3(256), 0, s, X0, idx, n.w1f, b.A1, skip);
+  hipLaunchKernelGGL(k_nn_conv16<false>, dim3(nb, nn::NZG), dim3(256), 0, s, b.A1, (const uint4 *)n.p2,
+                     (const uint16_t *)nullptr, b.A2, skip);
+  hipLaunchKernelGGL(k_nn_conv16<false>, dim3(nb, nn::NZG), dim3(256), 0, s, b.A2, (const uint4 *)n.p3,
+                     (const uint16_t *)nullptr, b.A3, skip);
+}
+
+/* Loss and gradient over `nb` samples of the training set addressed by the
+ * device index list b.idx (already uploaded).  Leaves grad in n.grad and the
+ * per-sample loss partials in b.lossPart. */
+static void netLossGrad(Net &n, NetBuffers &b, int nb, float invN)
+{
+  hipStream_t s = n.stream;
+  netForward123(n, n.setX, b.idx, b, nb, nullptr, s);
+  hipLaunchKernelGGL(k_nn_out, dim3(nb, 8), dim3(256), 0, s, b.A3, n.w4f, (const int *)nullptr,
+                     (const int64_t *)nullptr, (const uint16_t *)nullptr, (const float *)nullptr, (float *)nullptr,
+                     (PcgState *)nullptr, n.setM, b.idx, b.out);
+  hipLaunchKernelGGL(k_nn_loss, dim3(nb), dim3(256), 0, s, b.out, n.setT, n.setM, b.idx, invN, b.dY, b.lossPart);
+  /* layer 4 */
+  hipLaunchKernelGGL(k_nn_dw_small<true>, dim3(nb, 8), dim3(256), 0, s, (const uint16_t *)nullptr,
+                     (const int *)nullptr, (const uint16_t *)nullptr, b.A3, b.dY, b.part);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW4 + 255) / 256), dim3(256), 0, s, b.part, nb * 8, nn::NW4,
+                     n.grad + nn::OFF_W4);
+  hipLaunchKernelGGL(k_nn_bwd4, dim3(nb, nn::NZG), dim3(256), 0, s, b.dY, n.w4f, b.A3, b.G3);
+  /* layer 3: dW3 = corr(A2, G3); G2 = conv(G3, W3b) * (A2 > 0) */
+  hipLaunchKernelGGL(k_nn_dw16, dim3(nb, nn::NZG), dim3(256), 0, s, b.A2, b.G3, b.part);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW2,
+                     n.grad + nn::OFF_W3);
+  hipLaunchKernelGGL(k_nn_conv16<true>, dim3(nb, nn::NZG), dim3(256), 0, s, b.G3, (const uint4 *)n.b3, b.A2, b.G2,
+                     (const int *)nullptr);
+  /* layer 2 */
+  hipLaunchKernelGGL(k_nn_dw16, dim3(nb, nn::NZG), dim3(256), 0, s, b.A1, b.G2, b.part);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW2,
+                     n.grad + nn::OFF_W2);
+  hipLaunchKernelGGL(k_nn_conv16<true>, dim3(nb, nn::NZG), dim3(256), 0, s, b.G2, (const uint4 *)n.b2, b.A1, b.G1,
+                     (const int *)nullptr);
+  /* layer 1 */
+  hipLaunchKernelGGL(k_nn_dw_small<false>, dim3(nb, nn::NZG), dim3(256), 0, s, n.setX, b.idx, b.G1,
+                     (const uint16_t *)nullptr, (const float *)nullptr, b.part);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW1 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW1,
+                     n.grad + nn::OFF_W1);
+}
+
+static int growSet(Net &n, int need)
+{
+  if (need <= n.capS) return MSBGX_OK;
+  int cap = std::max(need, 2 * n.capS);
+  uint16_t *x = nullptr;
+  float *t = nullptr;
+  uint8_t *m = nullptr;
+  if (int rc = dalloc(&x, (size_t)cap * nn::PS)) return rc;
+  if (int rc = dalloc(&t, (size_t)cap * nn::NCELL)) return rc;
+  if (int rc = dalloc(&m, (size_t)cap * nn::NCELL)) return rc;
+  HIPCHK(hipMemsetAsync(x, 0, (size_t)cap * nn::PS * 2, n.stream));
+  if (n.nS) {
+    HIPCHK(hipMemcpyAsync(x, n.setX, (size_t)n.nS * nn::PS * 2, hipMemcpyDeviceToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(t, n.setT, (size_t)n.nS * nn::NCELL * 4, hipMemcpyDeviceToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(m, n.setM, (size_t)n.nS * nn::NCELL, hipMemcpyDeviceToDevice, n.stream));
+  }
+  HIPCHK(hipStreamSynchronize(n.stream));
+  auto f = [](void *p) { if (p) (void)hipFree(p); };
+  f(n.setX); f(n.setT); f(n.setM);
+  n.setX = x; n.setT = t; n.setM = m; n.capS = cap;
+  return MSBGX_OK;
+}
+
+/* Host-provided samples: x[count][18^3] (patch, x fastest), target and mask
+ * [count][16^3].  Replaces the training set. */
+int netSetSamples(Net &n, int count, const float *x, const float *target, const uint8_t *mask)
+{
+  n.nS = 0;
+  if (int rc = growSet(n, count)) return rc;
+  std::vector<uint16_t> px((size_t)count * nn::PS, 0);
+  auto f2bf = [](float v) {
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+  };
+  for (int i = 0; i < count; i++)
+    for (int w = 1; w <= 18; w++)
+      for (int v = 1; v <= 18; v++)
+        for (int u = 1; u <= 18; u++)
+          px[(size_t)i * nn::PS + nn::SL + w * nn::PL + v * nn::E + u] =
+              f2bf(x[(size_t)i * 5832 + (u - 1) + 18 * (v - 1) + 324 * (w - 1)]);
+  HIPCHK(hipMemcpy(n.setX, px.data(), px.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(n.setT, target, (size_t)count * nn::NCELL * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(n.setM, mask, (size_t)count * nn::NCELL, hipMemcpyHostToDevice));
+  n.maskCount.assign(count, 0);
+  for (int i = 0; i < count; i++)
+    for (int c = 0; c < nn::NCELL; c++) n.maskCount[i] += mask[(size_t)i * nn::NCELL + c] ? 1 : 0;
+  n.nS = count;
+  return MSBGX_OK;
+}
+
+/* Loss (mean over masked cells), gradient and outputs over the whole set. */
+int netLossGradAll(Net &n, double *loss, float *grad, float *out)
+{
+  const int nb = n.nS;
+  if (nb == 0) { setError("no samples"); return MSBGX_ESTATE; }
+  if (int rc = allocBuffers(n.tb, nb, true, n.stream)) return rc;
+  std::vector<int> idx(nb);
+  long cnt = 0;
+  for (int i = 0; i < nb; i++) { idx[i] = i; cnt += n.maskCount[i]; }
+  HIPCHK(hipMemcpyAsync(n.tb.idx, idx.data(), nb * sizeof(int), hipMemcpyHostToDevice, n.stream));
+  netLossGrad(n, n.tb, nb, cnt ? 1.0f / (float)cnt : 0.0f);
+  HIPCHK(hipGetLastError());
+  std::vector<double> lp(nb);
+  HIPCHK(hipMemcpyAsync(lp.data(), n.tb.lossPart, nb * sizeof(double), hipMemcpyDeviceToHost, n.stream));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, n.grad, nn::NPARAM * sizeof(float), hipMemcpyDeviceToHost, n.stream));
+  if (out) HIPCHK(hipMemcpyAsync(out, n.tb.out, (size_t)nb * nn::NCELL * 4, hipMemcpyDeviceToHost, n.stream));
+  HIPCHK(hipStreamSynchronize(n.stream));
+  double L = 0.0;
+  for (double v : lp) L += v;
+  *loss = L;
+  return MSBGX_OK;
+}
+
+int netTrain(Net &n, int steps, int batch, float lr, uint64_t seed, double *lossFirst, double *lossLast)
+{
+  if (n.nS == 0) { setError("no training samples"); return MSBGX_ESTATE; }
+  batch = std::min(batch, n.nS);
+  if (int rc = allocBuffers(n.tb, batch, true, n.stream)) return rc;
+  uint64_t st = seed ? seed : 1;
+  auto next = [&]() {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    return st;
+  };
+  std::vector<int> idx(batch);
+  std::vector<double> lp(batch);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, n.stream));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double first = 0.0, last = 0.0, ema = 0.0;
+  for (int it = 0; it < steps; it++) {
+    long cnt = 0;
+    for (int i = 0; i < batch; i++) {
+      idx[i] = (int)(next() % (uint64_t)n.nS);
+      cnt += n.maskCount[idx[i]];
+    }
+    HIPCHK(hipMemcpyAsync(n.tb.idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, n.stream));
+    netLossGrad(n, n.tb, batch, cnt ? 1.0f / (float)cnt : 0.0f);
+    n.adamStep++;
+    const float c1 = 1.0f - std::pow(b1, (float)n.adamStep), c2 = 1.0f - std::pow(b2, (float)n.adamStep);
+    hipLaunchKernelGGL(k_nn_adam, dim3((nn::NPARAM + 255) / 256), dim3(256), 0, n.stream, n.w, n.grad, n.m, n.v,
+                       nn::NPARAM, lr, b1, b2, eps, c1, c2);
+    if (int rc = netPack(n)) return rc;
+    HIPCHK(hipMemcpyAsync(lp.data(), n.tb.lossPart, batch * sizeof(double), hipMemcpyDeviceToHost, n.stream));
+    HIPCHK(hipStreamSynchronize(n.stream));   // idx and lp are reused next step
+    double L = 0.0;
+    for (double v : lp) L += v;
+    ema = it == 0 ? L : 0.95 * ema + 0.05 * L;
+    if (it == 0) first = L;
+    last = ema;
+  }
+  HIPCHK(hipEventRecord(e1, n.stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  n.lastTrainMs = ms;
+  HIPCHK(hipMemcpy(n.host.data(), n.w, nn::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  if (lossFirst) *lossFirst = first;
+  if (lossLast) *lossLast = last;
+  return MSBGX_OK;
+}
+
+int netNumSamples(const Net &n) { return n.nS; }
+double netLastTrainMs(const Net &n) { return n.lastTrainMs; }
+
+/* ------------------------------------------------- PCG with the correction */
+
+struct LearnedCtx {
+  NetBuffers ib;                // inference buffers for the level-0 PCG blocks
+  float *zprev = nullptr, *zplain = nullptr;
+  int64_t n0 = 0;
+  const Net *graphNet = nullptr;   // net whose buffers the captured graph uses
+  hipGraphExec_t graph = nullptr;
+  int graphChunk = 0;
+};
+
+static void freeLearned(LearnedCtx *c)
+{
+  if (!c) return;
+  freeBuffers(c->ib);
+  if (c->zprev) (void)hipFree(c->zprev);
+  if (c->zplain) (void)hipFree(c->zplain);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  delete c;
+}
+
+/* z += N(r - A z) on the level-0 PCG blocks (skipped while st->skipNet). */
+static void netCorrect(Grid &g, Net &n, LearnedCtx &c, hipStream_t s)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *skip = &d.state->skipNet;
+  const int nb = g.nActive0;
+  if (nb == 0) return;
+  multiplyLaunch(g, 0, true, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], nullptr, skip, s);
+  hipLaunchKernelGGL(k_nn_gather, dim3(nb), dim3(256), 0, s, m0.d_ch[C_Q], d.dActive0, m0.d_level, m0.d_off, g.nbx,
+                     g.nby, g.nbz, c.ib.X0, skip);
+  netForward123(n, c.ib.X0, nullptr, c.ib, nb, skip, s);
+  hipLaunchKernelGGL(k_nn_out, dim3(nb, 8), dim3(256), 0, s, c.ib.A3, n.w4f, d.dActive0, m0.d_off, m0.d_flags, m0.d_D,
+                     m0.d_ch[C_Z], d.state, (const uint8_t *)nullptr, (const int *)nullptr, (float *)nullptr);
+}
+
+/* ch_out += N(ch_in) on the level-0 PCG blocks (per-kernel entry point). */
+int netApply(Net &n, Grid &g, int chIn, int chOut)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  if (!d.learned) d.learned = new LearnedCtx();
+  LearnedCtx &c = *d.learned;
+  const int nb = g.nActive0;
+  if (nb == 0) return MSBGX_OK;
+  if (int rc = allocBuffers(c.ib, nb, false, d.stream)) return rc;
+  HIPCHK(hipStreamSynchronize(n.stream));
+  hipLaunchKernelGGL(k_nn_gather, dim3(nb), dim3(256), 0, d.stream, m0.d_ch[chIn], d.dActive0, m0.d_level, m0.d_off,
+                     g.nbx, g.nby, g.nbz, c.ib.X0, (const int *)nullptr);
+  netForward123(n, c.ib.X0, nullptr, c.ib, nb, nullptr, d.stream);
+  hipLaunchKernelGGL(k_nn_out, dim3(nb, 8), dim3(256), 0, d.stream, c.ib.A3, n.w4f, d.dActive0, m0.d_off, m0.d_flags,
+                     m0.d_D, m0.d_ch[chOut], (PcgState *)nullptr, (const uint8_t *)nullptr, (const int *)nullptr,
+                     (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return MSBGX_OK;
+}
+
+static void pcgIterationLearned(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *done = &d.state->done;
+  const int *skip = &d.state->skipNet;
+  const int64_t n0 = c.n0;
+  const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
+  finalize(d, nPart, FIN_ALPHA, nullptr);
+  hipLaunchKernelGGL(k_update_xr, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
+  finalize(d, kVecBlocks, FIN_CONV, d.hist);
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zprev, n0, skip, 0);
+  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zplain, n0, skip, 0);
+  netCorrect(g, n, c, d.stream);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zprev, n0, d.partials,
+                     skip);
+  finalize(d, kVecBlocks, FIN_RZOLD, nullptr);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zplain, n0, d.partials,
+                     skip);
+  finalize(d, kVecBlocks, FIN_RZPLAIN, nullptr);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
+                     done);
+  finalize(d, kVecBlocks, FIN_BETA, nullptr);
+  hipLaunchKernelGGL(k_nn_axpy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], c.zplain,
+                     n0, d.state);
+}
+
+int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters, float *rel, int *fellBack)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  if (!d.learned) d.learned = new LearnedCtx();
+  LearnedCtx &c = *d.learned;
+  c.n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+  if (int rc = allocBuffers(c.ib, std::max(1, g.nActive0), false, d.stream)) return rc;
+  if (!c.zprev)
+    if (int rc = dalloc(&c.zprev, std::max<int64_t>(1, c.n0))) return rc;
+  if (!c.zplain)
+    if (int rc = dalloc(&c.zplain, std::max<int64_t>(1, c.n0))) return rc;
+  if (d.histCap < p.max_iter + 1) {
+    if (d.hist) (void)hipFree(d.hist);
+    d.hist = nullptr;
+    if (int rc = dalloc(&d.hist, p.max_iter + 1)) return rc;
+    d.histCap = p.max_iter + 1;
+  }
+  const int chunk = std::max(1, g.graphChunk);
+  if (c.graph && (c.graphNet != &n || c.graphChunk != chunk)) { (void)hipGraphExecDestroy(c.graph); c.graph = nullptr; }
+  PcgState init{};
+  init.tolSq = (double)p.tol * (double)p.tol;
+  init.maxIter = p.max_iter;
+  init.learned = 1;
+  HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
+  HIPCHK(hipStreamSynchronize(n.stream));   // weights packed
+  HIPCHK(hipEventRecord(d.ev0, d.stream));
+  hipLaunchKernelGGL(k_copy_norm, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_B], m0.d_ch[C_R], c.n0,
+                     d.partials);
+  finalize(d, kVecBlocks, FIN_BNORM, nullptr);
+  const int *done = &d.state->done;
+  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+  const int *skip = &d.state->skipNet;
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zplain, c.n0, skip, 0);
+  netCorrect(g, n, c, d.stream);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zplain, c.n0,
+                     d.partials, skip);
+  finalize(d, kVecBlocks, FIN_RZPLAIN, nullptr);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], c.n0,
+                     d.partials, done);
+  finalize(d, kVecBlocks, FIN_RHO0, nullptr);
+  /* d = z on every MG0 block; the PCG range takes the plain z if the
+   * corrected one was rejected */
+  for (int lev = 0; lev < g.nLevels; lev++)
+    if (m0.dAct[lev].n)
+      hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
+                         m0.d_off, g.bsx0, m0.d_ch[C_Z], m0.d_ch[C_D], done);
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, c.zplain, m0.d_ch[C_D], c.n0,
+                     &d.state->usePlain, 1);
+  HIPCHK(hipGetLastError());
+  const bool useGraph = !(g.options & MSBGX_OPT_NO_GRAPH);
+  PcgState hs{};
+  for (int it = 0; it < p.max_iter;) {
+    const int k = std::min(chunk, p.max_iter - it);
+    if (useGraph && k == chunk) {
+      if (!c.graph) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(d.stream, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < chunk; i++) pcgIterationLearned(g, n, c, p);
+        HIPCHK(hipStreamEndCapture(d.stream, &graph));
+        HIPCHK(hipGraphInstantiate(&c.graph, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        c.graphNet = &n;
+        c.graphChunk = chunk;
+      }
+      HIPCHK(hipGraphLaunch(c.graph, d.stream));
+    } else {
+      for (int i = 0; i < k; i++) pcgIterationLearned(g, n, c, p);
+    }
+    it += k;
+    HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    if (hs.done) break;
+  }
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_R], c.n0,
+                     d.partials, d.zeroFlag);
+  finalize(d, kVecBlocks, FIN_FINAL, d.hist + p.max_iter);
+  HIPCHK(hipEventRecord(d.ev1, d.stream));
+  HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+  double fin = 0.0;
+  HIPCHK(hipMemcpyAsync(&fin, d.hist + p.max_iter, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+  g.lastSolveMs = ms;
+  g.history.assign(hs.iter, 0.0);
+  if (hs.iter > 0) {
+    std::vector<double> rr(hs.iter);
+    HIPCHK(hipMemcpy(rr.data(), d.hist, hs.iter * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < hs.iter; i++) g.history[i] = std::sqrt(rr[i] / hs.bnorm2);
+  }
+  *fellBack = hs.learned ? -1 : hs.fellBack;
+  if (hs.brk == 2) { *iters = 0; *rel = 0.0f; return MSBGX_OK; }
+  *rel = (float)std::sqrt(fin / hs.bnorm2);
+  *iters = hs.conv > 0 ? hs.conv : p.max_iter;
+  return MSBGX_OK;
+}
+
+/* Training pairs from the plain MG-PCG on a prepared grid: a solve to p.tol
+ * gives x*; a second solve from 0 stops after every `every`-th iteration,
+ * and `perPair` random level-0 PCG blocks of that iterate become samples. */
+int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxPairs, int perPair, uint64_t seed,
+               int *pairsOut)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+  *pairsOut = 0;
+  if (g.nActive0 == 0 || every < 1 || maxPairs < 1 || perPair < 1) return MSBGX_OK;
+  int it = 0;
+  float rr = 0.f;
+  if (int rc = deviceZero(g, C_X, 0)) return rc;
+  if (int rc = deviceSolve(g, p, &it, &rr)) return rc;
+  float *xs = nullptr, *err = nullptr;
+  int *blk = nullptr;
+  if (int rc = dalloc(&xs, n0)) return rc;
+  if (int rc = dalloc(&err, n0)) return rc;
+  if (int rc = dalloc(&blk, perPair)) return rc;
+  HIPCHK(hipMemcpyAsync(xs, m0.d_ch[C_X], n0 * sizeof(float), hipMemcpyDeviceToDevice, d.stream));
+  if (int rc = deviceZero(g, C_X, 0)) return rc;
+  /* second solve, eagerly, sampling after chosen iterations */
+  PcgState init{};
+  init.tolSq = (double)p.tol * (double)p.tol;
+  init.maxIter = p.max_iter;
+  HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
+  hipLaunchKernelGGL(k_copy_norm, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_B], m0.d_ch[C_R], n0,
+                     d.partials);
+  finalize(d, kVecBlocks, FIN_BNORM, nullptr);
+  const int *done = &d.state->done;
+  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+  for (int lev = 0; lev < g.nLevels; lev++)
+    if (m0.dAct[lev].n)
+      hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
+                         m0.d_off, g.bsx0, m0.d_ch[C_Z], m0.d_ch[C_D], done);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
+                     done);
+  finalize(d, kVecBlocks, FIN_RHO0, nullptr);
+  uint64_t st = seed ? seed : 1;
+  auto next = [&]() {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    return st;
+  };
+  std::vector<int> pick(perPair);
+  PcgState hs{};
+  int pairs = 0;
+  for (int k = 1; k <= p.max_iter && pairs < maxPairs; k++) {
+    pcgIteration(g, p);
+    HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    if (hs.brk || hs.done) break;   // a converged iteration leaves Z unset
+    if (k % every == 0) {
+      /* z = V(r_k) is in Z */
+      multiplyLaunch(g, 0, true, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], nullptr, d.zeroFlag, d.stream);
+      hipLaunchKernelGGL(k_nn_sub, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, xs, m0.d_ch[C_X], err, n0);
+      for (int j = 0; j < perPair; j++) pick[j] = g.active0[next() % (uint64_t)g.nActive0];
+      HIPCHK(hipMemcpyAsync(blk, pick.data(), perPair * sizeof(int), hipMemcpyHostToDevice, d.stream));
+      if (int rc = growSet(n, n.nS + perPair)) return rc;
+      /* err and Z are indexed by the MG0 pool; the PCG range [0, n0) is its head */
+      hipLaunchKernelGGL(k_nn_sample, dim3(perPair), dim3(256), 0, d.stream, m0.d_ch[C_Q], err, m0.d_ch[C_Z], blk,
+                         m0.d_level, m0.d_off, m0.d_flags, m0.d_D, g.nbx, g.nby, g.nbz, n.setX + (size_t)n.nS * nn::PS,
+                         n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
+      HIPCHK(hipGetLastError());
+      std::vector<uint8_t> mk((size_t)perPair * nn::NCELL);
+      HIPCHK(hipMemcpyAsync(mk.data(), n.setM + (size_t)n.nS * nn::NCELL, mk.size(), hipMemcpyDeviceToHost, d.stream));
+      HIPCHK(hipStreamSynchronize(d.stream));
+      for (int j = 0; j < perPair; j++) {
+        int cnt = 0;
+        for (int c = 0; c < nn::NCELL; c++) cnt += mk[(size_t)j * nn::NCELL + c];
+        n.maskCount.push_back(cnt);
+      }
+      n.nS += perPair;
+      pairs++;
+    }
+  }
+  (void)hipFree(xs);
+  (void)hipFree(err);
+  (void)hipFree(blk);
+  if (int rc = deviceZero(g, C_X, 0)) return rc;
+  *pairsOut = pairs;
+  return MSBGX_OK;
+}
+
+}  // namespace msbgx

@@ -145,6 +145,22 @@ int deviceCopy(Grid &g, int src, int dst);
 int deviceVCycle(Grid &g, int nPre, int nPost, int nCoarse);
 int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
 int deviceSetPrecision(Grid &g, int prec);
+void deviceInvalidateGraphs(Grid &g);
+/* learned correction (learned.hip) */
+struct Net;
+int netCreate(Net **out, uint64_t seed);
+void netDestroy(Net *n);
+int netSetWeights(Net &n, const float *w);
+int netGetWeights(Net &n, float *w);
+int netSetSamples(Net &n, int count, const float *x, const float *target, const uint8_t *mask);
+int netLossGradAll(Net &n, double *loss, float *grad, float *out);
+int netTrain(Net &n, int steps, int batch, float lr, uint64_t seed, double *lossFirst, double *lossLast);
+int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxPairs, int perPair, uint64_t seed,
+               int *pairsOut);
+int netNumSamples(const Net &n);
+double netLastTrainMs(const Net &n);
+int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters, float *rel, int *fellBack);
+int netApply(Net &n, Grid &g, int chIn, int chOut);
 /* batched block sets (batch.cpp / solver.hip) */
 int buildBatchGrid(Grid &merged, const std::vector<const Grid *> &probs);
 int deviceUploadBatch(Grid &merged, const std::vector<const Grid *> &probs);

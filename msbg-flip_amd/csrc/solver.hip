@@ -78,7 +78,17 @@ struct PcgState {
   double bnorm2, rho, rhoNew, dq, rr, tolSq;
   float alpha, beta;
   int done, conv, iter, brk, maxIter, pad;
+  /* learned correction: active flag, restart request (beta = 0), a
+   * non-finite network output seen, kernels of the correction skip while
+   * skipNet is set, iteration of the fallback, the direction update takes
+   * the plain V-cycle output (usePlain); r.z_old for the flexible beta and
+   * r.V(r) for the fallback; |r|^2 of the last kStallWindow iterations */
+  int learned, restart, bad, skipNet, fellBack, usePlain;
+  double rzOld, rzPlain;
+  double rrWin[5];
 };
+constexpr int kStallWindow = 5;
+constexpr double kStallFactor = 0.81;   // |r| must drop by 10% over the window
 
 template <int B> struct Cfg {
   static constexpr int S = B >= 8 ? 8 : B;     // SIMD segment width (src/msbg3.cpp:1700-1743)
@@ -698,13 +708,14 @@ __device__ double sumPartials(const double *p, int n)
   return out;
 }
 
-enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL };
+enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL, FIN_RZOLD, FIN_RZPLAIN };
 
 __device__ void applyFinalize(PcgState *st, double s, int op, double *out);
 
 __global__ void __launch_bounds__(256) k_finalize(PcgState *st, const double *partials, int n, int op, double *out)
 {
-  const bool skipIfDone = (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA);
+  const bool skipIfDone =
+      (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA || op == FIN_RZOLD || op == FIN_RZPLAIN);
   if (skipIfDone && st->done) return;
   const double s = sumPartials(partials, n);
   if (threadIdx.x != 0) return;
@@ -731,7 +742,15 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       st->done = (s <= 1e-30) ? 1 : 0;
       st->conv = -1; st->iter = 0; st->brk = (s <= 1e-30) ? 2 : 0;
       break;
-    case FIN_RHO0: st->rho = s; break;
+    case FIN_RHO0:
+      st->rho = s;
+      st->usePlain = 0;
+      if (st->learned && (!(s > 1e-30) || !isfinite(s) || st->bad)) {
+        /* the corrected z is unusable: start from the plain one */
+        st->learned = 0; st->fellBack = 0; st->usePlain = 1; st->rho = st->rzPlain;
+      }
+      break;
+    case FIN_RZPLAIN: if (st->learned) st->rzPlain = s; break;
     case FIN_ALPHA: {
       st->dq = s;
       const double rho = st->rho;
@@ -747,17 +766,47 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       if (out) out[st->iter - 1] = s;
       if (s <= st->tolSq * st->bnorm2) { st->done = 1; st->conv = st->iter; }
       else if (st->iter >= st->maxIter) st->done = 1;
+      if (st->learned) {
+        /* stall test of the learned correction: fall back to the plain
+         * preconditioner (and restart the directions) when |r| dropped by
+         * less than 10% over the last kStallWindow iterations */
+        double &slot = st->rrWin[st->iter % kStallWindow];
+        if (st->iter > kStallWindow && !(s < kStallFactor * slot)) {
+          st->learned = 0; st->restart = 1; st->fellBack = st->iter;
+        }
+        slot = s;
+      }
       break;
+    case FIN_RZOLD: if (st->learned) st->rzOld = s; break;
     case FIN_BETA: {
+      st->usePlain = 0;
+      if (st->learned) {
+        /* flexible (Polak-Ribiere) beta while the learned correction is on */
+        const double be = (s - st->rzOld) / st->rho;
+        if (st->bad || !(s > 1e-30) || !isfinite(s) || !isfinite(be)) {
+          /* unusable corrected z: restart from the plain V-cycle output */
+          st->learned = 0; st->fellBack = st->iter; st->usePlain = 1; st->restart = 0;
+          st->beta = 0.0f;
+          st->rho = st->rzPlain;
+          if (!(st->rho > 1e-30) || !isfinite(st->rho)) { st->done = 1; st->brk = 1; }
+          break;
+        }
+        st->beta = st->restart ? 0.0f : (float)be;
+        st->restart = 0;
+        st->rho = s;
+        break;
+      }
       if (!(s > 1e-30) || !isfinite(s)) { st->done = 1; st->brk = 1; break; }
-      const double be = s / st->rho;
+      const double be = s / st->rho;   // Fletcher-Reeves for the fixed V-cycle preconditioner
       if (!isfinite(be)) { st->done = 1; st->brk = 1; break; }
-      st->beta = (float)be;
+      st->beta = st->restart ? 0.0f : (float)be;
+      st->restart = 0;
       st->rho = s;
       break;
     }
     case FIN_FINAL: if (out) out[0] = s; break;
   }
+  st->skipNet = (st->done || !st->learned) ? 1 : 0;
 }
 
 /* Divergence right-hand side of level-0 blocks from the scene's MAC velocity
@@ -801,6 +850,9 @@ struct BatchDev {
   int graphChunk = 0;
 };
 
+struct LearnedCtx;
+static void freeLearned(LearnedCtx *c);
+
 struct Dev {
   hipStream_t stream = nullptr;
   PcgState *state = nullptr;
@@ -822,6 +874,7 @@ struct Dev {
   int *nbr = nullptr, *probOf = nullptr;
   float *omegaOf = nullptr;
   BatchDev *batch = nullptr;
+  LearnedCtx *learned = nullptr;   // buffers and graph of the learned-correction solve
 };
 
 static Dev &dev(Grid &g) { return *(Dev *)g.dev; }
@@ -859,6 +912,7 @@ void deviceFree(Grid &g)
     if (d.dActive0) (void)hipFree(d.dActive0);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     f(d.nbr); f(d.probOf); f(d.omegaOf);
+    freeLearned(d.learned);
     if (BatchDev *b = d.batch) {
       if (b->graph) (void)hipGraphExecDestroy(b->graph);
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
@@ -941,6 +995,8 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   }
   Dev &d = dev(g);
   if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  freeLearned(d.learned);
+  d.learned = nullptr;
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
   int maxBlocksPerLaunch = std::max(g.nBlocks, kVecBlocks);
   if (d.partialsCap < maxBlocksPerLaunch) {
@@ -1467,7 +1523,7 @@ int deviceSetPrecision(Grid &g, int prec)
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d.stream));
-  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }   // kernel args changed
+  deviceInvalidateGraphs(g);   // kernel arguments changed
   return MSBGX_OK;
 }
 
@@ -2016,4 +2072,19 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
   return MSBGX_OK;
 }
 
+}  // namespace msbgx
+
+/* The learned correction (network kernels, training, PCG integration). */
+#include "learned.hip"
+
+namespace msbgx {
+/* Captured graphs bake in launch shapes and kernel choices: drop them. */
+void deviceInvalidateGraphs(Grid &g)
+{
+  if (!g.dev) return;
+  Dev &d = dev(g);
+  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  if (d.batch && d.batch->graph) { (void)hipGraphExecDestroy(d.batch->graph); d.batch->graph = nullptr; }
+  if (d.learned && d.learned->graph) { (void)hipGraphExecDestroy(d.learned->graph); d.learned->graph = nullptr; }
+}
 }  // namespace msbgx

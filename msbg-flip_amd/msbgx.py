@@ -179,6 +179,21 @@ def _load():
         "msbgx_batch_residual_history": (C.c_int, [P, C.c_int, P, C.c_int]),
         "msbgx_autotune": (C.c_int, [P, C.POINTER(SolveParams), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "msbgx_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
+        "msbgx_net_create": (C.c_int, [C.POINTER(P), C.c_uint64]),
+        "msbgx_net_destroy": (None, [P]),
+        "msbgx_net_get_weights": (C.c_int, [P, P]),
+        "msbgx_net_set_weights": (C.c_int, [P, P]),
+        "msbgx_net_set_samples": (C.c_int, [P, C.c_int, P, P, P]),
+        "msbgx_net_loss_grad": (C.c_int, [P, C.POINTER(C.c_double), P, P]),
+        "msbgx_net_collect": (C.c_int, [P, P, C.POINTER(SolveParams), C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                        C.POINTER(C.c_int)]),
+        "msbgx_net_num_samples": (C.c_int, [P]),
+        "msbgx_net_train": (C.c_int, [P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
+        "msbgx_net_last_train_ms": (C.c_double, [P]),
+        "msbgx_net_apply": (C.c_int, [P, P, C.c_int, C.c_int]),
+        "msbgx_solve_pressure_learned": (C.c_int, [P, P, C.POINTER(SolveParams), C.POINTER(C.c_int),
+                                                   C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -199,8 +214,13 @@ EXPORTED = [
     "msbgx_vcycle", "msbgx_time_kernel", "msbgx_level_map_from_scene", "msbgx_set_precision",
     "msbgx_solve_pressure_refined", "msbgx_level0_blocks", "msbgx_batch_create", "msbgx_batch_size",
     "msbgx_batch_block_base", "msbgx_batch_solve", "msbgx_batch_residual_history", "msbgx_autotune",
-    "msbgx_set_launch_config",
+    "msbgx_set_launch_config", "msbgx_net_create", "msbgx_net_destroy", "msbgx_net_get_weights",
+    "msbgx_net_set_weights", "msbgx_net_set_samples", "msbgx_net_loss_grad", "msbgx_net_collect",
+    "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
+    "msbgx_solve_pressure_learned",
 ]
+
+NET_PARAMS = 14688
 
 
 class MsbgxError(RuntimeError):
@@ -380,6 +400,82 @@ class MultiresSparseGrid:
 
     def set_launch_config(self, graph_chunk, smoother_mask):
         _ck(lib.msbgx_set_launch_config(self._h, graph_chunk, smoother_mask), "set_launch_config")
+
+
+class LearnedNet:
+    """Handle over ``msbgx_net``: the learned preconditioner correction."""
+
+    def __init__(self, seed=1):
+        self._h = C.c_void_p()
+        _ck(lib.msbgx_net_create(C.byref(self._h), seed), "net_create")
+
+    def destroy(self):
+        if self._h:
+            lib.msbgx_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def weights(self):
+        w = np.zeros(NET_PARAMS, np.float32)
+        _ck(lib.msbgx_net_get_weights(self._h, _ptr(w)), "net_get_weights")
+        return w
+
+    @weights.setter
+    def weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == NET_PARAMS
+        _ck(lib.msbgx_net_set_weights(self._h, _ptr(w)), "net_set_weights")
+
+    def set_samples(self, x, target, mask):
+        self._samples = [np.ascontiguousarray(x, np.float32), np.ascontiguousarray(target, np.float32),
+                         np.ascontiguousarray(mask, np.uint8)]
+        n = self._samples[0].shape[0]
+        _ck(lib.msbgx_net_set_samples(self._h, n, *[_ptr(a) for a in self._samples]), "net_set_samples")
+
+    def loss_grad(self, want_out=False):
+        n = lib.msbgx_net_num_samples(self._h)
+        loss = C.c_double()
+        g = np.zeros(NET_PARAMS, np.float32)
+        out = np.zeros((n, 4096), np.float32) if want_out else None
+        _ck(lib.msbgx_net_loss_grad(self._h, C.byref(loss), _ptr(g), _ptr(out) if want_out else None), "net_loss_grad")
+        return loss.value, g, out
+
+    def collect(self, grid, every=1, max_pairs=100, blocks_per_pair=64, seed=1, tol=1e-7, max_iter=2000,
+                n_pre=3, n_post=3, n_coarse=24):
+        p = SolveParams(n_pre, n_post, n_coarse, max_iter, tol)
+        pairs = C.c_int()
+        _ck(lib.msbgx_net_collect(self._h, grid._h, C.byref(p), every, max_pairs, blocks_per_pair, seed,
+                                  C.byref(pairs)), "net_collect")
+        return pairs.value
+
+    @property
+    def num_samples(self):
+        return lib.msbgx_net_num_samples(self._h)
+
+    def train(self, steps, batch=256, lr=1e-3, seed=1):
+        a, b = C.c_double(), C.c_double()
+        _ck(lib.msbgx_net_train(self._h, steps, batch, lr, seed, C.byref(a), C.byref(b)), "net_train")
+        return a.value, b.value
+
+    def last_train_ms(self):
+        return lib.msbgx_net_last_train_ms(self._h)
+
+    def apply(self, grid, ch_in, ch_out):
+        _ck(lib.msbgx_net_apply(self._h, grid._h, ch_in, ch_out), "net_apply")
+
+    def solve(self, grid, n_pre=3, n_post=3, n_coarse=24, max_iter=500, tol=1e-6):
+        """MG-PCG with the learned correction; returns (iters, rel, fallback_iter)."""
+        p = SolveParams(n_pre, n_post, n_coarse, max_iter, tol)
+        it, rr, fb = C.c_int(), C.c_float(), C.c_int()
+        _ck(lib.msbgx_solve_pressure_learned(grid._h, self._h, C.byref(p), C.byref(it), C.byref(rr), C.byref(fb)),
+            "solve_learned")
+        return it.value, rr.value, fb.value
 
 
 class BatchGrid(MultiresSparseGrid):
