@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form (64 problems) runs")
     ap.add_argument("--batch-max-iter", type=int, default=400)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner A/B")
+    ap.add_argument("--learned-pairs", type=int, default=2000)
+    ap.add_argument("--learned-steps", type=int, default=2000)
+    ap.add_argument("--learned-eval-n", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true", help="launch PCG iterations eagerly (no hipGraph)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="only run a block copy of known size plus a few smoother passes (for rocprofv3 --pmc)")
@@ -199,6 +203,56 @@ def batched_runs(msbgx, args, log):
     return rec
 
 
+def learned_runs(msbgx, args, log):
+    """Learned-preconditioner form (dam-break workload): training pairs from
+    plain MG-PCG solves of seeded dam-break fields at 256^3 (box x<0.4, y<0.6
+    plus a sphere r~U[0.1,0.2]), every iterate sampled (`blocks_per_pair`
+    random level-0 blocks each) until --learned-pairs iterates; Adam on the
+    GPU; then plain vs learned MG-PCG to 1e-6 on unseen seeds at 512^3."""
+    t0 = time.time()
+    net = msbgx.LearnedNet(seed=2024)
+    pairs, seed = 0, 0
+    while pairs < args.learned_pairs:
+        seed += 1
+        c = msbgx.config_c4(256, seed=1000 + seed)
+        g = msbgx.build_grid(c)
+        g.prepare_pressure_solve_flip(scene=c.scene)
+        pairs += net.collect(g, every=1, max_pairs=args.learned_pairs - pairs, blocks_per_pair=16, seed=seed,
+                             tol=1e-7, max_iter=2000)
+        g.destroy()
+    t_collect = time.time() - t0
+    log(f"learned: {pairs} pairs from {seed} solves, {net.num_samples} block samples, {t_collect:.1f} s")
+    first, last = net.train(steps=args.learned_steps, batch=64, lr=3e-3, seed=7)
+    train_ms = net.last_train_ms()
+    log(f"learned: {args.learned_steps} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
+    evals = []
+    for s in (501, 502):
+        c = msbgx.config_c4(args.learned_eval_n, seed=s)
+        g = msbgx.build_grid(c)
+        g.prepare_pressure_solve_flip(scene=c.scene)
+        g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)     # warm-up
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it_p, rr_p = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+        ms_p = g.last_solve_ms()
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        net.solve(g, max_iter=args.max_iter, tol=args.tol)                # warm-up
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it_l, rr_l, fb = net.solve(g, max_iter=args.max_iter, tol=args.tol)
+        ms_l = g.last_solve_ms()
+        rec = {"seed": s, "n": c.n, "plain_ms": round(ms_p, 2), "plain_iterations": it_p, "plain_rel": rr_p,
+               "learned_ms": round(ms_l, 2), "learned_iterations": it_l, "learned_rel": rr_l,
+               "fallback_iteration": fb, "speedup": round(ms_p / ms_l, 3)}
+        log(f"learned: {rec}")
+        evals.append(rec)
+        g.destroy()
+    w = net.weights
+    net.destroy()
+    return {"train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
+                      "block_samples": int(pairs * 16), "collect_s": round(t_collect, 1), "steps": args.learned_steps,
+                      "batch": 64, "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
+            "eval": evals, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
+
+
 def read_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 --pmc summary (profiles/hbm_traffic.txt), or None."""
@@ -320,6 +374,8 @@ def main():
         line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
+    if not args.no_learned:
+        line["learned_form"] = learned_runs(msbgx, args, log)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)
