@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--n", type=int, default=0, help="override the domain size")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=2000)
-    ap.add_argument("--cpu-iters", type=int, default=3, help="PCG iterations in the CPU-baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="PCG iterations in the CPU-baseline sample (0: the whole solve to --tol)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP16/BF16 precision-form runs")
@@ -289,15 +290,21 @@ def cpu_baseline(cfg, args, gpu_iters, log):
     t0 = time.time()
     o.prepare_scene(cfg.scene)
     t_prep = time.time() - t0
-    k = max(1, args.cpu_iters)
+    k = args.cpu_iters if args.cpu_iters > 0 else max(1, gpu_iters)
     t0 = time.time()
     it, rr, _ = o.solve(max_iter=k, tol=args.tol)
     t_k = time.time() - t0
     # the solve runs one preconditioner application before the loop: count it as one iteration
     per_iter = t_k / (k + 1)
-    est = per_iter * (gpu_iters + 1)
-    log(f"cpu oracle: prepare {t_prep:.1f} s, {k} iterations {t_k:.1f} s, {per_iter*1e3:.1f} ms/iter, "
-        f"extrapolated to {gpu_iters} iterations: {est:.1f} s")
+    est = t_k if it == gpu_iters else per_iter * (gpu_iters + 1)
+    log(f"cpu oracle: prepare {t_prep:.1f} s, {it} iterations (rel {rr:.2e}) {t_k:.1f} s on {threads} threads, "
+        f"{per_iter*1e3:.1f} ms/iter; for the GPU's {gpu_iters} iterations: {est:.1f} s")
+    o1 = oracle_lib.build_oracle(cfg, threads=1)
+    o1.prepare_scene(cfg.scene)
+    t0 = time.time()
+    o1.solve(max_iter=1, tol=args.tol)
+    t1 = (time.time() - t0) / 2
+    log(f"cpu oracle, 1 thread: {t1*1e3:.0f} ms per iteration")
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -307,11 +314,16 @@ def cpu_baseline(cfg, args, gpu_iters, log):
                     break
     except OSError:
         pass
+    whole = it == gpu_iters
     return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "one_thread_ms_per_iteration": round(t1 * 1e3, 1),
             "sample": (f"oracle/msbg_oracle.cpp (C++ restatement of the reference solver, -O3 -march=native, OpenMP "
-                       f"over the blocks of a colour), {k} PCG iterations (+1 initial V-cycle) of the same solve "
-                       f"timed = {t_k:.1f} s on {threads} threads of {model}; extrapolated linearly to the "
-                       f"{gpu_iters} iterations the solve needs (setup {t_prep:.1f} s excluded)")}
+                       f"over the blocks of a colour) on {threads} threads of {model}: "
+                       + (f"the whole solve to {args.tol:g} ({it} PCG iterations, the GPU's count) timed"
+                          if whole else
+                          f"{it} PCG iterations (+1 initial V-cycle) timed = {t_k:.1f} s, extrapolated linearly "
+                          f"to the GPU's {gpu_iters} iterations")
+                       + f"; setup {t_prep:.1f} s excluded; 1 thread: {t1*1e3:.0f} ms per iteration")}
 
 
 def main():
