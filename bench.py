@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--learned-pairs", type=int, default=2000)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--learned-eval-n", type=int, default=512)
-    ap.add_argument("--no-graph", action="store_true", help="launch PCG iterations eagerly (no hipGraph)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch PCG iterations eagerly (no hipGraph) in every section (profiling runs)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="only run a block copy of known size plus a few smoother passes (for rocprofv3 --pmc)")
     ap.add_argument("--literal", action="store_true", help="run the V-cycle residual/transfer work even when "
@@ -67,7 +68,9 @@ def make_cfg(msbgx, name, n):
 def gpu_run(msbgx, cfg, args, log):
     t0 = time.time()
     g = msbgx.build_grid(cfg)
-    g.set_options((msbgx.OPT_NO_GRAPH if args.no_graph else 0) | (msbgx.OPT_LITERAL_VCYCLE if args.literal else 0))
+    opts = (msbgx.OPT_NO_GRAPH if args.no_graph else 0) | (msbgx.OPT_LITERAL_VCYCLE if args.literal else 0)
+    if opts:
+        g.set_options(opts)
     g.prepare_pressure_solve_flip(scene=cfg.scene)
     log(f"prepare {cfg.name} n={cfg.n}: {time.time() - t0:.1f} s")
     lv, fl = g.block_info(0)
@@ -246,12 +249,17 @@ def learned_runs(msbgx, args, log):
         log(f"learned: {rec}")
         evals.append(rec)
         g.destroy()
+    ms_c, fl_c = net.time_conv(blocks=2048, reps=20)
+    conv = {"kernel": "k_nn_conv16<false> (16->16, v_mfma_f32_16x16x32_bf16)", "blocks": 2048,
+            "launch_ms": round(ms_c, 4), "achieved": round(fl_c / (ms_c * 1e-3) / 1e12, 1),
+            "peak": 2500.0, "unit": "TFLOP/s", "frac": round(fl_c / (ms_c * 1e-3) / 1e12 / 2500.0, 4)}
+    log(f"learned: conv16 roofline {conv}")
     w = net.weights
     net.destroy()
     return {"train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
                       "block_samples": int(pairs * 16), "collect_s": round(t_collect, 1), "steps": args.learned_steps,
                       "batch": 64, "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
-            "eval": evals, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
+            "eval": evals, "conv16_roofline": conv, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
 
 
 def read_traffic(kernel, workload):
@@ -333,6 +341,8 @@ def main():
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.no_graph:
+        os.environ["MSBGX_NO_GRAPH"] = "1"   # every handle created below launches eagerly
     import msbgx
     cfg = make_cfg(msbgx, args.config, args.n)
     if args.pmc_calibration:

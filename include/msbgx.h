@@ -204,6 +204,10 @@ double msbgx_net_last_train_ms(const msbgx_net *n);
 /* Per-kernel entry point: ch_out += N(ch_in) on the level-0 PCG blocks
  * (MG level 0 channels; the halo is read from level-0 neighbours). */
 int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out);
+/* Measurement: mean duration of one forward 16 -> 16 MFMA layer launch over
+ * `blocks` patches (HIP events, `reps` launches) and its algorithmic FLOP
+ * (blocks * 18^3 * 27 * 16 * 16 * 2). */
+int msbgx_net_time_conv(msbgx_net *n, int blocks, int reps, double *ms_per_launch, double *flop_per_launch);
 /* MG-PCG with the learned correction (flexible beta).  A non-finite network
  * output or a residual that drops by less than 10% over 5 iterations turns
  * the correction off for the rest of the solve and restarts the plain

@@ -7,6 +7,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -80,6 +81,15 @@ using namespace msbgx;
 
 struct msbgx_grid : Grid {};
 
+/* MSBGX_NO_GRAPH=1 in the environment makes new handles launch PCG
+ * iterations eagerly (profiling runs whose tracer does not handle graph
+ * capture). */
+static unsigned defaultOptions()
+{
+  const char *e = getenv("MSBGX_NO_GRAPH");
+  return (e && e[0] == '1') ? MSBGX_OPT_NO_GRAPH : 0u;
+}
+
 /* Single-grid operations that have no meaning on a batch handle. */
 static int checkSingle(const msbgx_grid *g)
 {
@@ -126,6 +136,7 @@ int msbgx_create(msbgx_grid **out, int sx0, int sy0, int sz0, int bsx0, int n_le
   for (int i = 0; i < 3; i++) g->domainBc[i][0] = g->domainBc[i][1] = MSBGX_DBC_OPEN;
   std::vector<int> zero(g->nBlocks, 0);
   setRefinementMap(*g, zero.data());
+  g->options = defaultOptions();
   *out = g;
   return MSBGX_OK;
 }
@@ -402,6 +413,7 @@ int msbgx_batch_create(msbgx_grid **out, msbgx_grid *const *problems, int n)
     return rc;
   }
   m->prepared = true;
+  m->options = defaultOptions();
   *out = m;
   return MSBGX_OK;
 }
@@ -531,6 +543,12 @@ int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out)
     return MSBGX_EINVAL;
   }
   return netApply(*netOf(n), *g, ch_in, ch_out);
+}
+
+int msbgx_net_time_conv(msbgx_net *n, int blocks, int reps, double *ms_per_launch, double *flop_per_launch)
+{
+  if (!n || blocks < 1 || reps < 1 || !ms_per_launch || !flop_per_launch) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netTimeConv(*netOf(n), blocks, reps, ms_per_launch, flop_per_launch);
 }
 
 int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_params *p, int *iters, float *rel,

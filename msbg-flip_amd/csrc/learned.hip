@@ -859,6 +859,37 @@ int netTrain(Net &n, int steps, int batch, float lr, uint64_t seed, double *loss
 int netNumSamples(const Net &n) { return n.nS; }
 double netLastTrainMs(const Net &n) { return n.lastTrainMs; }
 
+/* Measurement: `reps` launches of the forward 16 -> 16 MFMA layer over `nb`
+ * patches (the network's current weights, zero activations) between HIP
+ * events on the network's stream; algorithmic FLOP per launch counts the
+ * 18^3 patch positions: nb * 5832 * 27 * 16 * 16 * 2. */
+int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLaunch)
+{
+  NetBuffers b;
+  if (int rc = allocBuffers(b, nb, false, n.stream)) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int i = 0; i < 2; i++)
+    hipLaunchKernelGGL(k_nn_conv16<false>, dim3(nb, nn::NZG), dim3(256), 0, n.stream, b.A1, (const uint4 *)n.p2,
+                       (const uint16_t *)nullptr, b.A2, (const int *)nullptr);
+  HIPCHK(hipEventRecord(e0, n.stream));
+  for (int i = 0; i < reps; i++)
+    hipLaunchKernelGGL(k_nn_conv16<false>, dim3(nb, nn::NZG), dim3(256), 0, n.stream, b.A1, (const uint4 *)n.p2,
+                       (const uint16_t *)nullptr, b.A2, (const int *)nullptr);
+  HIPCHK(hipEventRecord(e1, n.stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  freeBuffers(b);
+  *msPerLaunch = ms / reps;
+  *flopPerLaunch = (double)nb * 5832.0 * nn::T * nn::C * nn::C * 2.0;
+  return MSBGX_OK;
+}
+
 /* ------------------------------------------------- PCG with the correction */
 
 struct LearnedCtx {

@@ -161,6 +161,7 @@ int netNumSamples(const Net &n);
 double netLastTrainMs(const Net &n);
 int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters, float *rel, int *fellBack);
 int netApply(Net &n, Grid &g, int chIn, int chOut);
+int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLaunch);
 /* batched block sets (batch.cpp / solver.hip) */
 int buildBatchGrid(Grid &merged, const std::vector<const Grid *> &probs);
 int deviceUploadBatch(Grid &merged, const std::vector<const Grid *> &probs);

@@ -192,6 +192,7 @@ def _load():
                                       C.POINTER(C.c_double)]),
         "msbgx_net_last_train_ms": (C.c_double, [P]),
         "msbgx_net_apply": (C.c_int, [P, P, C.c_int, C.c_int]),
+        "msbgx_net_time_conv": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "msbgx_solve_pressure_learned": (C.c_int, [P, P, C.POINTER(SolveParams), C.POINTER(C.c_int),
                                                    C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     }
@@ -217,7 +218,7 @@ EXPORTED = [
     "msbgx_set_launch_config", "msbgx_net_create", "msbgx_net_destroy", "msbgx_net_get_weights",
     "msbgx_net_set_weights", "msbgx_net_set_samples", "msbgx_net_loss_grad", "msbgx_net_collect",
     "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
-    "msbgx_solve_pressure_learned",
+    "msbgx_solve_pressure_learned", "msbgx_net_time_conv",
 ]
 
 NET_PARAMS = 14688
@@ -465,6 +466,12 @@ class LearnedNet:
 
     def last_train_ms(self):
         return lib.msbgx_net_last_train_ms(self._h)
+
+    def time_conv(self, blocks=1024, reps=20):
+        """(ms per launch, algorithmic FLOP per launch) of the forward MFMA layer."""
+        ms, fl = C.c_double(), C.c_double()
+        _ck(lib.msbgx_net_time_conv(self._h, blocks, reps, C.byref(ms), C.byref(fl)), "net_time_conv")
+        return ms.value, fl.value
 
     def apply(self, grid, ch_in, ch_out):
         _ck(lib.msbgx_net_apply(self._h, grid._h, ch_in, ch_out), "net_apply")
