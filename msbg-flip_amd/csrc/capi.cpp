@@ -228,7 +228,8 @@ int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene)
     return MSBGX_EINVAL;
   }
   if (scene->n_spheres < 0 || scene->n_spheres > MSBGX_SCENE_MAX_SPHERES || scene->n_boxes < 0 ||
-      scene->n_boxes > MSBGX_SCENE_MAX_BOXES || !(scene->dt != 0.0f)) {
+      scene->n_boxes > MSBGX_SCENE_MAX_BOXES || !(scene->dt != 0.0f) || scene->particles_per_cell < 0 ||
+      scene->particles_per_cell > 64) {
     setError("invalid scene");
     return MSBGX_EINVAL;
   }

@@ -40,6 +40,7 @@ class Scene(C.Structure):
         ("n_spheres", C.c_int32), ("n_boxes", C.c_int32), ("wall_faces_closed", C.c_int32),
         ("seed", C.c_uint64), ("dt", C.c_float), ("vel_scale", C.c_float),
         ("sphere", (C.c_double * 4) * MAX_SPHERES), ("box", (C.c_double * 6) * MAX_BOXES),
+        ("particles_per_cell", C.c_int32), ("particle_sigma", C.c_float),
     ]
 
 
@@ -55,7 +56,8 @@ class Callbacks(C.Structure):
                 ("user", C.c_void_p)]
 
 
-def make_scene(n, spheres=(), boxes=(), seed=1234, dt=1.0, vel_scale=1.0, walls_closed=True):
+def make_scene(n, spheres=(), boxes=(), seed=1234, dt=1.0, vel_scale=1.0, walls_closed=True,
+               particles_per_cell=0, particle_sigma=0.5):
     """Build a Scene for an n^3 domain (n may be a 3-tuple)."""
     s = Scene()
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
@@ -71,6 +73,7 @@ def make_scene(n, spheres=(), boxes=(), seed=1234, dt=1.0, vel_scale=1.0, walls_
             s.box[i][k] = float(b[k])
     s.seed, s.dt, s.vel_scale = int(seed), float(dt), float(vel_scale)
     s.wall_faces_closed = 1 if walls_closed else 0
+    s.particles_per_cell, s.particle_sigma = int(particles_per_cell), float(particle_sigma)
     return s
 
 
@@ -127,10 +130,13 @@ def config_c4(n=256, seed=1234):
     return Config("c4_dam_break", n, 16, 3, s, level_mode=0, level_d0=1)
 
 
-def config_c5(n=1024, seed=1234):
-    """Full scale: dam-break box plus sphere r=0.15 at (0.7,0.8,0.5), 32^3 blocks, 4 levels."""
-    s = make_scene(n, spheres=[(0.7, 0.8, 0.5, 0.15)], boxes=[(0.0, 0.4, 0.0, 0.6, 0.0, 1.0)], seed=seed)
-    return Config("c5_dam_break_full", n, 32, 4, s, level_mode=0, level_d0=1)
+def config_c5(n=1024, seed=1234, bsx=32, levels=4):
+    """Full scale: dam-break box plus sphere r=0.15 at (0.7,0.8,0.5), 32^3 blocks, 4 levels; the
+    right-hand side is the divergence of the particle-to-grid velocity of 8 seeded particles per
+    liquid cell (uniform positions, velocity sigma 0.5)."""
+    s = make_scene(n, spheres=[(0.7, 0.8, 0.5, 0.15)], boxes=[(0.0, 0.4, 0.0, 0.6, 0.0, 1.0)], seed=seed,
+                   particles_per_cell=8, particle_sigma=0.5)
+    return Config("c5_dam_break_particles", n, bsx, levels, s, level_mode=0, level_d0=1)
 
 
 def _load():

@@ -18,6 +18,7 @@ SMALL_CONFIGS = {
     "c4_128_multires": lambda: msbgx.config_c4(n=128),
     "c2_128_bsx16": lambda: msbgx.config_c2(n=128, bsx=16),
     "c2_256_bsx32": lambda: msbgx.config_c2(n=256, bsx=32),
+    "c5_128_particles_bsx32": lambda: msbgx.config_c5(n=128),
 }
 
 
