@@ -29,7 +29,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c4", "c5", "c5b16"],
+                    help="c5b16: the full-scale particle workload on 16^3 base blocks")
     ap.add_argument("--n", type=int, default=0, help="override the domain size")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=2000)
@@ -62,6 +63,8 @@ def make_cfg(msbgx, name, n):
         return msbgx.config_c2(n or 512, 32)
     if name == "c4":
         return msbgx.config_c4(n or 256)
+    if name == "c5b16":
+        return msbgx.config_c5(n or 1024, bsx=16, levels=4)
     return msbgx.config_c5(n or 1024)
 
 
