@@ -7,6 +7,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -65,6 +66,18 @@ int prepareWith(Grid &g, Sampler &s, const msbgx_scene *scene)
   std::vector<float> rhs;
   int rc = buildMetadata(g, s, rhs);
   if (rc) return rc;
+  /* non-finite inputs are rejected here instead of surfacing as a PCG
+   * breakdown (the reference would propagate them, src/msbg3.cpp:3168-3171) */
+  for (int L = 0; L < g.nLevels; L++) {
+    const MgLevel &m = g.mg[L];
+    for (int d = 0; d < 3; d++)
+      for (float v : m.W[d])
+        if (!std::isfinite(v)) { setError("non-finite face coefficient"); return MSBGX_EINVAL; }
+    for (float v : m.D)
+      if (!std::isfinite(v)) { setError("non-finite diagonal (face coefficients)"); return MSBGX_EINVAL; }
+  }
+  for (float v : rhs)
+    if (!std::isfinite(v)) { setError("non-finite right-hand side"); return MSBGX_EINVAL; }
   g.hasScene = scene != nullptr;
   if (scene) g.scene = *scene;
   g.hostRhs = rhs;

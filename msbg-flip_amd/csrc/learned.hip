@@ -1072,6 +1072,7 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
     for (int i = 0; i < hs.iter; i++) g.history[i] = std::sqrt(rr[i] / hs.bnorm2);
   }
   *fellBack = hs.learned ? -1 : hs.fellBack;
+  if (hs.brk == 3) { setError("non-finite right-hand side"); return MSBGX_EINVAL; }
   if (hs.brk == 2) { *iters = 0; *rel = 0.0f; return MSBGX_OK; }
   *rel = (float)std::sqrt(fin / hs.bnorm2);
   *iters = hs.conv > 0 ? hs.conv : p.max_iter;

@@ -739,8 +739,8 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
   switch (op) {
     case FIN_BNORM:
       st->bnorm2 = s;
-      st->done = (s <= 1e-30) ? 1 : 0;
-      st->conv = -1; st->iter = 0; st->brk = (s <= 1e-30) ? 2 : 0;
+      st->done = (s <= 1e-30 || !isfinite(s)) ? 1 : 0;
+      st->conv = -1; st->iter = 0; st->brk = !isfinite(s) ? 3 : ((s <= 1e-30) ? 2 : 0);
       break;
     case FIN_RHO0:
       st->rho = s;
@@ -1295,6 +1295,10 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
     std::vector<double> rr(hs.iter);
     HIPCHK(hipMemcpy(rr.data(), d.hist, hs.iter * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < hs.iter; i++) g.history[i] = std::sqrt(rr[i] / hs.bnorm2);
+  }
+  if (hs.brk == 3) {
+    setError("non-finite right-hand side");
+    return MSBGX_EINVAL;
   }
   if (hs.brk == 2) {  // |b|^2 <= 1e-30
     *iters = 0;
@@ -1998,6 +2002,7 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
   for (int q = 0; q < P; q++) {
     for (int i = 0; i < hs[q].iter; i++)
       m.probHistory[q].push_back(std::sqrt(hh[(size_t)q * b.histStride + i] / hs[q].bnorm2));
+    if (hs[q].brk == 3) { setError("non-finite right-hand side in a batched problem"); return MSBGX_EINVAL; }
     if (hs[q].brk == 2) { iters[q] = 0; rel[q] = 0.0f; continue; }
     rel[q] = (float)std::sqrt(fin[q] / hs[q].bnorm2);
     iters[q] = hs[q].conv > 0 ? hs[q].conv : p.max_iter;

@@ -103,3 +103,24 @@ def test_scene_functions_host_device_contract():
     center = (1 + 4 * (1 + 4 * 1))                       # block (1,1,1) holds the sphere centre
     fl = o.get_block(11, 0, center, 8).reshape(8, 8, 8)
     assert (fl[7, 7, 7] & msbgx.CELL_VOID) == 0
+
+
+@pytest.mark.parametrize("what", ["face", "rhs", "inf_face"])
+def test_non_finite_inputs_are_rejected_before_upload(what):
+    """NaN / Inf in the face coefficients or the right-hand side is an
+    argument error (reported before any device work)."""
+    n = 32
+    g = msbgx.MultiresSparseGrid.create(n, n, n, 16, 2)
+    flags = np.zeros((n, n, n), np.uint16)
+    fu, fv, fw = (np.ones((n, n, n), np.float32) for _ in range(3))
+    rhs = np.zeros((n, n, n), np.float32)
+    rhs[5, 5, 5] = 1.0
+    if what == "face":
+        fv[3, 4, 5] = np.nan
+    elif what == "inf_face":
+        fu[7, 7, 7] = np.inf
+    else:
+        rhs[9, 9, 9] = np.nan
+    with pytest.raises(msbgx.MsbgxError) as e:
+        g.prepare_pressure_solve_flip(dense=(flags, fu, fv, fw, rhs))
+    assert e.value.rc == -1 and "non-finite" in str(e.value)
