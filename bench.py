@@ -210,14 +210,14 @@ def batched_runs(msbgx, args, log):
     return rec
 
 
-def learned_runs(msbgx, args, log):
+def learned_runs(msbgx, args, log, mode="fine"):
     """Learned-preconditioner form (dam-break workload): training pairs from
     plain MG-PCG solves of seeded dam-break fields at 256^3 (box x<0.4, y<0.6
     plus a sphere r~U[0.1,0.2]), every iterate sampled (`blocks_per_pair`
     random level-0 blocks each) until --learned-pairs iterates; Adam on the
     GPU; then plain vs learned MG-PCG to 1e-6 on unseen seeds at 512^3."""
     t0 = time.time()
-    net = msbgx.LearnedNet(seed=2024)
+    net = msbgx.LearnedNet(seed=2024, mode=msbgx.NET_COARSE if mode == "coarse" else msbgx.NET_FINE)
     pairs, seed = 0, 0
     while pairs < args.learned_pairs:
         seed += 1
@@ -225,13 +225,14 @@ def learned_runs(msbgx, args, log):
         g = msbgx.build_grid(c)
         g.prepare_pressure_solve_flip(scene=c.scene)
         pairs += net.collect(g, every=1, max_pairs=args.learned_pairs - pairs, blocks_per_pair=16, seed=seed,
-                             tol=1e-7, max_iter=2000)
+                             tol=1e-7, max_iter=2000)   # coarse form: one sample per 16^3-block tile
         g.destroy()
     t_collect = time.time() - t0
-    log(f"learned: {pairs} pairs from {seed} solves, {net.num_samples} block samples, {t_collect:.1f} s")
-    first, last = net.train(steps=args.learned_steps, batch=64, lr=3e-3, seed=7)
+    log(f"learned ({mode}): {pairs} pairs from {seed} solves, {net.num_samples} samples, {t_collect:.1f} s")
+    net_samples = net.num_samples
+    first, last = net.train(steps=args.learned_steps, batch=64 if mode == "fine" else 16, lr=3e-3, seed=7)
     train_ms = net.last_train_ms()
-    log(f"learned: {args.learned_steps} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
+    log(f"learned ({mode}): {args.learned_steps} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
     evals = []
     for s in (501, 502):
         c = msbgx.config_c4(args.learned_eval_n, seed=s)
@@ -249,7 +250,7 @@ def learned_runs(msbgx, args, log):
         rec = {"seed": s, "n": c.n, "plain_ms": round(ms_p, 2), "plain_iterations": it_p, "plain_rel": rr_p,
                "learned_ms": round(ms_l, 2), "learned_iterations": it_l, "learned_rel": rr_l,
                "fallback_iteration": fb, "speedup": round(ms_p / ms_l, 3)}
-        log(f"learned: {rec}")
+        log(f"learned ({mode}): {rec}")
         evals.append(rec)
         g.destroy()
     ms_c, fl_c = net.time_conv(blocks=2048, reps=20)
@@ -259,8 +260,9 @@ def learned_runs(msbgx, args, log):
     log(f"learned: conv16 roofline {conv}")
     w = net.weights
     net.destroy()
-    return {"train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
-                      "block_samples": int(pairs * 16), "collect_s": round(t_collect, 1), "steps": args.learned_steps,
+    return {"form": mode,
+            "train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
+                      "samples": net_samples, "collect_s": round(t_collect, 1), "steps": args.learned_steps,
                       "batch": 64, "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
             "eval": evals, "conv16_roofline": conv, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
 
@@ -400,7 +402,8 @@ def main():
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:
-        line["learned_form"] = learned_runs(msbgx, args, log)
+        line["learned_form"] = learned_runs(msbgx, args, log, "fine")
+        line["learned_coarse_form"] = learned_runs(msbgx, args, log, "coarse")
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)

@@ -176,6 +176,17 @@ int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask);
  * 3 (dy+1) + (dx+1).  Grids with 16^3 base blocks only. */
 #define MSBGX_NET_PARAMS 14688
 typedef struct msbgx_net msbgx_net;
+/* Where the network acts (msbgx_net_set_mode; the training set is cleared
+ * when the form changes):
+ *   MSBGX_NET_FINE    z = V(r) + N(r - A V(r)) on every level-0 16^3 block;
+ *   MSBGX_NET_COARSE  a coarse correction: the network runs on the grid of
+ *                     block means of r (one cell per block, tiled in 16^3
+ *                     tiles with a one-cell halo), its output is extended
+ *                     constant over each level-0 PCG block's unknowns (zc),
+ *                     and z = zc + V(r - A zc).  Trained on block means of
+ *                     r_k (input) and of x* - x_k over the unknowns (target). */
+#define MSBGX_NET_FINE 0
+#define MSBGX_NET_COARSE 1
 /* He-normal initial weights from `seed`. */
 int msbgx_net_create(msbgx_net **out, uint64_t seed);
 void msbgx_net_destroy(msbgx_net *n);
@@ -195,6 +206,12 @@ int msbgx_net_loss_grad(msbgx_net *n, double *loss, float *grad, float *out);
  * iterates sampled in *pairs.  The grid's pressure is zero afterwards. */
 int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, int every, int max_pairs,
                       int blocks_per_pair, uint64_t seed, int *pairs);
+int msbgx_net_set_mode(msbgx_net *n, int mode);
+int msbgx_net_get_mode(const msbgx_net *n);
+/* Stall test of msbgx_solve_pressure_learned: fall back when |r| drops by
+ * less than `drop` (relative) over `window` iterations (0 <= window <= 16;
+ * 0 disables the test; default 5 and 0.1). */
+int msbgx_net_set_stall(msbgx_net *n, int window, float drop);
 int msbgx_net_num_samples(const msbgx_net *n);
 /* Adam on random minibatches of `batch` samples; first-step and smoothed
  * last-step losses; msbgx_net_last_train_ms gives the GPU time. */
@@ -209,9 +226,10 @@ int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out);
  * (blocks * 18^3 * 27 * 16 * 16 * 2). */
 int msbgx_net_time_conv(msbgx_net *n, int blocks, int reps, double *ms_per_launch, double *flop_per_launch);
 /* MG-PCG with the learned correction (flexible beta).  A non-finite network
- * output or a residual that drops by less than 10% over 5 iterations turns
- * the correction off for the rest of the solve and restarts the plain
- * MG-PCG; *fallback_iter is the iteration where that happened, -1 if never. */
+ * output, a corrected direction with r.z <= 0, or a stalled residual (see
+ * msbgx_net_set_stall) turns the correction off for the rest of the solve
+ * and restarts the plain MG-PCG; *fallback_iter is the iteration where that
+ * happened, -1 if never. */
 int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_params *p, int *iters,
                                  float *rel_residual, int *fallback_iter);
 

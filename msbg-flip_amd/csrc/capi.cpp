@@ -537,6 +537,20 @@ int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, 
   return netCollect(*netOf(n), *g, *p, every, max_pairs, blocks_per_pair, seed, pairs);
 }
 
+int msbgx_net_set_mode(msbgx_net *n, int mode)
+{
+  if (!n || (mode != MSBGX_NET_FINE && mode != MSBGX_NET_COARSE)) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netSetMode(*netOf(n), mode);
+}
+
+int msbgx_net_set_stall(msbgx_net *n, int window, float drop)
+{
+  if (!n || window < 0 || window > 16 || !(drop >= 0.0f && drop < 1.0f)) { setError("bad argument"); return MSBGX_EINVAL; }
+  return netSetStall(*netOf(n), window, drop);
+}
+
+int msbgx_net_get_mode(const msbgx_net *n) { return n ? netMode(*reinterpret_cast<const Net *>(n)) : -1; }
+
 int msbgx_net_num_samples(const msbgx_net *n) { return n ? netNumSamples(*reinterpret_cast<const Net *>(n)) : 0; }
 
 int msbgx_net_train(msbgx_net *n, int steps, int batch, float lr, uint64_t seed, double *loss_first,

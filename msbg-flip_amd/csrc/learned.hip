@@ -533,6 +533,172 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_axpy(float *__restrict__ d, 
     d[i] = fmaf(beta, d[i], zz[i]);
 }
 
+/* ------------------------------------------------ coarse-correction form */
+
+/* dst[i] = v over [0, n) unless *skip */
+__global__ void k_nn_fill(float *__restrict__ dst, int n, float v, const int *skip)
+{
+  if (skip && *skip) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
+}
+
+/* coarse[blocks[i]] = mean of src over the block's cells (all cells, or the
+ * unknowns only with `unknownOnly`; 0 when there are none) */
+__global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__ src, const int *__restrict__ blocks,
+                                                       const int64_t *__restrict__ off,
+                                                       const uint16_t *__restrict__ flags,
+                                                       const float *__restrict__ D, int unknownOnly,
+                                                       float *__restrict__ coarse, const int *skip)
+{
+  if (skip && *skip) return;
+  __shared__ double red[4];
+  const int bid = blocks[blockIdx.x];
+  const int64_t o = off[bid];
+  double acc = 0.0, cnt = 0.0;
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
+    if (unknownOnly && !unk) continue;
+    acc += (double)src[o + c];
+    cnt += 1.0;
+  }
+  const double sum = blockReduceSum(acc, red);
+  const double num = blockReduceSum(cnt, red);
+  if (threadIdx.x == 0) coarse[bid] = num > 0.0 ? (float)(sum / num) : 0.0f;
+}
+
+/* Patch of tile t of the dense block-resolution grid: tile (tx, ty, tz)
+ * covers coarse cells 16 t .. 16 t + 15 per axis; cells outside the raster
+ * read 0.  Optionally divides by scale[t]. */
+__global__ void __launch_bounds__(256) k_nn_gather_dense(const float *__restrict__ coarse, int nbx, int nby, int nbz,
+                                                         int ntx, int nty, const float *__restrict__ scale,
+                                                         uint16_t *__restrict__ X0, const int *skip)
+{
+  if (skip && *skip) return;
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const float sc = scale ? scale[t] : 1.0f;
+  uint16_t *dst = X0 + (int64_t)t * nn::PS + nn::SL;
+  for (int pos = threadIdx.x; pos < nn::NPOS; pos += blockDim.x) {
+    const int u = pos % nn::E, v = (pos / nn::E) % nn::E, w = pos / nn::PL;
+    float val = 0.0f;
+    if (u >= 1 && u <= 18 && v >= 1 && v <= 18 && w >= 1 && w <= 18) {
+      const int cx = 16 * tx + u - 2, cy = 16 * ty + v - 2, cz = 16 * tz + w - 2;
+      if (cx >= 0 && cy >= 0 && cz >= 0 && cx < nbx && cy < nby && cz < nbz)
+        val = coarse[cx + nbx * (cy + nby * cz)] * sc;
+    }
+    dst[pos] = nnF2bf(val);
+  }
+}
+
+/* Last layer on the tiles of the dense grid: out[cell] for the coarse cells
+ * inside the raster (grid: tiles x 8 cell-plane pairs). */
+__global__ void __launch_bounds__(256) k_nn_out_dense(const uint16_t *__restrict__ A3, const float *__restrict__ w4,
+                                                      int nbx, int nby, int nbz, int ntx, int nty,
+                                                      float *__restrict__ out, const int *skip)
+{
+  if (skip && *skip) return;
+  __shared__ uint4 lds[nn::LDS_POS * 2];
+  __shared__ float w[nn::T * nn::C];
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int cz0 = 2 * blockIdx.y, z0 = cz0 + 2;
+  nnStage16(lds, A3 + (int64_t)t * nn::PS * nn::C, z0);
+  for (int k = threadIdx.x; k < nn::T * nn::C; k += blockDim.x) w[k] = w4[k];
+  __syncthreads();
+  const int ldsOrigin = (z0 - 1) * nn::PL - 32;
+  for (int q = threadIdx.x; q < 2 * 256; q += blockDim.x) {
+    const int lx = q & 15, ly = (q >> 4) & 15, lz = cz0 + (q >> 8);
+    const int pos = (lz + 2) * nn::PL + (ly + 2) * nn::E + (lx + 2);
+    float acc = 0.0f;
+#pragma unroll
+    for (int tp = 0; tp < nn::T; tp++) {
+      const int lp = pos + nn::tapOff(tp) - ldsOrigin;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint4 v = lds[lp * 2 + h];
+        const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          acc = fmaf(w[tp * nn::C + 8 * h + 2 * e], nnBf2f((uint16_t)(wv[e] & 0xffffu)), acc);
+          acc = fmaf(w[tp * nn::C + 8 * h + 2 * e + 1], nnBf2f((uint16_t)(wv[e] >> 16)), acc);
+        }
+      }
+    }
+    const int cx = 16 * tx + lx, cy = 16 * ty + ly, cz = 16 * tz + lz;
+    if (cx < nbx && cy < nby && cz < nbz) out[cx + nbx * (cy + nby * cz)] = acc;
+  }
+}
+
+/* zc = coarse[bid] on the unknowns of each PCG block (0 elsewhere in the
+ * block); a non-finite value is dropped and flagged. */
+__global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restrict__ coarse,
+                                                           const int *__restrict__ blocks,
+                                                           const int64_t *__restrict__ off,
+                                                           const uint16_t *__restrict__ flags,
+                                                           const float *__restrict__ D, float *__restrict__ zc,
+                                                           PcgState *st)
+{
+  if (st->skipNet) return;
+  const int bid = blocks[blockIdx.x];
+  const int64_t o = off[bid];
+  float v = coarse[bid];
+  if (!isfinite(v)) {
+    if (threadIdx.x == 0) st->bad = 1;
+    v = 0.0f;
+  }
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
+    zc[o + c] = unk ? v : 0.0f;
+  }
+}
+
+/* z += zc over [0, n) unless *skip */
+__global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, int64_t n, const int *skip)
+{
+  if (*skip) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    z[i] += zc[i];
+}
+
+/* Coarse-form training sample of tile t: input patch of the dense residual
+ * means, target = dense error means on PCG blocks (mask 1 there), both
+ * divided by the target's RMS over the tile. */
+__global__ void __launch_bounds__(256) k_nn_tile_target(const float *__restrict__ cerr,
+                                                        const uint8_t *__restrict__ cmask, int nbx, int nby, int nbz,
+                                                        int ntx, int nty, float *__restrict__ scale,
+                                                        float *__restrict__ tgt, uint8_t *__restrict__ tmask)
+{
+  __shared__ double red[4];
+  __shared__ float sc;
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  auto cellOf = [&](int c, int &idx) {
+    const int cx = 16 * tx + (c & 15), cy = 16 * ty + ((c >> 4) & 15), cz = 16 * tz + (c >> 8);
+    idx = cx + nbx * (cy + nby * cz);
+    return cx < nbx && cy < nby && cz < nbz && cmask[idx];
+  };
+  double acc = 0.0, cnt = 0.0;
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    int idx;
+    if (!cellOf(c, idx)) continue;
+    acc += (double)cerr[idx] * (double)cerr[idx];
+    cnt += 1.0;
+  }
+  const double ss = blockReduceSum(acc, red);
+  const double nm = blockReduceSum(cnt, red);
+  if (threadIdx.x == 0) {
+    sc = (ss > 0.0 && nm > 0.0) ? (float)(1.0 / sqrt(ss / nm)) : 0.0f;
+    scale[t] = sc;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+    int idx;
+    const bool ok = cellOf(c, idx) && sc > 0.0f;
+    tgt[(int64_t)t * nn::NCELL + c] = ok ? cerr[idx] * sc : 0.0f;
+    tmask[(int64_t)t * nn::NCELL + c] = ok ? 1 : 0;
+  }
+}
+
 /* ------------------------------------------------------------------ host */
 
 struct NetBuffers {
@@ -546,6 +712,9 @@ struct NetBuffers {
 };
 
 struct Net {
+  int mode = MSBGX_NET_FINE;                     // MSBGX_NET_FINE or MSBGX_NET_COARSE
+  int stallWin = 5;                              // fallback when |r| drops by less than
+  float stallDrop = 0.1f;                        // stallDrop over stallWin iterations
   hipStream_t stream = nullptr;
   std::vector<float> host;                       // FP32 parameters (host mirror)
   float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
@@ -857,6 +1026,22 @@ int netTrain(Net &n, int steps, int batch, float lr, uint64_t seed, double *loss
 }
 
 int netNumSamples(const Net &n) { return n.nS; }
+int netSetMode(Net &n, int mode)
+{
+  if (mode != n.mode) {
+    n.mode = mode;
+    n.nS = 0;               // samples of the other form are not reused
+    n.maskCount.clear();
+  }
+  return MSBGX_OK;
+}
+int netMode(const Net &n) { return n.mode; }
+int netSetStall(Net &n, int window, float drop)
+{
+  n.stallWin = window;
+  n.stallDrop = drop;
+  return MSBGX_OK;
+}
 double netLastTrainMs(const Net &n) { return n.lastTrainMs; }
 
 /* Measurement: `reps` launches of the forward 16 -> 16 MFMA layer over `nb`
@@ -893,12 +1078,16 @@ int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLa
 /* ------------------------------------------------- PCG with the correction */
 
 struct LearnedCtx {
-  NetBuffers ib;                // inference buffers for the level-0 PCG blocks
+  NetBuffers ib;                // inference buffers for the level-0 PCG blocks (or the coarse tiles)
   float *zprev = nullptr, *zplain = nullptr;
+  /* coarse form: dense block-resolution grids, tile counts, the prolonged
+   * correction and the residual it leaves (MG0 pool size) */
+  float *cin = nullptr, *cout = nullptr, *zc = nullptr, *rs = nullptr;
+  int ntx = 0, nty = 0, ntz = 0, ncoarse = 0;
   int64_t n0 = 0;
   const Net *graphNet = nullptr;   // net whose buffers the captured graph uses
   hipGraphExec_t graph = nullptr;
-  int graphChunk = 0;
+  int graphChunk = 0, graphMode = -1;
 };
 
 static void freeLearned(LearnedCtx *c)
@@ -907,6 +1096,8 @@ static void freeLearned(LearnedCtx *c)
   freeBuffers(c->ib);
   if (c->zprev) (void)hipFree(c->zprev);
   if (c->zplain) (void)hipFree(c->zplain);
+  for (float *p : {c->cin, c->cout, c->zc, c->rs})
+    if (p) (void)hipFree(p);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   delete c;
 }
@@ -949,8 +1140,90 @@ int netApply(Net &n, Grid &g, int chIn, int chOut)
   return MSBGX_OK;
 }
 
+/* Buffers of the coarse form on grid g: dense block-resolution grids, the
+ * tiles of 16^3 coarse cells covering the raster, zc and rs. */
+static int coarseSetup(Grid &g, LearnedCtx &c, hipStream_t s)
+{
+  if (c.cin) return MSBGX_OK;
+  c.ntx = (g.nbx + 15) / 16;
+  c.nty = (g.nby + 15) / 16;
+  c.ntz = (g.nbz + 15) / 16;
+  c.ncoarse = g.nbx * g.nby * g.nbz;
+  const size_t nc = (size_t)g.mg[0].ncells;
+  if (int rc = dalloc(&c.cin, c.ncoarse)) return rc;
+  if (int rc = dalloc(&c.cout, c.ncoarse)) return rc;
+  if (int rc = dalloc(&c.zc, nc)) return rc;
+  if (int rc = dalloc(&c.rs, nc)) return rc;
+  HIPCHK(hipMemsetAsync(c.zc, 0, nc * sizeof(float), s));
+  HIPCHK(hipMemsetAsync(c.rs, 0, nc * sizeof(float), s));
+  if (int rc = allocBuffers(c.ib, c.ntx * c.nty * c.ntz, false, s)) return rc;
+  HIPCHK(hipStreamSynchronize(s));
+  return MSBGX_OK;
+}
+
+/* Coarse form, learned path (skipped while skipNet): zc = P N(R r) with R the
+ * per-block mean and P the constant extension over a block's unknowns; then
+ * z = zc + V(r - A zc). */
+static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p, hipStream_t s)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *skip = &d.state->skipNet;
+  const int nt = c.ntx * c.nty * c.ntz;
+  if (g.nActive0 > 0) {
+    hipLaunchKernelGGL(k_nn_fill, dim3((c.ncoarse + 255) / 256), dim3(256), 0, s, c.cin, c.ncoarse, 0.0f, skip);
+    hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, m0.d_ch[C_R], d.dActive0, m0.d_off,
+                       m0.d_flags, m0.d_D, 0, c.cin, skip);
+    hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, c.cin, g.nbx, g.nby, g.nbz, c.ntx, c.nty,
+                       (const float *)nullptr, c.ib.X0, skip);
+    netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
+    hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, g.nbx, g.nby, g.nbz, c.ntx,
+                       c.nty, c.cout, skip);
+    hipLaunchKernelGGL(k_nn_prolong_blocks, dim3(g.nActive0), dim3(256), 0, s, c.cout, d.dActive0, m0.d_off,
+                       m0.d_flags, m0.d_D, c.zc, d.state);
+  }
+  multiplyLaunch(g, 0, true, c.zc, m0.d_ch[C_R], c.rs, nullptr, skip, s);
+  std::swap(m0.d_ch[C_R], c.rs);   // the V-cycle below reads its right-hand side from rs
+  zeroActive(g, 0, m0.d_ch[C_Z], skip, s);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, skip, s);
+  std::swap(m0.d_ch[C_R], c.rs);
+  hipLaunchKernelGGL(k_nn_add, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], c.zc, c.n0, skip);
+}
+
+static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *done = &d.state->done;
+  const int *skip = &d.state->skipNet;
+  const int *skipPlain = &d.state->skipPlain;
+  const int64_t n0 = c.n0;
+  const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
+  finalize(d, nPart, FIN_ALPHA, nullptr);
+  hipLaunchKernelGGL(k_update_xr, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
+  finalize(d, kVecBlocks, FIN_CONV, d.hist);
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zprev, n0, skip, 0);
+  coarseLearnedPath(g, n, c, p, d.stream);
+  zeroActive(g, 0, m0.d_ch[C_Z], skipPlain, d.stream);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, skipPlain, d.stream);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zprev, n0, d.partials,
+                     skip);
+  finalize(d, kVecBlocks, FIN_RZOLD, nullptr);
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
+                     done);
+  finalize(d, kVecBlocks, FIN_BETA, nullptr);
+  /* a rejected corrected z restarts along r */
+  hipLaunchKernelGGL(k_nn_axpy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z],
+                     m0.d_ch[C_R], n0, d.state);
+}
+
 static void pcgIterationLearned(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p)
 {
+  if (n.mode == MSBGX_NET_COARSE) {
+    pcgIterationLearnedCoarse(g, n, c, p);
+    return;
+  }
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *done = &d.state->done;
@@ -997,12 +1270,21 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
     if (int rc = dalloc(&d.hist, p.max_iter + 1)) return rc;
     d.histCap = p.max_iter + 1;
   }
+  const bool coarse = n.mode == MSBGX_NET_COARSE;
+  if (coarse)
+    if (int rc = coarseSetup(g, c, d.stream)) return rc;
   const int chunk = std::max(1, g.graphChunk);
-  if (c.graph && (c.graphNet != &n || c.graphChunk != chunk)) { (void)hipGraphExecDestroy(c.graph); c.graph = nullptr; }
+  if (c.graph && (c.graphNet != &n || c.graphChunk != chunk || c.graphMode != n.mode)) {
+    (void)hipGraphExecDestroy(c.graph);
+    c.graph = nullptr;
+  }
   PcgState init{};
   init.tolSq = (double)p.tol * (double)p.tol;
   init.maxIter = p.max_iter;
   init.learned = 1;
+  init.coarseMode = coarse ? 1 : 0;
+  init.stallWin = n.stallWin;
+  init.stallFactor = (1.0 - (double)n.stallDrop) * (1.0 - (double)n.stallDrop);
   HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
   HIPCHK(hipStreamSynchronize(n.stream));   // weights packed
   HIPCHK(hipEventRecord(d.ev0, d.stream));
@@ -1010,25 +1292,32 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
                      d.partials);
   finalize(d, kVecBlocks, FIN_BNORM, nullptr);
   const int *done = &d.state->done;
-  zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
-  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
   const int *skip = &d.state->skipNet;
-  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zplain, c.n0, skip, 0);
-  netCorrect(g, n, c, d.stream);
-  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zplain, c.n0,
-                     d.partials, skip);
-  finalize(d, kVecBlocks, FIN_RZPLAIN, nullptr);
+  if (coarse) {
+    coarseLearnedPath(g, n, c, p, d.stream);
+    zeroActive(g, 0, m0.d_ch[C_Z], &d.state->skipPlain, d.stream);
+    vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->skipPlain, d.stream);
+  } else {
+    zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
+    vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);
+    hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zplain, c.n0, skip,
+                       0);
+    netCorrect(g, n, c, d.stream);
+    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zplain, c.n0,
+                       d.partials, skip);
+    finalize(d, kVecBlocks, FIN_RZPLAIN, nullptr);
+  }
   hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], c.n0,
                      d.partials, done);
   finalize(d, kVecBlocks, FIN_RHO0, nullptr);
-  /* d = z on every MG0 block; the PCG range takes the plain z if the
-   * corrected one was rejected */
+  /* d = z on every MG0 block; the PCG range takes the plain z (fine form) or
+   * r (coarse form) if the corrected one was rejected */
   for (int lev = 0; lev < g.nLevels; lev++)
     if (m0.dAct[lev].n)
       hipLaunchKernelGGL(k_copy_blocks, dim3(m0.dAct[lev].n), dim3(256), 0, d.stream, m0.dAct[lev].ptr, m0.d_level,
                          m0.d_off, g.bsx0, m0.d_ch[C_Z], m0.d_ch[C_D], done);
-  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, c.zplain, m0.d_ch[C_D], c.n0,
-                     &d.state->usePlain, 1);
+  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, coarse ? m0.d_ch[C_R] : c.zplain,
+                     m0.d_ch[C_D], c.n0, &d.state->usePlain, 1);
   HIPCHK(hipGetLastError());
   const bool useGraph = !(g.options & MSBGX_OPT_NO_GRAPH);
   PcgState hs{};
@@ -1044,6 +1333,7 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
         (void)hipGraphDestroy(graph);
         c.graphNet = &n;
         c.graphChunk = chunk;
+        c.graphMode = n.mode;
       }
       HIPCHK(hipGraphLaunch(c.graph, d.stream));
     } else {
@@ -1127,12 +1417,58 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   std::vector<int> pick(perPair);
   PcgState hs{};
   int pairs = 0;
+  /* coarse form: dense grids of the residual and error block means */
+  const bool coarse = n.mode == MSBGX_NET_COARSE;
+  LearnedCtx *lc = nullptr;
+  float *cerr = nullptr, *tsc = nullptr;
+  uint8_t *cmask = nullptr;
+  int nt = 0;
+  if (coarse) {
+    if (!d.learned) d.learned = new LearnedCtx();
+    lc = d.learned;
+    lc->n0 = n0;
+    if (int rc = coarseSetup(g, *lc, d.stream)) return rc;
+    nt = lc->ntx * lc->nty * lc->ntz;
+    if (int rc = dalloc(&cerr, lc->ncoarse)) return rc;
+    if (int rc = dalloc(&tsc, nt)) return rc;
+    if (int rc = dalloc(&cmask, lc->ncoarse)) return rc;
+    std::vector<uint8_t> hm(lc->ncoarse, 0);
+    for (int b : g.active0) hm[b] = 1;
+    HIPCHK(hipMemcpy(cmask, hm.data(), hm.size(), hipMemcpyHostToDevice));
+  }
   for (int k = 1; k <= p.max_iter && pairs < maxPairs; k++) {
     pcgIteration(g, p);
     HIPCHK(hipMemcpyAsync(&hs, d.state, sizeof(hs), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     if (hs.brk || hs.done) break;   // a converged iteration leaves Z unset
-    if (k % every == 0) {
+    if (k % every == 0 && coarse) {
+      /* input: block means of r_k; target: block means of e_k = x* - x_k over the unknowns */
+      hipLaunchKernelGGL(k_nn_sub, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, xs, m0.d_ch[C_X], err, n0);
+      hipLaunchKernelGGL(k_nn_fill, dim3((lc->ncoarse + 255) / 256), dim3(256), 0, d.stream, lc->cin, lc->ncoarse,
+                         0.0f, (const int *)nullptr);
+      hipLaunchKernelGGL(k_nn_fill, dim3((lc->ncoarse + 255) / 256), dim3(256), 0, d.stream, cerr, lc->ncoarse, 0.0f,
+                         (const int *)nullptr);
+      hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, m0.d_ch[C_R], d.dActive0,
+                         m0.d_off, m0.d_flags, m0.d_D, 0, lc->cin, (const int *)nullptr);
+      hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, err, d.dActive0, m0.d_off,
+                         m0.d_flags, m0.d_D, 1, cerr, (const int *)nullptr);
+      if (int rc = growSet(n, n.nS + nt)) return rc;
+      hipLaunchKernelGGL(k_nn_tile_target, dim3(nt), dim3(256), 0, d.stream, cerr, cmask, g.nbx, g.nby, g.nbz,
+                         lc->ntx, lc->nty, tsc, n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
+      hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, d.stream, lc->cin, g.nbx, g.nby, g.nbz, lc->ntx,
+                         lc->nty, tsc, n.setX + (size_t)n.nS * nn::PS, (const int *)nullptr);
+      HIPCHK(hipGetLastError());
+      std::vector<uint8_t> mk((size_t)nt * nn::NCELL);
+      HIPCHK(hipMemcpyAsync(mk.data(), n.setM + (size_t)n.nS * nn::NCELL, mk.size(), hipMemcpyDeviceToHost, d.stream));
+      HIPCHK(hipStreamSynchronize(d.stream));
+      for (int j = 0; j < nt; j++) {
+        int cnt = 0;
+        for (int cc = 0; cc < nn::NCELL; cc++) cnt += mk[(size_t)j * nn::NCELL + cc];
+        n.maskCount.push_back(cnt);
+      }
+      n.nS += nt;
+      pairs++;
+    } else if (k % every == 0) {
       /* z = V(r_k) is in Z */
       multiplyLaunch(g, 0, true, m0.d_ch[C_Z], m0.d_ch[C_R], m0.d_ch[C_Q], nullptr, d.zeroFlag, d.stream);
       hipLaunchKernelGGL(k_nn_sub, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, xs, m0.d_ch[C_X], err, n0);
@@ -1159,6 +1495,8 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   (void)hipFree(xs);
   (void)hipFree(err);
   (void)hipFree(blk);
+  for (void *q : {(void *)cerr, (void *)tsc, (void *)cmask})
+    if (q) (void)hipFree(q);
   if (int rc = deviceZero(g, C_X, 0)) return rc;
   *pairsOut = pairs;
   return MSBGX_OK;

@@ -158,6 +158,9 @@ int netTrain(Net &n, int steps, int batch, float lr, uint64_t seed, double *loss
 int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxPairs, int perPair, uint64_t seed,
                int *pairsOut);
 int netNumSamples(const Net &n);
+int netSetMode(Net &n, int mode);
+int netMode(const Net &n);
+int netSetStall(Net &n, int window, float drop);
 double netLastTrainMs(const Net &n);
 int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters, float *rel, int *fellBack);
 int netApply(Net &n, Grid &g, int chIn, int chOut);

@@ -84,11 +84,13 @@ struct PcgState {
    * the plain V-cycle output (usePlain); r.z_old for the flexible beta and
    * r.V(r) for the fallback; |r|^2 of the last kStallWindow iterations */
   int learned, restart, bad, skipNet, fellBack, usePlain;
+  int coarseMode, skipPlain;   // coarse-correction form; kernels of the plain-only path skip while set
+  int stallWin, pad3;          // stall window (0: no stall test), at most kMaxStallWindow
+  double stallFactor;          // |r|^2 must fall below stallFactor * |r|^2 of stallWin iterations ago
   double rzOld, rzPlain;
-  double rrWin[5];
+  double rrWin[16];
 };
-constexpr int kStallWindow = 5;
-constexpr double kStallFactor = 0.81;   // |r| must drop by 10% over the window
+constexpr int kMaxStallWindow = 16;
 
 template <int B> struct Cfg {
   static constexpr int S = B >= 8 ? 8 : B;     // SIMD segment width (src/msbg3.cpp:1700-1743)
@@ -741,6 +743,7 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       st->bnorm2 = s;
       st->done = (s <= 1e-30 || !isfinite(s)) ? 1 : 0;
       st->conv = -1; st->iter = 0; st->brk = !isfinite(s) ? 3 : ((s <= 1e-30) ? 2 : 0);
+      if (st->coarseMode) st->rzPlain = s;   // fallback direction r0 has r0.r0 = |b|^2
       break;
     case FIN_RHO0:
       st->rho = s;
@@ -748,6 +751,9 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       if (st->learned && (!(s > 1e-30) || !isfinite(s) || st->bad)) {
         /* the corrected z is unusable: start from the plain one */
         st->learned = 0; st->fellBack = 0; st->usePlain = 1; st->rho = st->rzPlain;
+        /* coarse form: that first direction is r itself; the V-cycle-preconditioned
+         * iteration that follows starts afresh (beta = 0) */
+        if (st->coarseMode) st->restart = 1;
       }
       break;
     case FIN_RZPLAIN: if (st->learned) st->rzPlain = s; break;
@@ -766,12 +772,13 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       if (out) out[st->iter - 1] = s;
       if (s <= st->tolSq * st->bnorm2) { st->done = 1; st->conv = st->iter; }
       else if (st->iter >= st->maxIter) st->done = 1;
-      if (st->learned) {
+      if (st->coarseMode) st->rzPlain = s;   // fallback direction r has r.r = |r|^2
+      if (st->learned && st->stallWin > 0) {
         /* stall test of the learned correction: fall back to the plain
-         * preconditioner (and restart the directions) when |r| dropped by
-         * less than 10% over the last kStallWindow iterations */
-        double &slot = st->rrWin[st->iter % kStallWindow];
-        if (st->iter > kStallWindow && !(s < kStallFactor * slot)) {
+         * preconditioner (and restart the directions) when |r|^2 did not
+         * fall below stallFactor times its value stallWin iterations ago */
+        double &slot = st->rrWin[st->iter % st->stallWin];
+        if (st->iter > st->stallWin && !(s < st->stallFactor * slot)) {
           st->learned = 0; st->restart = 1; st->fellBack = st->iter;
         }
         slot = s;
@@ -785,7 +792,7 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
         const double be = (s - st->rzOld) / st->rho;
         if (st->bad || !(s > 1e-30) || !isfinite(s) || !isfinite(be)) {
           /* unusable corrected z: restart from the plain V-cycle output */
-          st->learned = 0; st->fellBack = st->iter; st->usePlain = 1; st->restart = 0;
+          st->learned = 0; st->fellBack = st->iter; st->usePlain = 1; st->restart = st->coarseMode ? 1 : 0;
           st->beta = 0.0f;
           st->rho = st->rzPlain;
           if (!(st->rho > 1e-30) || !isfinite(st->rho)) { st->done = 1; st->brk = 1; }
@@ -807,6 +814,7 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
     case FIN_FINAL: if (out) out[0] = s; break;
   }
   st->skipNet = (st->done || !st->learned) ? 1 : 0;
+  st->skipPlain = (st->done || st->learned) ? 1 : 0;
 }
 
 /* Divergence right-hand side of level-0 blocks from the scene's MAC velocity

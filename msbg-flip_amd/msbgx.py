@@ -194,6 +194,9 @@ def _load():
         "msbgx_net_collect": (C.c_int, [P, P, C.POINTER(SolveParams), C.c_int, C.c_int, C.c_int, C.c_uint64,
                                         C.POINTER(C.c_int)]),
         "msbgx_net_num_samples": (C.c_int, [P]),
+        "msbgx_net_set_mode": (C.c_int, [P, C.c_int]),
+        "msbgx_net_get_mode": (C.c_int, [P]),
+        "msbgx_net_set_stall": (C.c_int, [P, C.c_int, C.c_float]),
         "msbgx_net_train": (C.c_int, [P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
         "msbgx_net_last_train_ms": (C.c_double, [P]),
@@ -224,10 +227,12 @@ EXPORTED = [
     "msbgx_set_launch_config", "msbgx_net_create", "msbgx_net_destroy", "msbgx_net_get_weights",
     "msbgx_net_set_weights", "msbgx_net_set_samples", "msbgx_net_loss_grad", "msbgx_net_collect",
     "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
-    "msbgx_solve_pressure_learned", "msbgx_net_time_conv",
+    "msbgx_solve_pressure_learned", "msbgx_net_time_conv", "msbgx_net_set_mode", "msbgx_net_get_mode",
+    "msbgx_net_set_stall",
 ]
 
 NET_PARAMS = 14688
+NET_FINE, NET_COARSE = 0, 1
 
 
 class MsbgxError(RuntimeError):
@@ -412,9 +417,19 @@ class MultiresSparseGrid:
 class LearnedNet:
     """Handle over ``msbgx_net``: the learned preconditioner correction."""
 
-    def __init__(self, seed=1):
+    def __init__(self, seed=1, mode=0):
         self._h = C.c_void_p()
         _ck(lib.msbgx_net_create(C.byref(self._h), seed), "net_create")
+        if mode:
+            self.mode = mode
+
+    @property
+    def mode(self):
+        return lib.msbgx_net_get_mode(self._h)
+
+    @mode.setter
+    def mode(self, m):
+        _ck(lib.msbgx_net_set_mode(self._h, m), "net_set_mode")
 
     def destroy(self):
         if self._h:
@@ -460,6 +475,9 @@ class LearnedNet:
         _ck(lib.msbgx_net_collect(self._h, grid._h, C.byref(p), every, max_pairs, blocks_per_pair, seed,
                                   C.byref(pairs)), "net_collect")
         return pairs.value
+
+    def set_stall(self, window=5, drop=0.1):
+        _ck(lib.msbgx_net_set_stall(self._h, window, drop), "net_set_stall")
 
     @property
     def num_samples(self):

@@ -163,3 +163,34 @@ def test_training_on_pcg_pairs_lowers_the_loss():
     it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
     print(f"  trained net on its grid: {it} iterations rel {rr:.2e} fallback {fb}", flush=True)
     assert rr <= 1e-6
+
+
+def test_coarse_form_zero_and_nan_networks():
+    """Coarse correction with a zero network is the plain preconditioner;
+    a NaN network is dropped and the solve falls back."""
+    cfg = msbgx.config_c4(n=64)
+    it0 = plain_iters(cfg)
+    for fill, expect_fb in ((0.0, False), (np.nan, True)):
+        g = prepared(cfg)
+        net = msbgx.LearnedNet(seed=1, mode=msbgx.NET_COARSE)
+        net.weights = np.full(msbgx.NET_PARAMS, fill, np.float32)
+        it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
+        print(f"  coarse form, weights {fill}: {it} iterations (plain {it0}), rel {rr:.2e}, fallback {fb}", flush=True)
+        assert rr <= 1e-6 and it <= it0 + 3
+        assert (fb >= 0) == expect_fb
+
+
+def test_coarse_form_training_and_solve():
+    print("learned coarse form: collect + train", flush=True)
+    cfg = msbgx.config_c4(n=128)
+    g = prepared(cfg)
+    net = msbgx.LearnedNet(seed=5, mode=msbgx.NET_COARSE)
+    pairs = net.collect(g, every=1, max_pairs=80, seed=3, tol=1e-7)
+    assert pairs > 30 and net.num_samples == pairs        # 8^3 blocks: one tile per iterate
+    first, last = net.train(steps=300, batch=32, lr=3e-3, seed=9)
+    print(f"  {pairs} pairs: loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < first
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
+    print(f"  coarse-corrected solve: {it} iterations rel {rr:.2e} fallback {fb}", flush=True)
+    assert rr <= 1e-6
