@@ -182,9 +182,12 @@ typedef struct msbgx_net msbgx_net;
  *   MSBGX_NET_COARSE  a coarse correction: the network runs on the grid of
  *                     block means of r (one cell per block, tiled in 16^3
  *                     tiles with a one-cell halo), its output is extended
- *                     constant over each level-0 PCG block's unknowns (zc),
+ *                     constant over each level-0 PCG block's unknowns and
+ *                     scaled by the Galerkin step <zc, r> / <zc, A zc> (zc),
  *                     and z = zc + V(r - A zc).  Trained on block means of
- *                     r_k (input) and of x* - x_k over the unknowns (target). */
+ *                     r_k (input) and of x* - x_k over the unknowns (target).
+ * Setting the form also sets its default stall test (fine: 5 iterations,
+ * 10 %; coarse: 16 iterations, 2 %). */
 #define MSBGX_NET_FINE 0
 #define MSBGX_NET_COARSE 1
 /* He-normal initial weights from `seed`. */
@@ -210,7 +213,7 @@ int msbgx_net_set_mode(msbgx_net *n, int mode);
 int msbgx_net_get_mode(const msbgx_net *n);
 /* Stall test of msbgx_solve_pressure_learned: fall back when |r| drops by
  * less than `drop` (relative) over `window` iterations (0 <= window <= 16;
- * 0 disables the test; default 5 and 0.1). */
+ * 0 disables the test; defaults per form, see above). */
 int msbgx_net_set_stall(msbgx_net *n, int window, float drop);
 int msbgx_net_num_samples(const msbgx_net *n);
 /* Adam on random minibatches of `batch` samples; first-step and smoothed

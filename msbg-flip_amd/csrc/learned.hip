@@ -42,6 +42,11 @@
  *                   times the ReLU mask of the layer input)
  *   k_nn_out        16 -> 1 on the block cells, VALU from LDS; inference adds
  *                   the masked, finite correction to z
+ *   coarse form (MSBGX_NET_COARSE): k_nn_block_mean, k_nn_gather_dense,
+ *                   k_nn_out_dense, k_nn_prolong_blocks, k_nn_gamma -- the same
+ *                   network on the grid of block means, its output extended
+ *                   over each block and scaled by the Galerkin step
+ *                   <zc, r> / <zc, A zc> before z = zc + V(r - A zc)
  *   training: k_nn_loss (masked MSE gradient), k_nn_bwd4, k_nn_dw16,
  *   k_nn_dw_small, k_nn_reduce (fixed-order FP64 sum of per-workgroup
  *   partials), k_nn_adam, k_nn_pack
@@ -660,6 +665,33 @@ __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, in
     z[i] += zc[i];
 }
 
+/* Scale the correction by the Galerkin step gamma over the whole MG0 pool:
+ * zc *= gamma and rs = r - gamma A zc = (1 - gamma) r + gamma rs. */
+__global__ void k_nn_gamma(float *__restrict__ zc, float *__restrict__ rs, const float *__restrict__ r, int64_t n,
+                           const PcgState *st)
+{
+  if (st->skipNet) return;
+  const float gm = st->gamma;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    zc[i] = gm * zc[i];
+    rs[i] = fmaf(gm, rs[i] - r[i], r[i]);
+  }
+}
+
+/* partial <a, b - c> over [0, n) (FP64), unless *skip */
+__global__ void __launch_bounds__(kVecThreads) k_nn_dot_diff(const float *__restrict__ a, const float *__restrict__ b,
+                                                             const float *__restrict__ c, int64_t n,
+                                                             double *partials, const int *skip)
+{
+  if (*skip) return;
+  __shared__ double red[kVecThreads / 64];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i] * ((double)b[i] - (double)c[i]);
+  const double t = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 /* Coarse-form training sample of tile t: input patch of the dense residual
  * means, target = dense error means on PCG blocks (mask 1 there), both
  * divided by the target's RMS over the tile. */
@@ -1032,6 +1064,11 @@ int netSetMode(Net &n, int mode)
     n.mode = mode;
     n.nS = 0;               // samples of the other form are not reused
     n.maskCount.clear();
+    /* default stall tests: the fine correction is not step-length
+     * controlled and is dropped early; the coarse one is scaled by its
+     * Galerkin step and only a real stall switches it off */
+    n.stallWin = mode == MSBGX_NET_COARSE ? 16 : 5;
+    n.stallDrop = mode == MSBGX_NET_COARSE ? 0.02f : 0.1f;
   }
   return MSBGX_OK;
 }
@@ -1183,6 +1220,14 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
                        m0.d_flags, m0.d_D, c.zc, d.state);
   }
   multiplyLaunch(g, 0, true, c.zc, m0.d_ch[C_R], c.rs, nullptr, skip, s);
+  /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
+  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.n0, d.partials, skip);
+  finalize(d, kVecBlocks, FIN_ZCR, nullptr);
+  hipLaunchKernelGGL(k_nn_dot_diff, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.rs, c.n0,
+                     d.partials, skip);
+  finalize(d, kVecBlocks, FIN_GAMMA, nullptr);
+  hipLaunchKernelGGL(k_nn_gamma, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, c.rs, m0.d_ch[C_R],
+                     (int64_t)g.mg[0].ncells, d.state);
   std::swap(m0.d_ch[C_R], c.rs);   // the V-cycle below reads its right-hand side from rs
   zeroActive(g, 0, m0.d_ch[C_Z], skip, s);
   vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, skip, s);

@@ -87,6 +87,9 @@ struct PcgState {
   int coarseMode, skipPlain;   // coarse-correction form; kernels of the plain-only path skip while set
   int stallWin, pad3;          // stall window (0: no stall test), at most kMaxStallWindow
   double stallFactor;          // |r|^2 must fall below stallFactor * |r|^2 of stallWin iterations ago
+  double rrBest;               // smallest |r|^2 before the stall window
+  double zcr;                  // coarse form: <zc, r>
+  float gamma, pad4;           // coarse form: Galerkin step <zc, r> / <zc, A zc> of the correction
   double rzOld, rzPlain;
   double rrWin[16];
 };
@@ -710,14 +713,15 @@ __device__ double sumPartials(const double *p, int n)
   return out;
 }
 
-enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL, FIN_RZOLD, FIN_RZPLAIN };
+enum FinalizeOp { FIN_BNORM = 0, FIN_RHO0, FIN_ALPHA, FIN_CONV, FIN_BETA, FIN_FINAL, FIN_RZOLD, FIN_RZPLAIN,
+                  FIN_ZCR, FIN_GAMMA };
 
 __device__ void applyFinalize(PcgState *st, double s, int op, double *out);
 
 __global__ void __launch_bounds__(256) k_finalize(PcgState *st, const double *partials, int n, int op, double *out)
 {
-  const bool skipIfDone =
-      (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA || op == FIN_RZOLD || op == FIN_RZPLAIN);
+  const bool skipIfDone = (op == FIN_ALPHA || op == FIN_CONV || op == FIN_BETA || op == FIN_RZOLD ||
+                          op == FIN_RZPLAIN || op == FIN_ZCR || op == FIN_GAMMA);
   if (skipIfDone && st->done) return;
   const double s = sumPartials(partials, n);
   if (threadIdx.x != 0) return;
@@ -757,6 +761,13 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       }
       break;
     case FIN_RZPLAIN: if (st->learned) st->rzPlain = s; break;
+    case FIN_ZCR: st->zcr = s; break;
+    case FIN_GAMMA: {
+      /* s = <zc, A zc>: the step along zc that minimises the A-norm error */
+      const double gm = st->zcr / s;
+      st->gamma = (s > 1e-300 && isfinite(gm)) ? (float)gm : 0.0f;
+      break;
+    }
     case FIN_ALPHA: {
       st->dq = s;
       const double rho = st->rho;
@@ -774,14 +785,20 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       else if (st->iter >= st->maxIter) st->done = 1;
       if (st->coarseMode) st->rzPlain = s;   // fallback direction r has r.r = |r|^2
       if (st->learned && st->stallWin > 0) {
-        /* stall test of the learned correction: fall back to the plain
-         * preconditioner (and restart the directions) when |r|^2 did not
-         * fall below stallFactor times its value stallWin iterations ago */
-        double &slot = st->rrWin[st->iter % st->stallWin];
-        if (st->iter > st->stallWin && !(s < st->stallFactor * slot)) {
-          st->learned = 0; st->restart = 1; st->fellBack = st->iter;
-        }
+        /* stall test of the learned correction (residuals of a flexible CG
+         * need not decrease monotonically): fall back to the plain
+         * preconditioner and restart the directions when no |r|^2 of the
+         * last stallWin iterations fell below stallFactor times the
+         * smallest one before them */
+        const int W = st->stallWin;
+        double &slot = st->rrWin[st->iter % W];
+        if (st->iter > W) st->rrBest = st->iter == W + 1 ? slot : fmin(st->rrBest, slot);
         slot = s;
+        if (st->iter > W) {
+          double m = st->rrWin[0];
+          for (int k = 1; k < W; k++) m = fmin(m, st->rrWin[k]);
+          if (!(m < st->stallFactor * st->rrBest)) { st->learned = 0; st->restart = 1; st->fellBack = st->iter; }
+        }
       }
       break;
     case FIN_RZOLD: if (st->learned) st->rzOld = s; break;
