@@ -230,9 +230,18 @@ def learned_runs(msbgx, args, log, mode="fine"):
     t_collect = time.time() - t0
     log(f"learned ({mode}): {pairs} pairs from {seed} solves, {net.num_samples} samples, {t_collect:.1f} s")
     net_samples = net.num_samples
-    first, last = net.train(steps=args.learned_steps, batch=64 if mode == "fine" else 16, lr=3e-3, seed=7)
-    train_ms = net.last_train_ms()
-    log(f"learned ({mode}): {args.learned_steps} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
+    if mode == "fine":
+        sched = [(args.learned_steps, 3e-3, 64)]
+    else:   # random tile origins make the coarse samples harder: a longer, decaying schedule
+        sched = [(2000, 3e-3, 16), (2000, 1e-3, 16), (4000, 1e-3, 32), (4000, 3e-4, 32)]
+    train_ms, first, last = 0.0, None, None
+    for k, (steps, lr, bs) in enumerate(sched):
+        a, b = net.train(steps=steps, batch=bs, lr=lr, seed=7 + k)
+        first = a if first is None else first
+        last = b
+        train_ms += net.last_train_ms()
+    steps_total = sum(st for st, _, _ in sched)
+    log(f"learned ({mode}): {steps_total} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
     evals = []
     for s in (501, 502):
         c = msbgx.config_c4(args.learned_eval_n, seed=s)
@@ -262,8 +271,9 @@ def learned_runs(msbgx, args, log, mode="fine"):
     net.destroy()
     return {"form": mode,
             "train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
-                      "samples": net_samples, "collect_s": round(t_collect, 1), "steps": args.learned_steps,
-                      "batch": 64, "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
+                      "samples": net_samples, "collect_s": round(t_collect, 1), "steps": steps_total,
+                      "schedule": [{"steps": st, "lr": lr, "batch": bs} for st, lr, bs in sched],
+                      "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
             "eval": evals, "conv16_roofline": conv, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
 
 

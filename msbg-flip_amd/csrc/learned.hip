@@ -572,11 +572,14 @@ __global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__
 }
 
 /* Patch of tile t of the dense block-resolution grid: tile (tx, ty, tz)
- * covers coarse cells 16 t .. 16 t + 15 per axis; cells outside the raster
- * read 0.  Optionally divides by scale[t]. */
+ * covers coarse cells 16 t - o .. 16 t - o + 15 per axis (o: tile origin
+ * offset, 0 at inference, random in training so that the domain boundary
+ * is not always at a tile edge); cells outside the raster read 0.
+ * Optionally divides by scale[t]. */
 __global__ void __launch_bounds__(256) k_nn_gather_dense(const float *__restrict__ coarse, int nbx, int nby, int nbz,
-                                                         int ntx, int nty, const float *__restrict__ scale,
-                                                         uint16_t *__restrict__ X0, const int *skip)
+                                                         int ntx, int nty, int ox, int oy, int oz,
+                                                         const float *__restrict__ scale, uint16_t *__restrict__ X0,
+                                                         const int *skip)
 {
   if (skip && *skip) return;
   const int t = blockIdx.x;
@@ -587,7 +590,7 @@ __global__ void __launch_bounds__(256) k_nn_gather_dense(const float *__restrict
     const int u = pos % nn::E, v = (pos / nn::E) % nn::E, w = pos / nn::PL;
     float val = 0.0f;
     if (u >= 1 && u <= 18 && v >= 1 && v <= 18 && w >= 1 && w <= 18) {
-      const int cx = 16 * tx + u - 2, cy = 16 * ty + v - 2, cz = 16 * tz + w - 2;
+      const int cx = 16 * tx + u - 2 - ox, cy = 16 * ty + v - 2 - oy, cz = 16 * tz + w - 2 - oz;
       if (cx >= 0 && cy >= 0 && cz >= 0 && cx < nbx && cy < nby && cz < nbz)
         val = coarse[cx + nbx * (cy + nby * cz)] * sc;
     }
@@ -697,17 +700,18 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_dot_diff(const float *__rest
  * divided by the target's RMS over the tile. */
 __global__ void __launch_bounds__(256) k_nn_tile_target(const float *__restrict__ cerr,
                                                         const uint8_t *__restrict__ cmask, int nbx, int nby, int nbz,
-                                                        int ntx, int nty, float *__restrict__ scale,
-                                                        float *__restrict__ tgt, uint8_t *__restrict__ tmask)
+                                                        int ntx, int nty, int ox, int oy, int oz,
+                                                        float *__restrict__ scale, float *__restrict__ tgt,
+                                                        uint8_t *__restrict__ tmask)
 {
   __shared__ double red[4];
   __shared__ float sc;
   const int t = blockIdx.x;
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   auto cellOf = [&](int c, int &idx) {
-    const int cx = 16 * tx + (c & 15), cy = 16 * ty + ((c >> 4) & 15), cz = 16 * tz + (c >> 8);
+    const int cx = 16 * tx + (c & 15) - ox, cy = 16 * ty + ((c >> 4) & 15) - oy, cz = 16 * tz + (c >> 8) - oz;
     idx = cx + nbx * (cy + nby * cz);
-    return cx < nbx && cy < nby && cz < nbz && cmask[idx];
+    return cx >= 0 && cy >= 0 && cz >= 0 && cx < nbx && cy < nby && cz < nbz && cmask[idx];
   };
   double acc = 0.0, cnt = 0.0;
   for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
@@ -1211,8 +1215,8 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
     hipLaunchKernelGGL(k_nn_fill, dim3((c.ncoarse + 255) / 256), dim3(256), 0, s, c.cin, c.ncoarse, 0.0f, skip);
     hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, m0.d_ch[C_R], d.dActive0, m0.d_off,
                        m0.d_flags, m0.d_D, 0, c.cin, skip);
-    hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, c.cin, g.nbx, g.nby, g.nbz, c.ntx, c.nty,
-                       (const float *)nullptr, c.ib.X0, skip);
+    hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, c.cin, g.nbx, g.nby, g.nbz, c.ntx, c.nty, 0, 0,
+                       0, (const float *)nullptr, c.ib.X0, skip);
     netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
     hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, g.nbx, g.nby, g.nbz, c.ntx,
                        c.nty, c.cout, skip);
@@ -1467,7 +1471,15 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   LearnedCtx *lc = nullptr;
   float *cerr = nullptr, *tsc = nullptr;
   uint8_t *cmask = nullptr;
-  int nt = 0;
+  int nt = 0, tscCap = 0;
+  auto ensureCap = [](float **p, int &cap, int need) -> int {
+    if (need <= cap) return MSBGX_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    if (int rc = dalloc(p, need)) return rc;
+    cap = need;
+    return MSBGX_OK;
+  };
   if (coarse) {
     if (!d.learned) d.learned = new LearnedCtx();
     lc = d.learned;
@@ -1475,7 +1487,7 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
     if (int rc = coarseSetup(g, *lc, d.stream)) return rc;
     nt = lc->ntx * lc->nty * lc->ntz;
     if (int rc = dalloc(&cerr, lc->ncoarse)) return rc;
-    if (int rc = dalloc(&tsc, nt)) return rc;
+    if (int rc = ensureCap(&tsc, tscCap, 8 * nt)) return rc;
     if (int rc = dalloc(&cmask, lc->ncoarse)) return rc;
     std::vector<uint8_t> hm(lc->ncoarse, 0);
     for (int b : g.active0) hm[b] = 1;
@@ -1497,21 +1509,26 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
                          m0.d_off, m0.d_flags, m0.d_D, 0, lc->cin, (const int *)nullptr);
       hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, err, d.dActive0, m0.d_off,
                          m0.d_flags, m0.d_D, 1, cerr, (const int *)nullptr);
-      if (int rc = growSet(n, n.nS + nt)) return rc;
-      hipLaunchKernelGGL(k_nn_tile_target, dim3(nt), dim3(256), 0, d.stream, cerr, cmask, g.nbx, g.nby, g.nbz,
-                         lc->ntx, lc->nty, tsc, n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
-      hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, d.stream, lc->cin, g.nbx, g.nby, g.nbz, lc->ntx,
-                         lc->nty, tsc, n.setX + (size_t)n.nS * nn::PS, (const int *)nullptr);
+      /* random tile origin: the domain boundary falls anywhere inside a tile */
+      const int ox = (int)(next() % 16u), oy = (int)(next() % 16u), oz = (int)(next() % 16u);
+      const int tx = (g.nbx + ox + 15) / 16, ty = (g.nby + oy + 15) / 16, tz = (g.nbz + oz + 15) / 16;
+      const int ntp = tx * ty * tz;
+      if (int rc = growSet(n, n.nS + ntp)) return rc;
+      if (int rc = ensureCap(&tsc, tscCap, ntp)) return rc;
+      hipLaunchKernelGGL(k_nn_tile_target, dim3(ntp), dim3(256), 0, d.stream, cerr, cmask, g.nbx, g.nby, g.nbz, tx,
+                         ty, ox, oy, oz, tsc, n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
+      hipLaunchKernelGGL(k_nn_gather_dense, dim3(ntp), dim3(256), 0, d.stream, lc->cin, g.nbx, g.nby, g.nbz, tx, ty,
+                         ox, oy, oz, tsc, n.setX + (size_t)n.nS * nn::PS, (const int *)nullptr);
       HIPCHK(hipGetLastError());
-      std::vector<uint8_t> mk((size_t)nt * nn::NCELL);
+      std::vector<uint8_t> mk((size_t)ntp * nn::NCELL);
       HIPCHK(hipMemcpyAsync(mk.data(), n.setM + (size_t)n.nS * nn::NCELL, mk.size(), hipMemcpyDeviceToHost, d.stream));
       HIPCHK(hipStreamSynchronize(d.stream));
-      for (int j = 0; j < nt; j++) {
+      for (int j = 0; j < ntp; j++) {
         int cnt = 0;
         for (int cc = 0; cc < nn::NCELL; cc++) cnt += mk[(size_t)j * nn::NCELL + cc];
         n.maskCount.push_back(cnt);
       }
-      n.nS += nt;
+      n.nS += ntp;
       pairs++;
     } else if (k % every == 0) {
       /* z = V(r_k) is in Z */

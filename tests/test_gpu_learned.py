@@ -186,10 +186,12 @@ def test_coarse_form_training_and_solve():
     g = prepared(cfg)
     net = msbgx.LearnedNet(seed=5, mode=msbgx.NET_COARSE)
     pairs = net.collect(g, every=1, max_pairs=80, seed=3, tol=1e-7)
-    assert pairs > 30 and net.num_samples == pairs        # 8^3 blocks: one tile per iterate
-    first, last = net.train(steps=300, batch=32, lr=3e-3, seed=9)
-    print(f"  {pairs} pairs: loss {first:.4f} -> {last:.4f}", flush=True)
-    assert last < first
+    assert pairs > 30 and net.num_samples >= pairs        # 8^3 blocks: 1-8 tiles per iterate (random origin)
+    before, _, _ = net.loss_grad()
+    net.train(steps=1500, batch=32, lr=1e-3, seed=9)
+    after, _, _ = net.loss_grad()
+    print(f"  {pairs} pairs, {net.num_samples} tiles: set loss {before:.4f} -> {after:.4f}", flush=True)
+    assert after < before
     g.zero_channel(msbgx.CH_PRESSURE, 0)
     it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
     print(f"  coarse-corrected solve: {it} iterations rel {rr:.2e} fallback {fb}", flush=True)
