@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--learned-pairs", type=int, default=2000)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--learned-eval-n", type=int, default=512)
+    ap.add_argument("--learned-eval-config", default="c4", choices=["c4", "c5", "c5b16"],
+                    help="evaluation workload of the learned forms (c4: dam break at --learned-eval-n, seeds 501/502; "
+                         "c5/c5b16: the 1024^3 particle workload on 32^3 / 16^3 blocks)")
+    ap.add_argument("--learned-forms", default="fine,coarse")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch PCG iterations eagerly (no hipGraph) in every section (profiling runs)")
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -243,20 +247,32 @@ def learned_runs(msbgx, args, log, mode="fine"):
     steps_total = sum(st for st, _, _ in sched)
     log(f"learned ({mode}): {steps_total} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
     evals = []
-    for s in (501, 502):
-        c = msbgx.config_c4(args.learned_eval_n, seed=s)
+    if mode == "fine" and args.learned_eval_config == "c5":
+        log("learned (fine): the 32^3-block workload needs the coarse form; skipped")
+        eval_cfgs = []
+    elif args.learned_eval_config == "c4":
+        eval_cfgs = [msbgx.config_c4(args.learned_eval_n, seed=s) for s in (501, 502)]
+    else:
+        eval_cfgs = [make_cfg(msbgx, args.learned_eval_config, 0)]
+    for c in eval_cfgs:
+        s = c.scene.seed
         g = msbgx.build_grid(c)
         g.prepare_pressure_solve_flip(scene=c.scene)
-        g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)     # warm-up
-        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        warm = args.learned_eval_config == "c4"   # full-scale solves run long enough without a warm-up
+        if warm:
+            g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+            g.zero_channel(msbgx.CH_PRESSURE, 0)
+        log(f"learned ({mode}): plain solve of {c.name} {c.n}^3")
         it_p, rr_p = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
         ms_p = g.last_solve_ms()
         g.zero_channel(msbgx.CH_PRESSURE, 0)
-        net.solve(g, max_iter=args.max_iter, tol=args.tol)                # warm-up
-        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        if warm:
+            net.solve(g, max_iter=args.max_iter, tol=args.tol)
+            g.zero_channel(msbgx.CH_PRESSURE, 0)
+        log(f"learned ({mode}): learned solve of {c.name} {c.n}^3")
         it_l, rr_l, fb = net.solve(g, max_iter=args.max_iter, tol=args.tol)
         ms_l = g.last_solve_ms()
-        rec = {"seed": s, "n": c.n, "plain_ms": round(ms_p, 2), "plain_iterations": it_p, "plain_rel": rr_p,
+        rec = {"workload": c.name, "bsx": c.bsx, "seed": s, "n": c.n, "plain_ms": round(ms_p, 2), "plain_iterations": it_p, "plain_rel": rr_p,
                "learned_ms": round(ms_l, 2), "learned_iterations": it_l, "learned_rel": rr_l,
                "fallback_iteration": fb, "speedup": round(ms_p / ms_l, 3)}
         log(f"learned ({mode}): {rec}")
@@ -412,8 +428,11 @@ def main():
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:
-        line["learned_form"] = learned_runs(msbgx, args, log, "fine")
-        line["learned_coarse_form"] = learned_runs(msbgx, args, log, "coarse")
+        forms = args.learned_forms.split(",")
+        if "fine" in forms:
+            line["learned_form"] = learned_runs(msbgx, args, log, "fine")
+        if "coarse" in forms:
+            line["learned_coarse_form"] = learned_runs(msbgx, args, log, "coarse")
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)

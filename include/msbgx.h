@@ -173,7 +173,8 @@ int msbgx_set_launch_config(msbgx_grid *g, int graph_chunk, int smoother_mask);
  *   z = V(r) + N(r - A V(r)).
  * Parameters, FP32, in this order: W1[27][16] (tap, out), W2[27][16][16],
  * W3[27][16][16] (tap, in, out), W4[27][16] (tap, in); tap = 9 (dz+1) +
- * 3 (dy+1) + (dx+1).  Grids with 16^3 base blocks only. */
+ * 3 (dy+1) + (dx+1).  The fine form needs 16^3 base blocks; the coarse form
+ * works with any block size. */
 #define MSBGX_NET_PARAMS 14688
 typedef struct msbgx_net msbgx_net;
 /* Where the network acts (msbgx_net_set_mode; the training set is cleared

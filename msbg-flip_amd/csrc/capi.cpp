@@ -519,11 +519,15 @@ int msbgx_net_loss_grad(msbgx_net *n, double *loss, float *grad, float *out)
   return netLossGradAll(*netOf(n), loss, grad, out);
 }
 
-static int checkLearnedGrid(msbgx_grid *g)
+static int checkLearnedGrid(msbgx_grid *g, const msbgx_net *n)
 {
   if (int rc = checkSingle(g)) return rc;
   if (!g->prepared) { setError("not prepared"); return MSBGX_ESTATE; }
-  if (g->bsx0 != 16) { setError("the learned correction needs 16^3 base blocks"); return MSBGX_EINVAL; }
+  /* the fine form works on 16^3 blocks; the coarse form on block means of any size */
+  if (g->bsx0 != 16 && msbgx_net_get_mode(n) != MSBGX_NET_COARSE) {
+    setError("the fine learned correction needs 16^3 base blocks");
+    return MSBGX_EINVAL;
+  }
   return MSBGX_OK;
 }
 
@@ -531,7 +535,7 @@ int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, 
                       int blocks_per_pair, uint64_t seed, int *pairs)
 {
   if (!n || !pairs) { setError("null argument"); return MSBGX_EINVAL; }
-  if (int rc = checkLearnedGrid(g)) return rc;
+  if (int rc = checkLearnedGrid(g, n)) return rc;
   if (int rc = checkParams(p)) return rc;
   if (every < 1 || max_pairs < 0 || blocks_per_pair < 1) { setError("bad argument"); return MSBGX_EINVAL; }
   return netCollect(*netOf(n), *g, *p, every, max_pairs, blocks_per_pair, seed, pairs);
@@ -565,7 +569,8 @@ double msbgx_net_last_train_ms(const msbgx_net *n) { return n ? netLastTrainMs(*
 int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out)
 {
   if (!n) { setError("null argument"); return MSBGX_EINVAL; }
-  if (int rc = checkLearnedGrid(g)) return rc;
+  if (int rc = checkLearnedGrid(g, n)) return rc;
+  if (g->bsx0 != 16) { setError("msbgx_net_apply works on 16^3 blocks"); return MSBGX_EINVAL; }
   if (ch_in < 0 || ch_in > MSBGX_CH_Q || ch_out < 0 || ch_out > MSBGX_CH_Q || ch_in == ch_out) {
     setError("bad channel");
     return MSBGX_EINVAL;
@@ -583,7 +588,7 @@ int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_
                                  int *fallback_iter)
 {
   if (!n || !iters || !rel || !fallback_iter) { setError("null argument"); return MSBGX_EINVAL; }
-  if (int rc = checkLearnedGrid(g)) return rc;
+  if (int rc = checkLearnedGrid(g, n)) return rc;
   if (int rc = checkParams(p)) return rc;
   return deviceSolveLearned(*g, *netOf(n), *p, iters, rel, fallback_iter);
 }

@@ -552,7 +552,7 @@ __global__ void k_nn_fill(float *__restrict__ dst, int n, float v, const int *sk
 __global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__ src, const int *__restrict__ blocks,
                                                        const int64_t *__restrict__ off,
                                                        const uint16_t *__restrict__ flags,
-                                                       const float *__restrict__ D, int unknownOnly,
+                                                       const float *__restrict__ D, int ncell, int unknownOnly,
                                                        float *__restrict__ coarse, const int *skip)
 {
   if (skip && *skip) return;
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__
   const int bid = blocks[blockIdx.x];
   const int64_t o = off[bid];
   double acc = 0.0, cnt = 0.0;
-  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
     const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
     if (unknownOnly && !unk) continue;
     acc += (double)src[o + c];
@@ -643,8 +643,8 @@ __global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restri
                                                            const int *__restrict__ blocks,
                                                            const int64_t *__restrict__ off,
                                                            const uint16_t *__restrict__ flags,
-                                                           const float *__restrict__ D, float *__restrict__ zc,
-                                                           PcgState *st)
+                                                           const float *__restrict__ D, int ncell,
+                                                           float *__restrict__ zc, PcgState *st)
 {
   if (st->skipNet) return;
   const int bid = blocks[blockIdx.x];
@@ -654,7 +654,7 @@ __global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restri
     if (threadIdx.x == 0) st->bad = 1;
     v = 0.0f;
   }
-  for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
     const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
     zc[o + c] = unk ? v : 0.0f;
   }
@@ -1211,17 +1211,18 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   MgLevel &m0 = g.mg[0];
   const int *skip = &d.state->skipNet;
   const int nt = c.ntx * c.nty * c.ntz;
+  const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
   if (g.nActive0 > 0) {
     hipLaunchKernelGGL(k_nn_fill, dim3((c.ncoarse + 255) / 256), dim3(256), 0, s, c.cin, c.ncoarse, 0.0f, skip);
     hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, m0.d_ch[C_R], d.dActive0, m0.d_off,
-                       m0.d_flags, m0.d_D, 0, c.cin, skip);
+                       m0.d_flags, m0.d_D, ncell, 0, c.cin, skip);
     hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, c.cin, g.nbx, g.nby, g.nbz, c.ntx, c.nty, 0, 0,
                        0, (const float *)nullptr, c.ib.X0, skip);
     netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
     hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, g.nbx, g.nby, g.nbz, c.ntx,
                        c.nty, c.cout, skip);
     hipLaunchKernelGGL(k_nn_prolong_blocks, dim3(g.nActive0), dim3(256), 0, s, c.cout, d.dActive0, m0.d_off,
-                       m0.d_flags, m0.d_D, c.zc, d.state);
+                       m0.d_flags, m0.d_D, ncell, c.zc, d.state);
   }
   multiplyLaunch(g, 0, true, c.zc, m0.d_ch[C_R], c.rs, nullptr, skip, s);
   /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
@@ -1505,10 +1506,11 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
                          0.0f, (const int *)nullptr);
       hipLaunchKernelGGL(k_nn_fill, dim3((lc->ncoarse + 255) / 256), dim3(256), 0, d.stream, cerr, lc->ncoarse, 0.0f,
                          (const int *)nullptr);
+      const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
       hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, m0.d_ch[C_R], d.dActive0,
-                         m0.d_off, m0.d_flags, m0.d_D, 0, lc->cin, (const int *)nullptr);
+                         m0.d_off, m0.d_flags, m0.d_D, ncell, 0, lc->cin, (const int *)nullptr);
       hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, err, d.dActive0, m0.d_off,
-                         m0.d_flags, m0.d_D, 1, cerr, (const int *)nullptr);
+                         m0.d_flags, m0.d_D, ncell, 1, cerr, (const int *)nullptr);
       /* random tile origin: the domain boundary falls anywhere inside a tile */
       const int ox = (int)(next() % 16u), oy = (int)(next() % 16u), oz = (int)(next() % 16u);
       const int tx = (g.nbx + ox + 15) / 16, ty = (g.nby + oy + 15) / 16, tz = (g.nbz + oz + 15) / 16;
