@@ -64,6 +64,16 @@ __device__ __forceinline__ float fromBits16(uint16_t v, int prec)
   if (prec == MSBGX_PREC_FP16) return (float)__builtin_bit_cast(_Float16, v);
   return __uint_as_float((uint32_t)v << 16);
 }
+/* smoother coefficient k with the operand precision P fixed at compile time */
+template <int P> __device__ __forceinline__ float smCoefP(const LevelArgs &a, int k, int64_t q)
+{
+  if constexpr (P == MSBGX_PREC_FP32) {
+    const float *p = k == 0 ? a.W0 : (k == 1 ? a.W1 : (k == 2 ? a.W2 : a.D));
+    return p[q];
+  } else {
+    return fromBits16(a.H[k][q], P);
+  }
+}
 /* smoother coefficient k (0..2 = W_x, W_y, W_z; 3 = D) at cell q */
 __device__ __forceinline__ float smCoef(const LevelArgs &a, int k, int64_t q)
 {
@@ -348,22 +358,48 @@ struct ColCtx {
   float w6top, om, omm, HR;
 };
 
-template <int B>
+/* The seven coefficients of cell z of the thread's column; P < 0 reads the
+ * operand precision at run time, P >= 0 fixes it (lets the compiler batch
+ * the loads of the register-resident prologue). */
+template <int B, int P = -1>
 __device__ __forceinline__ void loadCoeffs(const ColCtx &c, int z, float &D, float &Bb, float &w1, float &w2, float &w3,
                                            float &w4, float &w5, float &w6)
 {
   const LevelArgs &a = *c.a;
+  auto co = [&](int k, int64_t q) { if constexpr (P < 0) return smCoef(a, k, q); else return smCoefP<P>(a, k, q); };
   const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
-  D = smCoef(a, 3, q);
+  D = co(3, q);
   Bb = c.Bv[q];
-  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : smCoef(a, 0, q);
-  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : smCoef(a, 1, q);
-  w5 = (z == 0 && c.mixed4) ? 1.0f : smCoef(a, 2, q);
-  if (c.tx < B - 1) w2 = smCoef(a, 0, q + 1);
-  else w2 = c.fk1 == FK_SAME ? smCoef(a, 0, c.fo1 + B * c.ty + B * B * z) : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
-  if (c.ty < B - 1) w4 = smCoef(a, 1, q + B);
-  else w4 = c.fk3 == FK_SAME ? smCoef(a, 1, c.fo3 + c.tx + B * B * z) : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
-  w6 = z < B - 1 ? smCoef(a, 2, q + B * B) : c.w6top;
+  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : co(0, q);
+  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : co(1, q);
+  w5 = (z == 0 && c.mixed4) ? 1.0f : co(2, q);
+  if (c.tx < B - 1) w2 = co(0, q + 1);
+  else w2 = c.fk1 == FK_SAME ? co(0, c.fo1 + B * c.ty + B * B * z) : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
+  if (c.ty < B - 1) w4 = co(1, q + B);
+  else w4 = c.fk3 == FK_SAME ? co(1, c.fo3 + c.tx + B * B * z) : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
+  w6 = z < B - 1 ? co(2, q + B * B) : c.w6top;
+}
+
+/* Register-resident prologue: h*x, skip bits and the coefficients of every
+ * slot of the thread's column, with the operand precision fixed. */
+template <int B, int P>
+__device__ __forceinline__ uint32_t loadColumn(ColRegs<B, true> &r, const ColCtx &c, const float *__restrict__ X,
+                                               float h)
+{
+  constexpr int S = Cfg<B>::S;
+  const LevelArgs &a = *c.a;
+  uint32_t skip = 0;
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    const int z = (sl - c.o + 2 * B) & (B - 1);
+    const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
+    const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
+    skip |= (sk ? 1u : 0u) << sl;
+    r.xs[sl] = sk ? 0.0f : X[q] * h;
+    float w6;
+    loadCoeffs<B, P>(c, z, r.cd[sl], r.cb[sl], r.w1[sl], r.w2[sl], r.w3[sl], r.w4[sl], r.w5[sl], w6);
+  }
+  return skip;
 }
 
 template <int B, bool REGC, int DIR, bool UPDATE, int I>
@@ -482,16 +518,18 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
 
   ColRegs<B, REGC> r;
   uint32_t skip = 0;
+  if constexpr (REGC) {
+    if (a.prec == MSBGX_PREC_FP16) skip = loadColumn<B, MSBGX_PREC_FP16>(r, c, X, h);
+    else if (a.prec == MSBGX_PREC_BF16) skip = loadColumn<B, MSBGX_PREC_BF16>(r, c, X, h);
+    else skip = loadColumn<B, MSBGX_PREC_FP32>(r, c, X, h);
+  } else {
 #pragma unroll
-  for (int sl = 0; sl < B; sl++) {
-    const int z = (sl - c.o + 2 * B) & (B - 1);
-    const int64_t q = off + c.tx + B * c.ty + B * B * z;
-    const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
-    skip |= (sk ? 1u : 0u) << sl;
-    r.xs[sl] = sk ? 0.0f : X[q] * h;
-    if constexpr (REGC) {
-      float w6;
-      loadCoeffs<B>(c, z, r.cd[sl], r.cb[sl], r.w1[sl], r.w2[sl], r.w3[sl], r.w4[sl], r.w5[sl], w6);
+    for (int sl = 0; sl < B; sl++) {
+      const int z = (sl - c.o + 2 * B) & (B - 1);
+      const int64_t q = off + c.tx + B * c.ty + B * B * z;
+      const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
+      skip |= (sk ? 1u : 0u) << sl;
+      r.xs[sl] = sk ? 0.0f : X[q] * h;
     }
   }
   __syncthreads();
