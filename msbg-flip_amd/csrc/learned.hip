@@ -1422,9 +1422,25 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
 /* Training pairs from the plain MG-PCG on a prepared grid: a solve to p.tol
  * gives x*; a second solve from 0 stops after every `every`-th iteration,
  * and `perPair` random level-0 PCG blocks of that iterate become samples. */
+/* Device scratch freed on every exit path. */
+struct DevScratch {
+  std::vector<void *> ptrs;
+  ~DevScratch()
+  {
+    for (void *q : ptrs) (void)hipFree(q);
+  }
+  template <typename T> int alloc(T **p, size_t n)
+  {
+    const int rc = dalloc(p, n);
+    if (!rc) ptrs.push_back((void *)*p);
+    return rc;
+  }
+};
+
 int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxPairs, int perPair, uint64_t seed,
                int *pairsOut)
 {
+  DevScratch scratch;
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
@@ -1436,9 +1452,9 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   if (int rc = deviceSolve(g, p, &it, &rr)) return rc;
   float *xs = nullptr, *err = nullptr;
   int *blk = nullptr;
-  if (int rc = dalloc(&xs, n0)) return rc;
-  if (int rc = dalloc(&err, n0)) return rc;
-  if (int rc = dalloc(&blk, perPair)) return rc;
+  if (int rc = scratch.alloc(&xs, n0)) return rc;
+  if (int rc = scratch.alloc(&err, n0)) return rc;
+  if (int rc = scratch.alloc(&blk, perPair)) return rc;
   HIPCHK(hipMemcpyAsync(xs, m0.d_ch[C_X], n0 * sizeof(float), hipMemcpyDeviceToDevice, d.stream));
   if (int rc = deviceZero(g, C_X, 0)) return rc;
   /* second solve, eagerly, sampling after chosen iterations */
@@ -1472,24 +1488,16 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   LearnedCtx *lc = nullptr;
   float *cerr = nullptr, *tsc = nullptr;
   uint8_t *cmask = nullptr;
-  int nt = 0, tscCap = 0;
-  auto ensureCap = [](float **p, int &cap, int need) -> int {
-    if (need <= cap) return MSBGX_OK;
-    if (*p) (void)hipFree(*p);
-    *p = nullptr;
-    if (int rc = dalloc(p, need)) return rc;
-    cap = need;
-    return MSBGX_OK;
-  };
   if (coarse) {
     if (!d.learned) d.learned = new LearnedCtx();
     lc = d.learned;
     lc->n0 = n0;
     if (int rc = coarseSetup(g, *lc, d.stream)) return rc;
-    nt = lc->ntx * lc->nty * lc->ntz;
-    if (int rc = dalloc(&cerr, lc->ncoarse)) return rc;
-    if (int rc = ensureCap(&tsc, tscCap, 8 * nt)) return rc;
-    if (int rc = dalloc(&cmask, lc->ncoarse)) return rc;
+    /* tiles per iterate with any origin offset: at most one more per axis */
+    const int maxTiles = (lc->ntx + 1) * (lc->nty + 1) * (lc->ntz + 1);
+    if (int rc = scratch.alloc(&cerr, lc->ncoarse)) return rc;
+    if (int rc = scratch.alloc(&tsc, maxTiles)) return rc;
+    if (int rc = scratch.alloc(&cmask, lc->ncoarse)) return rc;
     std::vector<uint8_t> hm(lc->ncoarse, 0);
     for (int b : g.active0) hm[b] = 1;
     HIPCHK(hipMemcpy(cmask, hm.data(), hm.size(), hipMemcpyHostToDevice));
@@ -1516,7 +1524,6 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
       const int tx = (g.nbx + ox + 15) / 16, ty = (g.nby + oy + 15) / 16, tz = (g.nbz + oz + 15) / 16;
       const int ntp = tx * ty * tz;
       if (int rc = growSet(n, n.nS + ntp)) return rc;
-      if (int rc = ensureCap(&tsc, tscCap, ntp)) return rc;
       hipLaunchKernelGGL(k_nn_tile_target, dim3(ntp), dim3(256), 0, d.stream, cerr, cmask, g.nbx, g.nby, g.nbz, tx,
                          ty, ox, oy, oz, tsc, n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
       hipLaunchKernelGGL(k_nn_gather_dense, dim3(ntp), dim3(256), 0, d.stream, lc->cin, g.nbx, g.nby, g.nbz, tx, ty,
@@ -1556,11 +1563,7 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
       pairs++;
     }
   }
-  (void)hipFree(xs);
-  (void)hipFree(err);
-  (void)hipFree(blk);
-  for (void *q : {(void *)cerr, (void *)tsc, (void *)cmask})
-    if (q) (void)hipFree(q);
+  HIPCHK(hipStreamSynchronize(d.stream));   // scratch is freed on return
   if (int rc = deviceZero(g, C_X, 0)) return rc;
   *pairsOut = pairs;
   return MSBGX_OK;
