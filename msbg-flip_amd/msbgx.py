@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmsbgx.so")
+LIB_PATH = os.environ.get("MSBGX_LIB") or os.path.join(HERE, "libmsbgx.so")   # MSBGX_LIB: sanitizer builds
 
 MAX_SPHERES = 64
 MAX_BOXES = 4
@@ -453,6 +453,18 @@ class LearnedNet:
         w = np.ascontiguousarray(w, np.float32)
         assert w.size == NET_PARAMS
         _ck(lib.msbgx_net_set_weights(self._h, _ptr(w)), "net_set_weights")
+
+    def save(self, path):
+        """Write the weights (the network's only persistent state) and its form
+        as an .npz file (plain arrays, no pickled objects)."""
+        np.savez(path, weights=self.weights, mode=np.int32(self.mode))
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as f:
+            net = cls(seed=1, mode=int(f["mode"]))
+            net.weights = f["weights"]
+        return net
 
     def set_samples(self, x, target, mask):
         self._samples = [np.ascontiguousarray(x, np.float32), np.ascontiguousarray(target, np.float32),

@@ -196,3 +196,24 @@ def test_coarse_form_training_and_solve():
     it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
     print(f"  coarse-corrected solve: {it} iterations rel {rr:.2e} fallback {fb}", flush=True)
     assert rr <= 1e-6
+
+
+def test_weights_round_trip_and_coarse_form_on_32_blocks(tmp_path):
+    """Saved weights reload bit for bit; the coarse form runs on 32^3 base
+    blocks (zero network = plain iterations)."""
+    net = msbgx.LearnedNet(seed=4, mode=msbgx.NET_COARSE)
+    p = tmp_path / "net.npz"
+    net.save(p)
+    net2 = msbgx.LearnedNet.load(p)
+    assert net2.mode == msbgx.NET_COARSE
+    np.testing.assert_array_equal(net.weights, net2.weights)
+    cfg = msbgx.config_c5(n=128)
+    g = prepared(cfg)
+    it0, _ = g.solve_pressure_flip(max_iter=1000, tol=1e-6)
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    net2.weights = np.zeros(msbgx.NET_PARAMS, np.float32)
+    it, rr, fb = net2.solve(g, max_iter=1000, tol=1e-6)
+    print(f"  32^3 blocks, zero coarse net: {it} iterations (plain {it0}) fallback {fb}", flush=True)
+    assert rr <= 1e-6 and abs(it - it0) <= 3
+    with pytest.raises(msbgx.MsbgxError):
+        msbgx.LearnedNet(seed=1).apply(g, msbgx.CH_R, msbgx.CH_Z)      # the fine form needs 16^3 blocks
