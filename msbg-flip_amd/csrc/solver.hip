@@ -479,6 +479,47 @@ struct PeriodUnroll<B, REGC, DIR, UPDATE, B> {
 
 /* One local sweep: ceil(T/B) periods of B hyperplanes.  B is even, so the
  * double-buffer index I & 1 alternates across period boundaries too. */
+/* The fifth local sweep reads only the fourth sweep's values (a Jacobi
+ * pass, src/msbg3.cpp:1341-1349), so with the coefficients in registers it
+ * needs no wavefront: every thread publishes its whole column to LDS once and
+ * then computes its B cells independently, reading each neighbour at the
+ * cell's own z (or z -+ 1) exactly as the wavefront form does. */
+template <int B>
+__device__ __forceinline__ void colJacobi(ColRegs<B, true> &r, uint32_t skip, float *colz, const ColCtx &c)
+{
+  constexpr int NT = Cfg<B>::NT, NF = Cfg<B>::NF;
+  const int tid = threadIdx.x, tx = c.tx, ty = c.ty;
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    const int z = (sl - c.o + 2 * B) & (B - 1);
+    colz[z * NT + tid] = r.xs[sl];
+  }
+  ldsBarrier();
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    if ((skip >> sl) & 1u) continue;
+    const int z = (sl - c.o + 2 * B) & (B - 1);
+    const int slM = (sl + B - 1) & (B - 1), slP = (sl + 1) & (B - 1);
+    const float F0 = r.xs[sl];
+    const float F1 = tx > 0 ? colz[z * NT + tid - 1] : c.faces[0 * NF + ty + B * z];
+    const float F2 = tx < B - 1 ? colz[z * NT + tid + 1] : c.faces[1 * NF + ty + B * z];
+    const float F3 = ty > 0 ? colz[z * NT + tid - B] : c.faces[2 * NF + tx + B * z];
+    const float F4 = ty < B - 1 ? colz[z * NT + tid + B] : c.faces[3 * NF + tx + B * z];
+    const float F5 = z > 0 ? r.xs[slM] : c.faces[4 * NF + tx + B * ty];
+    const float F6 = z < B - 1 ? r.xs[slP] : c.faces[5 * NF + tx + B * ty];
+    const float D = r.cd[sl], Bb = r.cb[sl];
+    const float w6 = z < B - 1 ? r.w5[slP] : c.w6top;
+    float Sm = F1 * r.w1[sl];
+    Sm = fmaf(F2, r.w2[sl], Sm);
+    Sm = fmaf(F3, r.w3[sl], Sm);
+    Sm = fmaf(F4, r.w4[sl], Sm);
+    Sm = fmaf(F5, r.w5[sl], Sm);
+    Sm = fmaf(F6, w6, Sm);
+    const float Y = D != 0.0f ? fmaf(c.om * (Bb + Sm), D, c.omm * F0) : F0;
+    c.Xg[c.off + tx + B * ty + B * B * z] = Y * c.HR;
+  }
+}
+
 template <int B, bool REGC, int DIR, bool UPDATE>
 __device__ __forceinline__ void colSweep(ColRegs<B, REGC> &r, uint32_t skip, float (*pub)[3][Cfg<B>::NT],
                                          const ColCtx &c)
@@ -498,6 +539,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   constexpr int S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
   __shared__ float faces[6 * NF];
   __shared__ float pub[2][3][NT];
+  __shared__ float colz[REGC ? B * NT : 1];   // one-shot Jacobi sweep: whole columns
   const int bid = blocks[blockIdx.x];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
@@ -540,14 +582,16 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
       colSweep<B, REGC, +1, true>(r, skip, pub, c);
       colSweep<B, REGC, -1, true>(r, skip, pub, c);
     }
-    colSweep<B, REGC, +1, false>(r, skip, pub, c);
+    if constexpr (REGC) colJacobi<B>(r, skip, colz, c);
+    else colSweep<B, REGC, +1, false>(r, skip, pub, c);
   } else {
 #pragma unroll 1
     for (int pair = 0; pair < 2; pair++) {
       colSweep<B, REGC, -1, true>(r, skip, pub, c);
       colSweep<B, REGC, +1, true>(r, skip, pub, c);
     }
-    colSweep<B, REGC, -1, false>(r, skip, pub, c);
+    if constexpr (REGC) colJacobi<B>(r, skip, colz, c);
+    else colSweep<B, REGC, -1, false>(r, skip, pub, c);
   }
 }
 
