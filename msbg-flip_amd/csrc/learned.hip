@@ -47,9 +47,10 @@
  *                   network on the grid of block means, its output extended
  *                   over each block and scaled by the Galerkin step
  *                   <zc, r> / <zc, A zc> before z = zc + V(r - A zc)
- *   training: k_nn_loss (masked MSE gradient), k_nn_bwd4, k_nn_dw16,
- *   k_nn_dw_small, k_nn_reduce (fixed-order FP64 sum of per-workgroup
- *   partials), k_nn_adam, k_nn_pack
+ *   training: k_nn_loss (masked MSE gradient), k_nn_bwd4, k_nn_dw16_mfma
+ *   (16 -> 16 weight gradients on MFMA, one plane per workgroup),
+ *   k_nn_dw_small, k_nn_reduce / k_nn_reduce_groups (fixed-order FP64 sums
+ *   of per-workgroup partials), k_nn_adam, k_nn_pack
  */
 
 namespace msbgx {
@@ -333,33 +334,81 @@ __global__ void __launch_bounds__(256) k_nn_bwd4(const float *__restrict__ dY, c
   }
 }
 
-/* Per-workgroup partial dW[t][ci][co] = sum_pos A[pos + off_t][ci] dZ[pos][co]
- * over the workgroup's two output planes (A staged in LDS). */
-__global__ void __launch_bounds__(256) k_nn_dw16(const uint16_t *__restrict__ A, const uint16_t *__restrict__ dZ,
-                                                 float *__restrict__ partial)
+/* Per-workgroup partial weight gradient of a 16 -> 16 layer on MFMA:
+ * dW[t][ci][co] = sum_pos A[pos + off_t][ci] dZ[pos][co] over one output plane
+ * (grid: samples x 18 planes).  Per tap, a 16 x 16 (ci x co) tile accumulates
+ * over K = 416 plane positions in 13 v_mfma_f32_16x16x32_bf16 (the ring
+ * positions and 400..415 carry dZ = 0); the wave w owns the taps t = w mod 4.  A and dZ are
+ * staged channel-major in LDS so a lane's 8 consecutive positions of one
+ * channel are contiguous. */
+constexpr int kDwTile = 1280;   // staged A positions: three planes, slack and the K padding
+constexpr int kDwGroups = 64;   // first-level groups of the weight-gradient reduction
+__global__ void __launch_bounds__(256) k_nn_dw16_mfma(const uint16_t *__restrict__ A, const uint16_t *__restrict__ dZ,
+                                                      float *__restrict__ partial)
 {
-  __shared__ uint4 lds[nn::LDS_POS * 2];
+  __shared__ uint16_t At[nn::C * kDwTile];
+  __shared__ __attribute__((aligned(16))) uint16_t Gt[nn::C * 416];
   const int i = blockIdx.x;
-  const int z0 = 1 + nn::ZP * blockIdx.y;
+  const int z = 1 + blockIdx.y;                            // output plane 1..18
   const int64_t base = (int64_t)i * nn::PS * nn::C;
-  nnStage16(lds, A + base, z0);
-  __syncthreads();
-  const uint16_t *L = reinterpret_cast<const uint16_t *>(lds);
-  const int co = threadIdx.x & 15, ci = threadIdx.x >> 4;
-  const int ldsOrigin = (z0 - 1) * nn::PL - 32;
-  float acc[nn::T];
+  const int p0 = nn::SL + (z - 1) * nn::PL - 32;           // buffer position of At[.][0]
+  for (int e = threadIdx.x; e < kDwTile * 2; e += blockDim.x) {
+    const int q = e >> 1, h = e & 1;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (p0 + q < nn::PS) v = reinterpret_cast<const uint4 *>(A + base + (int64_t)(p0 + q) * nn::C)[h];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int t = 0; t < nn::T; t++) acc[t] = 0.0f;
-  for (int q = 0; q < nn::ZP * nn::PL; q++) {
-    const int pos = z0 * nn::PL + q;
-    if (nnRing(pos)) continue;
-    const float g = nnBf2f(dZ[base + (int64_t)(nn::SL + pos) * nn::C + co]);
-#pragma unroll
-    for (int t = 0; t < nn::T; t++) acc[t] = fmaf(nnBf2f(L[(pos + nn::tapOff(t) - ldsOrigin) * nn::C + ci]), g, acc[t]);
+    for (int k = 0; k < 4; k++) {
+      At[(8 * h + 2 * k) * kDwTile + q] = (uint16_t)(w[k] & 0xffffu);
+      At[(8 * h + 2 * k + 1) * kDwTile + q] = (uint16_t)(w[k] >> 16);
+    }
   }
-  float *o = partial + ((int64_t)i * gridDim.y + blockIdx.y) * nn::NW2;
+  for (int e = threadIdx.x; e < 416 * 2; e += blockDim.x) {
+    const int q = e >> 1, h = e & 1;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (q < nn::PL && !nnRing(q))
+      v = reinterpret_cast<const uint4 *>(dZ + base + (int64_t)(nn::SL + z * nn::PL + q) * nn::C)[h];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int t = 0; t < nn::T; t++) o[(t * nn::C + ci) * nn::C + co] = acc[t];
+    for (int k = 0; k < 4; k++) {
+      Gt[(8 * h + 2 * k) * 416 + q] = (uint16_t)(w[k] & 0xffffu);
+      Gt[(8 * h + 2 * k + 1) * 416 + q] = (uint16_t)(w[k] >> 16);
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  float *out = partial + ((int64_t)i * gridDim.y + blockIdx.y) * nn::NW2;
+  for (int t = wave; t < nn::T; t += 4) {
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int aoff = 32 + nn::PL + nn::tapOff(t);          // At index of plane position 0 shifted by the tap
+#pragma unroll
+    for (int j = 0; j < 13; j++) {
+      const int k0 = 32 * j + 8 * kg;
+      uint16_t av[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) av[e] = At[m * kDwTile + aoff + k0 + e];
+      uint4 apk;
+      apk.x = av[0] | ((uint32_t)av[1] << 16); apk.y = av[2] | ((uint32_t)av[3] << 16);
+      apk.z = av[4] | ((uint32_t)av[5] << 16); apk.w = av[6] | ((uint32_t)av[7] << 16);
+      const uint4 bpk = *reinterpret_cast<const uint4 *>(&Gt[m * 416 + k0]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, apk), __builtin_bit_cast(bf16x8_t, bpk),
+                                                    acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[(t * nn::C + 4 * kg + r) * nn::C + m] = acc[r];
+  }
+}
+
+/* Two-level fixed-order reduction of partial rows: grid (ceil(nEnt/256),
+ * groups); group g sums rows g, g + groups, ... into part2[g][e]. */
+__global__ void k_nn_reduce_groups(const float *__restrict__ partial, int nPart, int nEnt, float *__restrict__ part2)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEnt) return;
+  double s = 0.0;
+  for (int p = blockIdx.y; p < nPart; p += gridDim.y) s += (double)partial[(int64_t)p * nEnt + e];
+  part2[(int64_t)blockIdx.y * nEnt + e] = (float)s;
 }
 
 /* Per-workgroup partials of the two 432-entry gradients:
@@ -797,7 +846,7 @@ static int allocBuffers(NetBuffers &b, int n, bool training, hipStream_t s)
     HIPCHK(hipMemsetAsync(b.G3, 0, act * 2, s));
     if (int rc = dalloc(&b.out, (size_t)n * nn::NCELL)) return rc;
     if (int rc = dalloc(&b.dY, (size_t)n * nn::NCELL)) return rc;
-    if (int rc = dalloc(&b.part, (size_t)n * nn::NZG * nn::NW2)) return rc;
+    if (int rc = dalloc(&b.part, (size_t)(n * 18 + kDwGroups) * nn::NW2)) return rc;
     if (int rc = dalloc(&b.lossPart, n)) return rc;
     if (int rc = dalloc(&b.idx, n)) return rc;
   }
@@ -898,6 +947,19 @@ static void netForward123(Net &n, const uint16_t *X0, const int *idx, NetBuffers
                      (const uint16_t *)nullptr, b.A3, skip);
 }
 
+/* dW of a 16 -> 16 layer: MFMA partials per (sample, plane), reduced in a
+ * fixed order (groups of rows, then the groups). */
+static void netWeightGrad16(const uint16_t *A, const uint16_t *dZ, NetBuffers &b, int nb, float *grad,
+                            hipStream_t s)
+{
+  const int rows = nb * 18;
+  float *part2 = b.part + (size_t)rows * nn::NW2;
+  hipLaunchKernelGGL(k_nn_dw16_mfma, dim3(nb, 18), dim3(256), 0, s, A, dZ, b.part);
+  hipLaunchKernelGGL(k_nn_reduce_groups, dim3((nn::NW2 + 255) / 256, kDwGroups), dim3(256), 0, s, b.part, rows,
+                     nn::NW2, part2);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, part2, kDwGroups, nn::NW2, grad);
+}
+
 /* Loss and gradient over `nb` samples of the training set addressed by the
  * device index list b.idx (already uploaded).  Leaves grad in n.grad and the
  * per-sample loss partials in b.lossPart. */
@@ -916,15 +978,11 @@ static void netLossGrad(Net &n, NetBuffers &b, int nb, float invN)
                      n.grad + nn::OFF_W4);
   hipLaunchKernelGGL(k_nn_bwd4, dim3(nb, nn::NZG), dim3(256), 0, s, b.dY, n.w4f, b.A3, b.G3);
   /* layer 3: dW3 = corr(A2, G3); G2 = conv(G3, W3b) * (A2 > 0) */
-  hipLaunchKernelGGL(k_nn_dw16, dim3(nb, nn::NZG), dim3(256), 0, s, b.A2, b.G3, b.part);
-  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW2,
-                     n.grad + nn::OFF_W3);
+  netWeightGrad16(b.A2, b.G3, b, nb, n.grad + nn::OFF_W3, s);
   hipLaunchKernelGGL(k_nn_conv16<true>, dim3(nb, nn::NZG), dim3(256), 0, s, b.G3, (const uint4 *)n.b3, b.A2, b.G2,
                      (const int *)nullptr);
   /* layer 2 */
-  hipLaunchKernelGGL(k_nn_dw16, dim3(nb, nn::NZG), dim3(256), 0, s, b.A1, b.G2, b.part);
-  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW2,
-                     n.grad + nn::OFF_W2);
+  netWeightGrad16(b.A1, b.G2, b, nb, n.grad + nn::OFF_W2, s);
   hipLaunchKernelGGL(k_nn_conv16<true>, dim3(nb, nn::NZG), dim3(256), 0, s, b.G2, (const uint4 *)n.b2, b.A1, b.G1,
                      (const int *)nullptr);
   /* layer 1 */
