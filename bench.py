@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-fp64-oracle", action="store_true", help="skip the CPU FP64 reference solve")
     ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form (64 problems) runs")
+    ap.add_argument("--no-flip", action="store_true", help="skip the FLIP caller steps (velocity update)")
     ap.add_argument("--batch-max-iter", type=int, default=400)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner A/B")
     ap.add_argument("--learned-pairs", type=int, default=2000)
@@ -309,6 +310,70 @@ def read_traffic(kernel, workload):
     return None
 
 
+def dense_level0(msbgx, g, ch, n, bsx):
+    """Channel `ch` of the level-0 blocks as a dense [z][y][x] array (0 elsewhere)."""
+    import numpy as np
+    lv, _ = g.block_info(0)
+    out = np.zeros((n, n, n), np.uint16 if ch == msbgx.CH_FLAGS else np.float32)
+    nbx = n // bsx
+    for b in range(len(lv)):
+        if lv[b] != 0:
+            continue
+        bx, by, bz = b % nbx, (b // nbx) % nbx, b // (nbx * nbx)
+        out[bz * bsx:(bz + 1) * bsx, by * bsx:(by + 1) * bsx, bx * bsx:(bx + 1) * bsx] = \
+            g.get_block(ch, 0, b, bsx).reshape(bsx, bsx, bsx)
+    return out, lv
+
+
+def flip_runs(msbgx, g, cfg, args, log):
+    """FLIP caller steps on the device around the solve: divergence right-hand
+    side from the scene's MAC velocity, plain MG-PCG to the tolerance, the
+    pressure-gradient update (corrected sign) and the divergence left after it;
+    HBM roofline of both kernels (HIP events); the CPU oracle's update on the
+    same arrays (1 thread)."""
+    import numpy as np
+    dt = cfg.scene.dt
+    g.set_precision(msbgx.PREC_FP32)
+    g.flip_set_velocity_scene(cfg.scene)
+    g.flip_rhs_from_velocity(dt)
+    blocks = g.level0_blocks()
+    b0 = np.concatenate([g.get_block(msbgx.CH_DIVERGENCE, 0, int(b), cfg.bsx) for b in blocks]).astype(np.float64)
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    it, rr = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+    vel = g.flip_get_velocity()
+    g.flip_update_velocity(dt, corrected=True)
+    g.flip_rhs_from_velocity(dt)
+    b1 = np.concatenate([g.get_block(msbgx.CH_DIVERGENCE, 0, int(b), cfg.bsx) for b in blocks]).astype(np.float64)
+    red = float(np.linalg.norm(b1) / max(np.linalg.norm(b0), 1e-300))
+    log(f"flip: solve {it} iterations rel {rr:.2e}; divergence after the corrected update {red:.2e} of before")
+    out = {"solve_iterations": it, "solve_rel_residual": rr, "divergence_after_update": red}
+    for which, name in ((3, "velocity_update"), (4, "rhs_from_velocity")):
+        ms, by, cells = g.time_kernel(which, reps=20)
+        gbs = by / (ms * 1e-3) / 1e9
+        out[name] = {"launch_ms": round(ms, 4), "cells": cells, "algorithmic_bytes": int(by),
+                     "bytes_per_cell": round(by / max(cells, 1), 2), "GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "Gcells_per_s": round(cells / (ms * 1e-3) / 1e9, 1)}
+        log(f"flip {name}: {ms:.4f} ms, {cells} cells, {gbs:.0f} GB/s")
+    if not args.no_cpu:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+            flags, lv = dense_level0(msbgx, g, msbgx.CH_FLAGS, cfg.n, cfg.bsx)
+            mass = ((flags & (msbgx.CELL_SOLID | msbgx.CELL_VOID)) == 0).astype(np.float32)
+            p, _ = dense_level0(msbgx, g, msbgx.CH_PRESSURE, cfg.n, cfg.bsx)
+            beta = [dense_level0(msbgx, g, ch, cfg.n, cfg.bsx)[0]
+                    for ch in (msbgx.CH_FACE_U, msbgx.CH_FACE_V, msbgx.CH_FACE_W)]
+            t0 = time.time()
+            oracle_lib.flip_update_velocity(lv, cfg.bsx, mass, 0.5, p, beta, vel, dt, negate_p=True)
+            t_cpu = time.time() - t0
+            out["cpu_oracle_update_s"] = round(t_cpu, 3)
+            out["cpu_oracle_update_threads"] = 1
+            log(f"flip cpu oracle update: {t_cpu:.2f} s (1 thread)")
+        except Exception as e:
+            log(f"flip cpu oracle failed: {e}")
+    return out
+
+
 def cpu_baseline(cfg, args, gpu_iters, log):
     """CPU oracle (C++ restatement of the reference solver) timed on the host
     cores on a bounded sample: `cpu_iters` PCG iterations of the same solve."""
@@ -318,7 +383,8 @@ def cpu_baseline(cfg, args, gpu_iters, log):
     so = os.path.join(outdir, "libmsbg_oracle.so")
     try:
         subprocess.run(["g++", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
-                        "-std=c++17", "-shared", "-o", so, os.path.join(ROOT, "oracle", "msbg_oracle.cpp")],
+                        "-std=c++17", "-shared", "-o", so, os.path.join(ROOT, "oracle", "msbg_oracle.cpp"),
+                        os.path.join(ROOT, "oracle", "flip_oracle.cpp")],
                        check=True, capture_output=True)
         os.environ["MSBG_ORACLE_SO"] = so
     except Exception as e:  # fall back to the prebuilt portable build
@@ -425,6 +491,8 @@ def main():
     }
     if not args.no_precision:
         line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
+    if not args.no_flip:
+        line["flip_caller"] = flip_runs(msbgx, r["g"], cfg, args, log)
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:

@@ -280,9 +280,47 @@ int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse);
  *   which 1: residual r = b - A x over the MG0 fluid blocks of the finest
  *            level (30 B per cell)
  *   which 2: PCG update x += a d, r -= a q, |r|^2 (24 B per cell)
- * The channel contents after the call are unspecified (z, q, x, r used). */
+ *   which 3: FLIP pressure-gradient velocity update over the level-0 blocks
+ *            (per cell flags 2 + velocity write 12, pressure 4 on fluid
+ *            cells, velocity and beta 8 per face with a fluid side)
+ *   which 4: FLIP divergence right-hand side from the velocity (per cell
+ *            flags 2 + b 4, velocity 4 per face of a fluid cell) 
+ * The channel contents after the call are unspecified (z, q, x, r, velocity
+ * and divergence used). */
 int msbgx_time_kernel(msbgx_grid *g, int which, int reps, double *ms_per_launch, double *bytes_per_launch,
                       int *cells_per_launch);
+
+/* FLIP caller steps on the device, either side of the solve in
+ * FlipSimulation::pressureProjection (src/flip_sim.cpp:1281-1350).  They need
+ * a prepared single grid.  The MAC velocity lives in three device channels
+ * over the level-0 blocks -- the reference's CH_VEC3_4 at resolution level 0
+ * (src/flip_sim.cpp:363-375): component c of cell (x,y,z) is the velocity on
+ * the cell's -c face; blocks of other levels hold no velocity, read as 0
+ * (getVC, src/flip_sim.cpp:118-125).  A cell is fluid (isFluid(mass)) iff the
+ * cell type sampled at prepare holds neither SOLID nor VOID (flipCellTypeCB,
+ * src/flip_sim.cpp:313-321); beta is the sampled face coefficient
+ * (flipFaceCoeffCB, src/flip_sim.cpp:323-327).  Host arrays are dense
+ * sx0*sy0*sz0 fine grids, x fastest. */
+#define MSBGX_FLIP_SIGN_REFERENCE 0  /* v -= dt*beta*(pR - pL) with the solved p, as written in
+                                        src/flip_sim.cpp:1317-1349                          */
+#define MSBGX_FLIP_SIGN_CORRECTED 1  /* the same update with -p: the MSBG path solves
+                                        A p = +div/dt (flipRhsCB), the update assumes the
+                                        compact path's A p = -div/dt (src/flip_sim.cpp:936-939) */
+/* Velocity channels from host arrays (cells of non-level-0 blocks ignored). */
+int msbgx_flip_set_velocity(msbgx_grid *g, const float *u, const float *v, const float *w);
+/* Velocity channels from a scene's MAC field (msbgx_scene_mac), on the GPU. */
+int msbgx_flip_set_velocity_scene(msbgx_grid *g, const msbgx_scene *scene);
+/* Velocity channels to host arrays (0 on cells of non-level-0 blocks). */
+int msbgx_flip_get_velocity(msbgx_grid *g, float *u, float *v, float *w);
+/* Reference: flipRhsCB (src/flip_sim.cpp:329-338) as sampled by the prepare
+ * (src/msbg3.cpp:2551-2579): the divergence channel of the level-0 blocks
+ * becomes isFluid ? div(v)/dt : 0, from the device velocity. */
+int msbgx_flip_rhs_from_velocity(msbgx_grid *g, float dt);
+/* Reference: the pressure-gradient update of FlipSimulation::pressureProjection
+ * (src/flip_sim.cpp:1317-1349) with the pressure channel: per face, 0 at
+ * index 0 along its axis or when neither side is fluid, else
+ * v - dt*beta*(pR - pL) (a non-fluid side's pressure 0, non-finite results 0). */
+int msbgx_flip_update_velocity(msbgx_grid *g, float dt, int sign_mode);
 
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding

@@ -114,6 +114,21 @@ static int checkSingle(const msbgx_grid *g)
   return MSBGX_OK;
 }
 
+static bool validScene(const msbgx_grid *g, const msbgx_scene *scene)
+{
+  if (!scene || scene->nx != g->sx0 || scene->ny != g->sy0 || scene->nz != g->sz0) {
+    setError("scene size must equal the grid size");
+    return false;
+  }
+  if (scene->n_spheres < 0 || scene->n_spheres > MSBGX_SCENE_MAX_SPHERES || scene->n_boxes < 0 ||
+      scene->n_boxes > MSBGX_SCENE_MAX_BOXES || !(scene->dt != 0.0f) || scene->particles_per_cell < 0 ||
+      scene->particles_per_cell > 64) {
+    setError("invalid scene");
+    return false;
+  }
+  return true;
+}
+
 extern "C" {
 
 const char *msbgx_last_error(void) { return g_lastError.c_str(); }
@@ -236,16 +251,7 @@ int msbgx_prepare_pressure_solve_dense(msbgx_grid *g, const uint16_t *flags, con
 int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene)
 {
   if (int rc = checkSingle(g)) return rc;
-  if (!scene || scene->nx != g->sx0 || scene->ny != g->sy0 || scene->nz != g->sz0) {
-    setError("scene size must equal the grid size");
-    return MSBGX_EINVAL;
-  }
-  if (scene->n_spheres < 0 || scene->n_spheres > MSBGX_SCENE_MAX_SPHERES || scene->n_boxes < 0 ||
-      scene->n_boxes > MSBGX_SCENE_MAX_BOXES || !(scene->dt != 0.0f) || scene->particles_per_cell < 0 ||
-      scene->particles_per_cell > 64) {
-    setError("invalid scene");
-    return MSBGX_EINVAL;
-  }
+  if (!validScene(g, scene)) return MSBGX_EINVAL;
   SceneSampler s;
   s.s = scene;
   return prepareWith(*g, s, scene);
@@ -394,7 +400,8 @@ int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse)
 int msbgx_time_kernel(msbgx_grid *g, int which, int reps, double *ms, double *bytes, int *cells)
 {
   MSBGX_PREPARED_OR_FAIL(g);
-  if (which < 0 || which > 2 || reps < 1 || !ms || !bytes || !cells) { setError("bad argument"); return MSBGX_EINVAL; }
+  if (which < 0 || which > 4 || reps < 1 || !ms || !bytes || !cells) { setError("bad argument"); return MSBGX_EINVAL; }
+  if (which >= 3) return deviceFlipTime(*g, which, reps, ms, bytes, cells);
   return deviceTimeKernel(*g, which, reps, ms, bytes, cells);
 }
 
@@ -591,6 +598,47 @@ int msbgx_solve_pressure_learned(msbgx_grid *g, msbgx_net *n, const msbgx_solve_
   if (int rc = checkLearnedGrid(g, n)) return rc;
   if (int rc = checkParams(p)) return rc;
   return deviceSolveLearned(*g, *netOf(n), *p, iters, rel, fallback_iter);
+}
+
+int msbgx_flip_set_velocity(msbgx_grid *g, const float *u, const float *v, const float *w)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!u || !v || !w) { setError("null velocity array"); return MSBGX_EINVAL; }
+  const float *vel[3] = {u, v, w};
+  return deviceFlipSetVelocity(*g, vel);
+}
+
+int msbgx_flip_set_velocity_scene(msbgx_grid *g, const msbgx_scene *scene)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!validScene(g, scene)) return MSBGX_EINVAL;
+  return deviceFlipSetVelocityScene(*g, *scene);
+}
+
+int msbgx_flip_get_velocity(msbgx_grid *g, float *u, float *v, float *w)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!u || !v || !w) { setError("null velocity array"); return MSBGX_EINVAL; }
+  float *vel[3] = {u, v, w};
+  return deviceFlipGetVelocity(*g, vel);
+}
+
+int msbgx_flip_rhs_from_velocity(msbgx_grid *g, float dt)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!std::isfinite(dt) || dt == 0.0f) { setError("dt must be finite and nonzero"); return MSBGX_EINVAL; }
+  return deviceFlipRhs(*g, dt);
+}
+
+int msbgx_flip_update_velocity(msbgx_grid *g, float dt, int sign_mode)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!std::isfinite(dt)) { setError("dt must be finite"); return MSBGX_EINVAL; }
+  if (sign_mode != MSBGX_FLIP_SIGN_REFERENCE && sign_mode != MSBGX_FLIP_SIGN_CORRECTED) {
+    setError("sign_mode must be MSBGX_FLIP_SIGN_REFERENCE or MSBGX_FLIP_SIGN_CORRECTED");
+    return MSBGX_EINVAL;
+  }
+  return deviceFlipUpdate(*g, dt, sign_mode == MSBGX_FLIP_SIGN_CORRECTED ? 1 : 0);
 }
 
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)

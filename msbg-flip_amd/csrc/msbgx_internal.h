@@ -137,6 +137,13 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel);
 int deviceGetBlock(const Grid &g, int ch, int L, int bid, void *dst);
 int deviceSetBlock(Grid &g, int ch, int L, int bid, const float *src);
 int deviceRelax(Grid &g, int L, bool reverse, int nIter, int chX, int chB);
+/* FLIP caller steps (flip.hip) */
+int deviceFlipSetVelocity(Grid &g, const float *const vel[3]);
+int deviceFlipGetVelocity(Grid &g, float *const vel[3]);
+int deviceFlipSetVelocityScene(Grid &g, const msbgx_scene &s);
+int deviceFlipRhs(Grid &g, float dt);
+int deviceFlipUpdate(Grid &g, float dt, int negate);
+int deviceFlipTime(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
 int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst);
 int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst);
 int deviceProlongate(Grid &g, int Lf, int ch);

@@ -982,7 +982,25 @@ struct Dev {
   float *omegaOf = nullptr;
   BatchDev *batch = nullptr;
   LearnedCtx *learned = nullptr;   // buffers and graph of the learned-correction solve
+  /* FLIP caller steps: velocity channels (MG0 pool layout) and the level-0 block list */
+  float *vel[3] = {nullptr, nullptr, nullptr};
+  int *level0 = nullptr;
+  uint8_t *level0NoFluid = nullptr;
+  int nLevel0 = 0;
 };
+
+static void freeFlip(Dev &d)
+{
+  for (int c = 0; c < 3; c++) {
+    if (d.vel[c]) (void)hipFree(d.vel[c]);
+    d.vel[c] = nullptr;
+  }
+  if (d.level0) (void)hipFree(d.level0);
+  if (d.level0NoFluid) (void)hipFree(d.level0NoFluid);
+  d.level0 = nullptr;
+  d.level0NoFluid = nullptr;
+  d.nLevel0 = 0;
+}
 
 static Dev &dev(Grid &g) { return *(Dev *)g.dev; }
 
@@ -1020,6 +1038,7 @@ void deviceFree(Grid &g)
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     f(d.nbr); f(d.probOf); f(d.omegaOf);
     freeLearned(d.learned);
+    freeFlip(d);
     if (BatchDev *b = d.batch) {
       if (b->graph) (void)hipGraphExecDestroy(b->graph);
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
@@ -1104,6 +1123,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
   freeLearned(d.learned);
   d.learned = nullptr;
+  freeFlip(d);
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
   int maxBlocksPerLaunch = std::max(g.nBlocks, kVecBlocks);
   if (d.partialsCap < maxBlocksPerLaunch) {
@@ -1456,6 +1476,7 @@ int deviceSetBlock(Grid &g, int ch, int L, int bid, const float *src)
   const MgLevel &m = g.mg[L];
   const int b = g.bsxL(m.level[bid]);
   HIPCHK(hipMemcpy(p + m.off[bid], src, (size_t)b * b * b * sizeof(float), hipMemcpyHostToDevice));
+  if (ch == C_B && L == 0) dev(g).b0Valid = false;   // the refinement's saved right-hand side is stale
   return b;
 }
 
@@ -2188,6 +2209,7 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
 
 /* The learned correction (network kernels, training, PCG integration). */
 #include "learned.hip"
+#include "flip.hip"
 
 namespace msbgx {
 /* Captured graphs bake in launch shapes and kernel choices: drop them. */

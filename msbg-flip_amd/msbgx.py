@@ -204,6 +204,11 @@ def _load():
         "msbgx_net_time_conv": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "msbgx_solve_pressure_learned": (C.c_int, [P, P, C.POINTER(SolveParams), C.POINTER(C.c_int),
                                                    C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+        "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
+        "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
+        "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
+        "msbgx_flip_rhs_from_velocity": (C.c_int, [P, C.c_float]),
+        "msbgx_flip_update_velocity": (C.c_int, [P, C.c_float, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -228,9 +233,12 @@ EXPORTED = [
     "msbgx_net_set_weights", "msbgx_net_set_samples", "msbgx_net_loss_grad", "msbgx_net_collect",
     "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
     "msbgx_solve_pressure_learned", "msbgx_net_time_conv", "msbgx_net_set_mode", "msbgx_net_get_mode",
-    "msbgx_net_set_stall",
+    "msbgx_net_set_stall", "msbgx_flip_set_velocity", "msbgx_flip_set_velocity_scene", "msbgx_flip_get_velocity",
+    "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity",
 ]
 
+OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
+FLIP_SIGN_REFERENCE, FLIP_SIGN_CORRECTED = 0, 1
 NET_PARAMS = 14688
 NET_FINE, NET_COARSE = 0, 1
 
@@ -318,6 +326,33 @@ class MultiresSparseGrid:
         it, rr = C.c_int(), C.c_float()
         _ck(lib.msbgx_solve_pressure(self._h, C.byref(p), C.byref(it), C.byref(rr)), "solve")
         return it.value, rr.value
+
+    # FLIP caller steps either side of the solve (src/flip_sim.cpp:1281-1350);
+    # dense arrays are indexed [z][y][x], component c on each cell's -c face.
+    def flip_set_velocity(self, u, v, w):
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+        if any(a.size != n for a in arrs):
+            raise ValueError("velocity arrays must hold sx0*sy0*sz0 values")
+        _ck(lib.msbgx_flip_set_velocity(self._h, *[_ptr(a) for a in arrs]), "flip_set_velocity")
+
+    def flip_set_velocity_scene(self, scene):
+        _ck(lib.msbgx_flip_set_velocity_scene(self._h, C.byref(scene)), "flip_set_velocity_scene")
+
+    def flip_get_velocity(self):
+        sx, sy, sz = self.shape
+        out = [np.zeros((sz, sy, sx), np.float32) for _ in range(3)]
+        _ck(lib.msbgx_flip_get_velocity(self._h, *[_ptr(a) for a in out]), "flip_get_velocity")
+        return out
+
+    def flip_rhs_from_velocity(self, dt):
+        _ck(lib.msbgx_flip_rhs_from_velocity(self._h, dt), "flip_rhs_from_velocity")
+
+    def flip_update_velocity(self, dt, corrected=True):
+        """Pressure-gradient update with the pressure channel; `corrected`
+        applies it to -p (the MSBG path solves A p = +div/dt)."""
+        mode = FLIP_SIGN_CORRECTED if corrected else FLIP_SIGN_REFERENCE
+        _ck(lib.msbgx_flip_update_velocity(self._h, dt, mode), "flip_update_velocity")
 
     def set_precision(self, prec):
         _ck(lib.msbgx_set_precision(self._h, prec), "set_precision")

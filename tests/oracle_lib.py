@@ -38,6 +38,9 @@ for name, res, args in [
     ("orc_solve", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float), P, C.c_int]),
     ("orc_export_level0", None, [P, C.c_int, P]),
     ("orc_solve_fp64", C.c_int, [P, C.c_double, C.c_int, P, C.POINTER(C.c_double)]),
+    ("orc_flip_rhs", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, C.c_float, P]),
+    ("orc_flip_update_velocity", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, P, P, P, C.c_float,
+                                                        C.c_int]),
 ]:
     f = getattr(_lib, name)
     f.restype, f.argtypes = res, args
@@ -167,3 +170,29 @@ def build_oracle(cfg, walls=1, threads=None):
         lv = level_map_from_scene(cfg.scene, cfg.bsx, cfg.levels, cfg.level_mode, cfg.level_d0)
     o.set_refinement_map(o.regularize_refinement_map(lv))
     return o
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def flip_rhs(block_level, bsx, mass, mass_eps, vel, dt):
+    """flipRhsCB on every cell with level-0 data (arrays [z][y][x])."""
+    nz, ny, nx = mass.shape
+    lv = np.ascontiguousarray(block_level, np.int32)
+    m, u, v, w = (_f32(a) for a in (mass, *vel))
+    out = np.zeros_like(m)
+    _lib.orc_flip_rhs(nx, ny, nz, bsx, _p(lv), _p(m), mass_eps, _p(u), _p(v), _p(w), dt, _p(out))
+    return out
+
+
+def flip_update_velocity(block_level, bsx, mass, mass_eps, p, beta, vel, dt, negate_p):
+    """Pressure-gradient update of src/flip_sim.cpp:1317-1349; returns new (u, v, w)."""
+    nz, ny, nx = mass.shape
+    lv = np.ascontiguousarray(block_level, np.int32)
+    m, pp = _f32(mass), _f32(p)
+    b = [_f32(a) for a in beta]
+    out = [_f32(a).copy() for a in vel]
+    _lib.orc_flip_update_velocity(nx, ny, nz, bsx, _p(lv), _p(m), mass_eps, _p(pp), *[_p(a) for a in b],
+                                  *[_p(a) for a in out], dt, int(negate_p))
+    return out

@@ -48,6 +48,27 @@ def test_create_rejects_bad_arguments():
         g.solve_pressure_flip()                               # solve before prepare
 
 
+def test_flip_steps_reject_bad_calls():
+    g = msbgx.MultiresSparseGrid.create(64, 64, 64, 16, 3)
+    z = np.zeros((64, 64, 64), np.float32)
+    for call in (lambda: g.flip_update_velocity(0.1), lambda: g.flip_rhs_from_velocity(0.1),
+                 lambda: g.flip_set_velocity(z, z, z), lambda: g.flip_get_velocity()):
+        with pytest.raises(msbgx.MsbgxError) as e:
+            call()                                            # no prepare yet
+        assert e.value.rc == msbgx.ESTATE
+    with pytest.raises(ValueError):
+        g.flip_set_velocity(z[:10], z, z)
+    for dt in (float("nan"), float("inf")):
+        with pytest.raises(msbgx.MsbgxError) as e:
+            g.flip_update_velocity(dt)
+        assert e.value.rc == msbgx.EINVAL
+    with pytest.raises(msbgx.MsbgxError) as e:
+        g.flip_rhs_from_velocity(0.0)
+    assert e.value.rc == msbgx.EINVAL
+    rc = msbgx.lib.msbgx_flip_update_velocity(g._h, 0.1, 7)
+    assert rc == msbgx.EINVAL
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_regularize_and_flags_match_oracle(seed):
     rng = np.random.default_rng(seed)
