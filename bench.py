@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--no-fp64-oracle", action="store_true", help="skip the CPU FP64 reference solve")
     ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form (64 problems) runs")
-    ap.add_argument("--no-flip", action="store_true", help="skip the FLIP caller steps (velocity update)")
+    ap.add_argument("--no-flip", action="store_true",
+                    help="skip the FLIP caller steps (velocity update, particle-to-grid transfer)")
+    ap.add_argument("--p2g-n", type=int, default=1024, help="domain size of the particle-to-grid run")
     ap.add_argument("--batch-max-iter", type=int, default=400)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner A/B")
     ap.add_argument("--learned-pairs", type=int, default=2000)
@@ -374,6 +376,44 @@ def flip_runs(msbgx, g, cfg, args, log):
     return out
 
 
+def p2g_runs(msbgx, args, log):
+    """Particle-to-grid transfer of the full-scale particle scene (dam break +
+    sphere, 8 particles per liquid cell) on the GPU, and GPU vs the CPU oracle
+    on the same scene at 256^3 (the oracle is serial)."""
+    out = {}
+    for n in sorted({args.p2g_n, 256}, reverse=True):
+        cfg = msbgx.config_c5(n, bsx=16, levels=4)
+        g = msbgx.build_grid(cfg)
+        t = msbgx.ParticleToGrid(g)
+        t0 = time.time()
+        npart = t.run_scene(cfg.scene)
+        wall = time.time() - t0
+        t.run_scene(cfg.scene)                   # second run: steady state
+        ms, msg, _ = t.last_times()
+        act = len(t.active_blocks())
+        rec = {"workload": f"dam break + sphere {n}^3, 8 particles per liquid cell, 16^3 blocks",
+               "particles": npart, "gpu_ms": round(ms, 3), "gather_ms": round(msg, 3),
+               "binning_ms": round(ms - msg, 3), "Gparticles_per_s": round(npart / (ms * 1e-3) / 1e9, 3),
+               "active_blocks": act, "first_call_wall_s": round(wall, 2)}
+        log(f"p2g {n}^3: {npart} particles, {ms:.2f} ms ({msg:.2f} ms gather), {act} active blocks")
+        if n == 256 and not args.no_cpu:
+            try:
+                sys.path.insert(0, os.path.join(ROOT, "tests"))
+                import oracle_lib
+                lv, _ = g.block_info(0)
+                hp = oracle_lib.scene_particles(cfg.scene)
+                t0 = time.time()
+                oracle_lib.p2g(lv, n, n, n, cfg.bsx, hp)
+                rec["cpu_oracle_s"] = round(time.time() - t0, 3)
+                rec["cpu_oracle_threads"] = 1
+                log(f"p2g cpu oracle {n}^3: {rec['cpu_oracle_s']:.2f} s (1 thread)")
+            except Exception as e:
+                log(f"p2g cpu oracle failed: {e}")
+        out[f"n{n}"] = rec
+        del t, g
+    return out
+
+
 def cpu_baseline(cfg, args, gpu_iters, log):
     """CPU oracle (C++ restatement of the reference solver) timed on the host
     cores on a bounded sample: `cpu_iters` PCG iterations of the same solve."""
@@ -493,6 +533,7 @@ def main():
         line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
     if not args.no_flip:
         line["flip_caller"] = flip_runs(msbgx, r["g"], cfg, args, log)
+        line["particle_to_grid"] = p2g_runs(msbgx, args, log)
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:

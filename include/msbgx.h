@@ -322,6 +322,52 @@ int msbgx_flip_rhs_from_velocity(msbgx_grid *g, float dt);
  * v - dt*beta*(pR - pL) (a non-fluid side's pressure 0, non-finite results 0). */
 int msbgx_flip_update_velocity(msbgx_grid *g, float dt, int sign_mode);
 
+/* Particle-to-grid transfer (reference: FlipSimulation::particleToGrid,
+ * src/flip_sim.cpp:579-828) on the GPU.  A transfer handle snapshots a grid's
+ * raster and level map (call after msbgx_set_refinement_map, before the
+ * prepare that will use its outputs).  Outputs live on the level-0 blocks,
+ * as the reference's resolution-level-0 grids: cell mass (particle count),
+ * face velocities (weighted mean, 0 where the weight sum <= mass_eps), face
+ * beta from the face density phi with the liquid/air interface clamp, and
+ * the active blocks (a cell with mass > mass_eps).  The reference's sums run
+ * in its thread partition's order; here each face gathers its 18 source
+ * cells in (z, y, x) order and each cell's particles in index order, so the
+ * result is deterministic. */
+typedef struct msbgx_particle {   /* layout of FlipParticle (src/flip_sim.h:48) */
+  float pos[3];                   /* cell units: cell (i,j,k) spans [i, i+1) x ... */
+  float vel[3];
+  int32_t phase;                  /* 0 liquid, 1 air */
+} msbgx_particle;
+typedef struct msbgx_p2g_params { /* FlipConfig fields (src/flip_sim.h:34-42)   */
+  float rho_l;                    /* 1000 */
+  float rho_g;                    /* 1    */
+  float mass_eps;                 /* 1e-5 */
+  float beta_min;                 /* 1e-6 */
+  float alpha_phi;                /* 1    */
+  int32_t particles_per_voxel;    /* 8    */
+} msbgx_p2g_params;
+typedef struct msbgx_p2g msbgx_p2g;
+void msbgx_default_p2g_params(msbgx_p2g_params *p);
+int msbgx_p2g_create(msbgx_p2g **out, const msbgx_grid *g);
+void msbgx_p2g_destroy(msbgx_p2g *t);
+/* Transfer of n particles from a host array (copied to the device first). */
+int msbgx_p2g_run(msbgx_p2g *t, int64_t n, const msbgx_particle *particles, const msbgx_p2g_params *p);
+/* Transfer of a scene's particles (msbgx_scene_particle for every liquid
+ * cell, generated on the GPU); *n_particles receives their count. */
+int msbgx_p2g_run_scene(msbgx_p2g *t, const msbgx_scene *scene, const msbgx_p2g_params *p, int64_t *n_particles);
+/* Outputs as dense sx0*sy0*sz0 host arrays, x fastest, 0 outside the level-0
+ * blocks; any pointer may be NULL. */
+int msbgx_p2g_get(msbgx_p2g *t, float *mass, float *u, float *v, float *w, float *beta_u, float *beta_v,
+                  float *beta_w);
+/* Active blocks (ascending), up to cap; returns their count. */
+int msbgx_p2g_active_blocks(msbgx_p2g *t, int *out, int cap);
+/* Device time of the last transfer (HIP events): whole transfer and the
+ * gather kernel alone, in ms, and the particle count. */
+int msbgx_p2g_last_times(const msbgx_p2g *t, double *ms_total, double *ms_gather, int64_t *n_particles);
+/* The last transfer's velocity into a prepared grid's FLIP velocity channels
+ * (same raster and level map). */
+int msbgx_p2g_to_velocity(msbgx_p2g *t, msbgx_grid *g);
+
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
  * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);

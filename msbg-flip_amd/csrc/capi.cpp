@@ -93,6 +93,9 @@ int prepareWith(Grid &g, Sampler &s, const msbgx_scene *scene)
 using namespace msbgx;
 
 struct msbgx_grid : Grid {};
+struct msbgx_p2g {
+  msbgx::P2gCtx *c = nullptr;
+};
 
 /* MSBGX_NO_GRAPH=1 in the environment makes new handles launch PCG
  * iterations eagerly (profiling runs whose tracer does not handle graph
@@ -639,6 +642,79 @@ int msbgx_flip_update_velocity(msbgx_grid *g, float dt, int sign_mode)
     return MSBGX_EINVAL;
   }
   return deviceFlipUpdate(*g, dt, sign_mode == MSBGX_FLIP_SIGN_CORRECTED ? 1 : 0);
+}
+
+void msbgx_default_p2g_params(msbgx_p2g_params *p)
+{
+  if (!p) return;
+  p->rho_l = 1000.0f;
+  p->rho_g = 1.0f;
+  p->mass_eps = 1e-5f;
+  p->beta_min = 1e-6f;
+  p->alpha_phi = 1.0f;
+  p->particles_per_voxel = 8;
+}
+
+int msbgx_p2g_create(msbgx_p2g **out, const msbgx_grid *g)
+{
+  if (!out) { setError("out is null"); return MSBGX_EINVAL; }
+  *out = nullptr;
+  if (int rc = checkSingle(g)) return rc;
+  msbgx_p2g *t = new msbgx_p2g();
+  const int rc = p2gCreate(&t->c, *g);
+  if (rc) {
+    p2gDestroy(t->c);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return MSBGX_OK;
+}
+
+void msbgx_p2g_destroy(msbgx_p2g *t)
+{
+  if (!t) return;
+  p2gDestroy(t->c);
+  delete t;
+}
+
+int msbgx_p2g_run(msbgx_p2g *t, int64_t n, const msbgx_particle *particles, const msbgx_p2g_params *p)
+{
+  if (!t || !p || n < 0 || (n > 0 && !particles)) { setError("bad argument"); return MSBGX_EINVAL; }
+  return p2gRun(*t->c, n, particles, *p);
+}
+
+int msbgx_p2g_run_scene(msbgx_p2g *t, const msbgx_scene *scene, const msbgx_p2g_params *p, int64_t *n_particles)
+{
+  if (!t || !p || !scene || !n_particles) { setError("bad argument"); return MSBGX_EINVAL; }
+  return p2gRunScene(*t->c, *scene, *p, n_particles);
+}
+
+int msbgx_p2g_get(msbgx_p2g *t, float *mass, float *u, float *v, float *w, float *beta_u, float *beta_v, float *beta_w)
+{
+  if (!t) { setError("null transfer handle"); return MSBGX_EINVAL; }
+  float *vel[3] = {u, v, w}, *beta[3] = {beta_u, beta_v, beta_w};
+  return p2gGet(*t->c, mass, vel, beta);
+}
+
+int msbgx_p2g_active_blocks(msbgx_p2g *t, int *out, int cap)
+{
+  if (!t || cap < 0) { setError("bad argument"); return MSBGX_EINVAL; }
+  return p2gActiveBlocks(*t->c, out, cap);
+}
+
+int msbgx_p2g_last_times(const msbgx_p2g *t, double *ms_total, double *ms_gather, int64_t *n_particles)
+{
+  if (!t) { setError("null transfer handle"); return MSBGX_EINVAL; }
+  p2gTimes(*t->c, ms_total, ms_gather, n_particles);
+  return MSBGX_OK;
+}
+
+int msbgx_p2g_to_velocity(msbgx_p2g *t, msbgx_grid *g)
+{
+  if (!t) { setError("null transfer handle"); return MSBGX_EINVAL; }
+  if (int rc = checkSingle(g)) return rc;
+  return p2gToVelocity(*t->c, *g);
 }
 
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)

@@ -144,6 +144,16 @@ int deviceFlipSetVelocityScene(Grid &g, const msbgx_scene &s);
 int deviceFlipRhs(Grid &g, float dt);
 int deviceFlipUpdate(Grid &g, float dt, int negate);
 int deviceFlipTime(Grid &g, int which, int reps, double *ms, double *bytes, int *cells);
+/* particle-to-grid transfer (p2g.hip) */
+struct P2gCtx;
+int p2gCreate(P2gCtx **out, const Grid &g);
+void p2gDestroy(P2gCtx *c);
+int p2gRun(P2gCtx &c, int64_t n, const msbgx_particle *host, const msbgx_p2g_params &p);
+int p2gRunScene(P2gCtx &c, const msbgx_scene &s, const msbgx_p2g_params &p, int64_t *nOut);
+int p2gGet(P2gCtx &c, float *mass, float *const vel[3], float *const beta[3]);
+int p2gActiveBlocks(P2gCtx &c, int *out, int cap);
+void p2gTimes(const P2gCtx &c, double *msTotal, double *msGather, int64_t *particles);
+int p2gToVelocity(P2gCtx &c, Grid &g);
 int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst);
 int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst);
 int deviceProlongate(Grid &g, int Lf, int ch);

@@ -2210,6 +2210,7 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
 /* The learned correction (network kernels, training, PCG integration). */
 #include "learned.hip"
 #include "flip.hip"
+#include "p2g.hip"
 
 namespace msbgx {
 /* Captured graphs bake in launch shapes and kernel choices: drop them. */

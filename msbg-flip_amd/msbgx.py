@@ -139,6 +139,16 @@ def config_c5(n=1024, seed=1234, bsx=32, levels=4):
     return Config("c5_dam_break_particles", n, bsx, levels, s, level_mode=0, level_d0=1)
 
 
+class P2GParams(C.Structure):
+    """msbgx_p2g_params (FlipConfig fields, src/flip_sim.h:34-42)."""
+    _fields_ = [("rho_l", C.c_float), ("rho_g", C.c_float), ("mass_eps", C.c_float), ("beta_min", C.c_float),
+                ("alpha_phi", C.c_float), ("particles_per_voxel", C.c_int32)]
+
+
+#: msbgx_particle / FlipParticle (src/flip_sim.h:48): position, velocity, phase
+PARTICLE_DTYPE = np.dtype([("pos", np.float32, 3), ("vel", np.float32, 3), ("phase", np.int32)])
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libmsbgx.so not built at {LIB_PATH}; run __graft_entry__.build()")
@@ -204,6 +214,15 @@ def _load():
         "msbgx_net_time_conv": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "msbgx_solve_pressure_learned": (C.c_int, [P, P, C.POINTER(SolveParams), C.POINTER(C.c_int),
                                                    C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+        "msbgx_default_p2g_params": (None, [C.POINTER(P2GParams)]),
+        "msbgx_p2g_create": (C.c_int, [C.POINTER(P), P]),
+        "msbgx_p2g_destroy": (None, [P]),
+        "msbgx_p2g_run": (C.c_int, [P, C.c_int64, P, C.POINTER(P2GParams)]),
+        "msbgx_p2g_run_scene": (C.c_int, [P, C.POINTER(Scene), C.POINTER(P2GParams), C.POINTER(C.c_int64)]),
+        "msbgx_p2g_get": (C.c_int, [P] + [P] * 7),
+        "msbgx_p2g_active_blocks": (C.c_int, [P, P, C.c_int]),
+        "msbgx_p2g_last_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "msbgx_p2g_to_velocity": (C.c_int, [P, P]),
         "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
         "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
         "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
@@ -234,7 +253,9 @@ EXPORTED = [
     "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
     "msbgx_solve_pressure_learned", "msbgx_net_time_conv", "msbgx_net_set_mode", "msbgx_net_get_mode",
     "msbgx_net_set_stall", "msbgx_flip_set_velocity", "msbgx_flip_set_velocity_scene", "msbgx_flip_get_velocity",
-    "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity",
+    "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity", "msbgx_default_p2g_params", "msbgx_p2g_create",
+    "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
+    "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
 ]
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
@@ -600,3 +621,61 @@ def level_map(cfg: Config):
     if cfg.level_mode < 0:
         return np.zeros(nb, np.int32)
     return level_map_from_scene(cfg.scene, cfg.bsx, cfg.levels, cfg.level_mode, cfg.level_d0)
+
+
+class ParticleToGrid:
+    """Particle-to-grid transfer (FlipSimulation::particleToGrid,
+    src/flip_sim.cpp:579-828) over a grid's raster and level map."""
+
+    def __init__(self, grid):
+        h = C.c_void_p()
+        _ck(lib.msbgx_p2g_create(C.byref(h), grid._h), "p2g_create")
+        self._h = h
+        self.shape = grid.shape
+        self.n_blocks = grid.n_blocks
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.msbgx_p2g_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def params(**kw):
+        p = P2GParams()
+        lib.msbgx_default_p2g_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def run(self, particles, **kw):
+        a = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+        p = self.params(**kw)
+        _ck(lib.msbgx_p2g_run(self._h, len(a), a.ctypes.data_as(C.c_void_p), C.byref(p)), "p2g_run")
+
+    def run_scene(self, scene, **kw):
+        p = self.params(**kw)
+        n = C.c_int64()
+        _ck(lib.msbgx_p2g_run_scene(self._h, C.byref(scene), C.byref(p), C.byref(n)), "p2g_run_scene")
+        return n.value
+
+    def get(self):
+        """dict of dense [z][y][x] arrays: mass, u, v, w, beta_u, beta_v, beta_w"""
+        sx, sy, sz = self.shape
+        names = ["mass", "u", "v", "w", "beta_u", "beta_v", "beta_w"]
+        out = {k: np.zeros((sz, sy, sx), np.float32) for k in names}
+        _ck(lib.msbgx_p2g_get(self._h, *[_ptr(out[k]) for k in names]), "p2g_get")
+        return out
+
+    def active_blocks(self):
+        out = np.zeros(self.n_blocks, np.int32)
+        n = _ck(lib.msbgx_p2g_active_blocks(self._h, _ptr(out), self.n_blocks), "p2g_active_blocks")
+        return out[:n].copy()
+
+    def last_times(self):
+        """(ms whole transfer, ms gather kernel, particles) of the last run"""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        _ck(lib.msbgx_p2g_last_times(self._h, C.byref(a), C.byref(b), C.byref(n)), "p2g_last_times")
+        return a.value, b.value, n.value
+
+    def to_velocity(self, grid):
+        _ck(lib.msbgx_p2g_to_velocity(self._h, grid._h), "p2g_to_velocity")

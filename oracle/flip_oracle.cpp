@@ -19,7 +19,9 @@
  * Compiled with -ffp-contract=off: every float operation is the one written.
  */
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 namespace {
 
@@ -137,3 +139,169 @@ void orc_flip_update_velocity(int nx, int ny, int nz, int bsx, const int *blockL
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------
+ * particleToGrid (src/flip_sim.cpp:579-828), serial.
+ *
+ * The reference scatters from TBB worker threads into thread-local block
+ * buffers and merges them, so the order of its float sums follows the thread
+ * partition.  This restatement is its scatter loop run by one thread over the
+ * particles in cell-major order (cells of the domain plus a one-cell halo, x
+ * fastest; within a cell by particle index), which fixes the order of every
+ * sum: each face receives its contributions by source cell, then particle.
+ * The grids are resolution-level-0 channels: accumulation covers the whole
+ * raster as the reference's block buffers do, and only blocks of level 0
+ * receive the results.
+ * ---------------------------------------------------------------------- */
+namespace {
+struct OrcParticle {
+  float pos[3], vel[3];
+  int32_t phase;
+};
+}  // namespace
+
+extern "C" {
+
+void orc_p2g(int nx, int ny, int nz, int bsx, const int *blockLevel, int64_t np, const void *particles, float rhoL,
+             float rhoG, float massEps, float betaMin, float alphaPhi, int ppv, float *outMass, float *outU,
+             float *outV, float *outW, float *outBU, float *outBV, float *outBW, uint8_t *outActive)
+{
+  const OrcParticle *P = (const OrcParticle *)particles;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const Level0Grid G{nx, ny, nz, bsx, blockLevel};
+  /* cell-major order over the domain plus a one-cell halo */
+  const int64_t hx = nx + 2, hy = ny + 2, hz = nz + 2;
+  std::vector<int64_t> bin(np), cnt(hx * hy * hz + 1, 0);
+  for (int64_t i = 0; i < np; i++) {
+    const int ix = (int)floorf(P[i].pos[0]), iy = (int)floorf(P[i].pos[1]), iz = (int)floorf(P[i].pos[2]);
+    bin[i] = (ix >= -1 && iy >= -1 && iz >= -1 && ix <= nx && iy <= ny && iz <= nz)
+                 ? ((int64_t)(iz + 1) * hy + (iy + 1)) * hx + (ix + 1)
+                 : -1;
+    if (bin[i] >= 0) cnt[bin[i] + 1]++;
+  }
+  for (int64_t c = 0; c < hx * hy * hz; c++) cnt[c + 1] += cnt[c];
+  std::vector<int64_t> order(cnt[hx * hy * hz]);
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < np; i++)
+      if (bin[i] >= 0) order[fill[bin[i]]++] = i;   // ascending i within a cell
+  }
+  std::vector<float> vx(n, 0.f), vy(n, 0.f), vz(n, 0.f), mass(n, 0.f), wu(n, 0.f), wv(n, 0.f), ww(n, 0.f),
+      ru(n, 0.f), rv(n, 0.f), rw(n, 0.f);
+  std::vector<int> liq(n, 0);
+  auto bv = [&](int ix, int iy, int iz, int64_t &vid) -> bool {
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= nx || iy >= ny || iz >= nz) return false;
+    vid = ((int64_t)iz * ny + iy) * nx + ix;
+    return true;
+  };
+  for (int64_t k : order) {
+    const OrcParticle &p = P[k];
+    const float mp = (p.phase == 0) ? rhoL : rhoG;
+    {
+      int64_t vid;
+      if (bv((int)floorf(p.pos[0]), (int)floorf(p.pos[1]), (int)floorf(p.pos[2]), vid)) {
+        mass[vid] += 1.f;
+        if (p.phase == 0) liq[vid]++;
+      }
+    }
+    for (int dir = 0; dir < 3; dir++) {
+      float *sv = dir == 0 ? vx.data() : (dir == 1 ? vy.data() : vz.data());
+      float *sw = dir == 0 ? wu.data() : (dir == 1 ? wv.data() : ww.data());
+      float *sr = dir == 0 ? ru.data() : (dir == 1 ? rv.data() : rw.data());
+      const float gx = dir == 0 ? p.pos[0] : p.pos[0] - 0.5f;
+      const float gy = dir == 1 ? p.pos[1] : p.pos[1] - 0.5f;
+      const float gz = dir == 2 ? p.pos[2] : p.pos[2] - 0.5f;
+      const int ix0 = (int)floorf(gx), iy0 = (int)floorf(gy), iz0 = (int)floorf(gz);
+      const float fx = gx - ix0, fy = gy - iy0, fz = gz - iz0;
+      for (int dz = 0; dz <= 1; dz++)
+        for (int dy = 0; dy <= 1; dy++)
+          for (int dx = 0; dx <= 1; dx++) {
+            const float w = (dx ? fx : 1.f - fx) * (dy ? fy : 1.f - fy) * (dz ? fz : 1.f - fz);
+            int64_t vid;
+            if (bv(ix0 + dx, iy0 + dy, iz0 + dz, vid)) {
+              sv[vid] += p.vel[dir] * w;
+              sw[vid] += w;
+              sr[vid] += mp * w;
+            }
+          }
+    }
+  }
+  /* write-out, normalisation and beta on the level-0 blocks (src/flip_sim.cpp:727-826) */
+  const float rhoTilde0 = (float)ppv;
+  const float etaPhi = logf(rhoL / rhoG);
+  const float rhoTildeMin = etaPhi * rhoG * rhoTilde0;
+  const float phiDenom = alphaPhi * rhoTilde0 * rhoL;
+  auto phiFromRho = [&](float rt) -> float {
+    if (rt < rhoTildeMin) return 0.f;
+    float v = sqrtf(std::max(rt - rhoTildeMin, 0.f) / phiDenom);
+    return std::min(v, 1.f);
+  };
+  auto betaFromPhi = [&](float phi) -> float {
+    float rho = rhoG + phi * (rhoL - rhoG);
+    return std::max(1.f / rho, betaMin);
+  };
+  auto cellPh = [&](int cx, int cy, int cz) -> int {
+    int64_t v2;
+    if (!G.has(cx, cy, cz, v2)) return -1;
+    if (mass[v2] <= massEps) return -1;
+    return (liq[v2] * 2 > (int)(mass[v2] + 0.5f)) ? 0 : 1;
+  };
+  const int nbx = nx / bsx, nby = ny / bsx, nbz = nz / bsx;
+  float *const outVel[3] = {outU, outV, outW};
+  float *const outBeta[3] = {outBU, outBV, outBW};
+  for (int64_t i = 0; i < n; i++) {
+    outMass[i] = 0.f;
+    for (int c = 0; c < 3; c++) outVel[c][i] = outBeta[c][i] = 0.f;
+  }
+  for (int bid = 0; bid < nbx * nby * nbz; bid++) {
+    outActive[bid] = 0;
+    if (blockLevel[bid] != 0) continue;
+    const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+    for (int lz = 0; lz < bsx; lz++)
+      for (int ly = 0; ly < bsx; ly++)
+        for (int lx = 0; lx < bsx; lx++) {
+          const int ix = bx * bsx + lx, iy = by * bsx + ly, iz = bz * bsx + lz;
+          const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+          float u = 0.f + vx[i], v = 0.f + vy[i], w = 0.f + vz[i];
+          if (wu[i] > massEps) u /= wu[i]; else u = 0.f;
+          if (wv[i] > massEps) v /= wv[i]; else v = 0.f;
+          if (ww[i] > massEps) w /= ww[i]; else w = 0.f;
+          outU[i] = u; outV[i] = v; outW[i] = w;
+          outMass[i] = 0.f + mass[i];
+          if (mass[i] > massEps) outActive[bid] = 1;
+          float phiU = phiFromRho(ru[i]), phiV = phiFromRho(rv[i]), phiW = phiFromRho(rw[i]);
+          { int pL = cellPh(ix - 1, iy, iz), pR = cellPh(ix, iy, iz);
+            if (pL == 0 && pR == 0) phiU = 1.f; else if (pL == 1 && pR == 1) phiU = 0.f; }
+          { int pB = cellPh(ix, iy - 1, iz), pT = cellPh(ix, iy, iz);
+            if (pB == 0 && pT == 0) phiV = 1.f; else if (pB == 1 && pT == 1) phiV = 0.f; }
+          { int pN = cellPh(ix, iy, iz - 1), pF = cellPh(ix, iy, iz);
+            if (pN == 0 && pF == 0) phiW = 1.f; else if (pN == 1 && pF == 1) phiW = 0.f; }
+          outBU[i] = betaFromPhi(phiU);
+          outBV[i] = betaFromPhi(phiV);
+          outBW[i] = betaFromPhi(phiW);
+        }
+  }
+}
+
+}  // extern "C"
+
+/* Particles of a scene's liquid cells (msbgx_scene_particle), cell-major,
+ * x fastest, phase 0 -- the order in which the GPU scene path generates them.
+ * Returns the count; writes when out is non-NULL. */
+#include "../include/msbgx_scene.h"
+extern "C" int64_t orc_scene_particles(const msbgx_scene *s, void *out)
+{
+  OrcParticle *P = (OrcParticle *)out;
+  int64_t k = 0;
+  for (int z = 0; z < s->nz; z++)
+    for (int y = 0; y < s->ny; y++)
+      for (int x = 0; x < s->nx; x++) {
+        if (!msbgx_scene_is_liquid(s, x, y, z)) continue;
+        for (int p = 0; p < s->particles_per_cell; p++, k++)
+          if (P) {
+            msbgx_scene_particle(s, x, y, z, p, P[k].pos, P[k].vel);
+            P[k].phase = 0;
+          }
+      }
+  return k;
+}

@@ -39,6 +39,8 @@ for name, res, args in [
     ("orc_export_level0", None, [P, C.c_int, P]),
     ("orc_solve_fp64", C.c_int, [P, C.c_double, C.c_int, P, C.POINTER(C.c_double)]),
     ("orc_flip_rhs", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, C.c_float, P]),
+    ("orc_scene_particles", C.c_int64, [C.POINTER(Scene), P]),
+    ("orc_p2g", None, [C.c_int] * 4 + [P, C.c_int64, P] + [C.c_float] * 5 + [C.c_int] + [P] * 8),
     ("orc_flip_update_velocity", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, P, P, P, C.c_float,
                                                         C.c_int]),
 ]:
@@ -195,4 +197,28 @@ def flip_update_velocity(block_level, bsx, mass, mass_eps, p, beta, vel, dt, neg
     out = [_f32(a).copy() for a in vel]
     _lib.orc_flip_update_velocity(nx, ny, nz, bsx, _p(lv), _p(m), mass_eps, _p(pp), *[_p(a) for a in b],
                                   *[_p(a) for a in out], dt, int(negate_p))
+    return out
+
+
+def p2g(block_level, nx, ny, nz, bsx, particles, rho_l=1000.0, rho_g=1.0, mass_eps=1e-5, beta_min=1e-6,
+        alpha_phi=1.0, ppv=8):
+    """particleToGrid (src/flip_sim.cpp:579-828) in the fixed cell-major order;
+    returns (dict of dense [z][y][x] outputs, per-block active flags)."""
+    from msbgx import PARTICLE_DTYPE
+    a = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+    lv = np.ascontiguousarray(block_level, np.int32)
+    names = ["mass", "u", "v", "w", "beta_u", "beta_v", "beta_w"]
+    out = {k: np.zeros((nz, ny, nx), np.float32) for k in names}
+    act = np.zeros(len(lv), np.uint8)
+    _lib.orc_p2g(nx, ny, nz, bsx, _p(lv), len(a), a.ctypes.data_as(C.c_void_p), rho_l, rho_g, mass_eps, beta_min,
+                 alpha_phi, ppv, *[_p(out[k]) for k in names], _p(act))
+    return out, act
+
+
+def scene_particles(scene):
+    """The scene's particles in the GPU scene path's order (host restatement)."""
+    from msbgx import PARTICLE_DTYPE
+    n = _lib.orc_scene_particles(C.byref(scene), None)
+    out = np.zeros(n, PARTICLE_DTYPE)
+    _lib.orc_scene_particles(C.byref(scene), out.ctypes.data_as(C.c_void_p))
     return out
