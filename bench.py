@@ -376,6 +376,44 @@ def flip_runs(msbgx, g, cfg, args, log):
     return out
 
 
+def compact_runs(msbgx, g, cfg, args, log):
+    """The FLIP caller's compact MIC(0)-PCG (the reference's default solver
+    kind) on the GPU as a comparator on the same system, to the same
+    tolerance, and the oracle's sequential MIC(0)-PCG on a bounded sample
+    (10 iterations, scaled to the GPU's count)."""
+    import numpy as np
+    g.set_precision(msbgx.PREC_FP32)
+    g.prepare_pressure_solve_flip(scene=cfg.scene)      # fresh right-hand side and channels
+    its, times = [], []
+    for _ in range(2):
+        it, rr = g.solve_pressure_compact(max_iter=args.max_iter, tol=args.tol)
+        its.append(it)
+        times.append(g.last_solve_ms())
+    out = {"solver": "MIC(0)-PCG (compact, hyperplane-wavefront triangular solves, one hipGraph per iteration)",
+           "iterations": its[-1], "rel_residual": rr, "gpu_ms": round(min(times), 2),
+           "ms_per_iteration": round(min(times) / max(its[-1], 1), 3)}
+    log(f"compact MIC(0)-PCG: {its[-1]} iterations, rel {rr:.2e}, {min(times):.1f} ms")
+    if not args.no_cpu:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+            flags, lv = dense_level0(msbgx, g, msbgx.CH_FLAGS, cfg.n, cfg.bsx)
+            mass = ((flags & (msbgx.CELL_SOLID | msbgx.CELL_VOID)) == 0).astype(np.float32)
+            beta = [dense_level0(msbgx, g, ch, cfg.n, cfg.bsx)[0]
+                    for ch in (msbgx.CH_FACE_U, msbgx.CH_FACE_V, msbgx.CH_FACE_W)]
+            div, _ = dense_level0(msbgx, g, msbgx.CH_DIVERGENCE, cfg.n, cfg.bsx)
+            k = 10
+            t0 = time.time()
+            _, ito, _, _, _ = oracle_lib.compact_pcg(lv, cfg.bsx, mass, 0.5, beta, div, tol=args.tol, max_iter=k)
+            t = time.time() - t0
+            out["cpu_oracle"] = {"sample_iterations": ito, "sample_s": round(t, 3), "threads": 1,
+                                 "estimated_s_for_gpu_iterations": round(t / max(ito, 1) * its[-1], 2)}
+            log(f"compact cpu oracle: {ito} iterations in {t:.2f} s (1 thread)")
+        except Exception as e:
+            log(f"compact cpu oracle failed: {e}")
+    return out
+
+
 def p2g_runs(msbgx, args, log):
     """Particle-to-grid transfer of the full-scale particle scene (dam break +
     sphere, 8 particles per liquid cell) on the GPU, and GPU vs the CPU oracle
@@ -534,6 +572,7 @@ def main():
     if not args.no_flip:
         line["flip_caller"] = flip_runs(msbgx, r["g"], cfg, args, log)
         line["particle_to_grid"] = p2g_runs(msbgx, args, log)
+        line["compact_mic0_pcg"] = compact_runs(msbgx, r["g"], cfg, args, log)
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:

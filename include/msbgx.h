@@ -368,6 +368,22 @@ int msbgx_p2g_last_times(const msbgx_p2g *t, double *ms_total, double *ms_gather
  * (same raster and level map). */
 int msbgx_p2g_to_velocity(msbgx_p2g *t, msbgx_grid *g);
 
+/* Compact MIC(0)-PCG (the FLIP caller's "paper baseline" solver kind,
+ * src/flip_sim.cpp:858-1151) on the GPU, as a comparator: the system over the
+ * fluid cells of the level-0 blocks in (z, y, x) order with diag = sum of the
+ * six face coefficients (0 at the domain boundary and towards cells without
+ * level-0 data), off-diagonals -beta between fluid cells, rhs = -div/dt (the
+ * divergence channel), MIC(0) with sigma = 0.25, PCG from x = 0 to
+ * |r| <= tol |b| (FP64 dots, float alpha/beta).  The solution (compact sign,
+ * A p = -div/dt) goes to the pressure channel, 0 elsewhere.  Returns the
+ * converged iteration, or the iterations run, and |r|/|b|;
+ * msbgx_last_solve_ms gives the device time. */
+int msbgx_solve_pressure_compact(msbgx_grid *g, int max_iter, float tol, int *iters, float *rel_residual);
+/* Reference: buildMIC0Compact + applyMIC0Compact (src/flip_sim.cpp:1026-1096)
+ * on the compact system: the factor and z = M^-1 rhs per row; returns the row
+ * count (cap = 0 queries it). */
+int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap);
+
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
  * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);

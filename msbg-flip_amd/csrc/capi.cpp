@@ -717,6 +717,25 @@ int msbgx_p2g_to_velocity(msbgx_p2g *t, msbgx_grid *g)
   return p2gToVelocity(*t->c, *g);
 }
 
+int msbgx_solve_pressure_compact(msbgx_grid *g, int max_iter, float tol, int *iters, float *rel_residual)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!iters || !rel_residual || max_iter < 0 || !(tol >= 0.0f)) { setError("bad argument"); return MSBGX_EINVAL; }
+  double ms = 0.0;
+  const int rc = deviceSolveCompact(*g, max_iter, tol, iters, rel_residual, &ms);
+  if (rc == MSBGX_OK) g->lastSolveMs = ms;
+  return rc;
+}
+
+int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (cap < 0 || (cap > 0 && (!precon || !z))) { setError("bad argument"); return MSBGX_EINVAL; }
+  int n = 0;
+  if (int rc = deviceCompactApply(*g, precon, z, cap, &n)) return rc;
+  return n;
+}
+
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)
 {
   const int rc = levelMapFromScene(s, bsx0, n_levels, mode, d0, out);

@@ -154,6 +154,9 @@ int p2gGet(P2gCtx &c, float *mass, float *const vel[3], float *const beta[3]);
 int p2gActiveBlocks(P2gCtx &c, int *out, int cap);
 void p2gTimes(const P2gCtx &c, double *msTotal, double *msGather, int64_t *particles);
 int p2gToVelocity(P2gCtx &c, Grid &g);
+/* compact MIC(0)-PCG comparator (compact.hip) */
+int deviceSolveCompact(Grid &g, int maxIter, float tol, int *iters, float *rel, double *ms);
+int deviceCompactApply(Grid &g, float *preconOut, float *zOut, int cap, int *nRows);
 int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst);
 int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst);
 int deviceProlongate(Grid &g, int Lf, int ch);

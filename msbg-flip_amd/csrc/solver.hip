@@ -2211,6 +2211,7 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
 #include "learned.hip"
 #include "flip.hip"
 #include "p2g.hip"
+#include "compact.hip"
 
 namespace msbgx {
 /* Captured graphs bake in launch shapes and kernel choices: drop them. */

@@ -223,6 +223,8 @@ def _load():
         "msbgx_p2g_active_blocks": (C.c_int, [P, P, C.c_int]),
         "msbgx_p2g_last_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         "msbgx_p2g_to_velocity": (C.c_int, [P, P]),
+        "msbgx_solve_pressure_compact": (C.c_int, [P, C.c_int, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+        "msbgx_compact_mic0": (C.c_int, [P, P, P, C.c_int]),
         "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
         "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
         "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
@@ -256,6 +258,7 @@ EXPORTED = [
     "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity", "msbgx_default_p2g_params", "msbgx_p2g_create",
     "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
     "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
+    "msbgx_solve_pressure_compact", "msbgx_compact_mic0",
 ]
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
@@ -374,6 +377,20 @@ class MultiresSparseGrid:
         applies it to -p (the MSBG path solves A p = +div/dt)."""
         mode = FLIP_SIGN_CORRECTED if corrected else FLIP_SIGN_REFERENCE
         _ck(lib.msbgx_flip_update_velocity(self._h, dt, mode), "flip_update_velocity")
+
+    def solve_pressure_compact(self, max_iter=1000, tol=1e-6):
+        """Compact MIC(0)-PCG comparator (src/flip_sim.cpp:858-1151); the
+        pressure channel receives its solution (compact sign)."""
+        it, rr = C.c_int(), C.c_float()
+        _ck(lib.msbgx_solve_pressure_compact(self._h, max_iter, tol, C.byref(it), C.byref(rr)), "solve_compact")
+        return it.value, rr.value
+
+    def compact_mic0(self):
+        """(MIC(0) factor, z = M^-1 rhs) per compact row ((z, y, x) order over the fluid cells)."""
+        n = _ck(lib.msbgx_compact_mic0(self._h, None, None, 0), "compact_mic0")
+        p, z = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
+        _ck(lib.msbgx_compact_mic0(self._h, _ptr(p), _ptr(z), n), "compact_mic0")
+        return p[:n], z[:n]
 
     def set_precision(self, prec):
         _ck(lib.msbgx_set_precision(self._h, prec), "set_precision")

@@ -305,3 +305,176 @@ extern "C" int64_t orc_scene_particles(const msbgx_scene *s, void *out)
       }
   return k;
 }
+
+/* ------------------------------------------------------------------------
+ * Compact MIC(0)-PCG (src/flip_sim.cpp:285-301, 858-1151), literally: rows
+ * = fluid cells with level-0 data in (z, y, x) order, CSR with ascending
+ * columns, MIC(0) with sigma = 0.25, PCG from x = 0, sequential FP64 dots.
+ * div holds div/dt per cell (the reference's sgDiv); the solution returns
+ * per cell in x (0 off the rows), the factor and z = M^-1 rhs in row order.
+ * ---------------------------------------------------------------------- */
+extern "C" int orc_compact_pcg(int nx, int ny, int nz, int bsx, const int *blockLevel, const float *mass,
+                               float massEps, const float *bu, const float *bv, const float *bw, const float *div,
+                               float tol, int maxIter, float *xOut, float *preconOut, float *zOut, int *itersOut,
+                               float *relOut)
+{
+  const Level0Grid G{nx, ny, nz, bsx, blockLevel};
+  const float *const sgBeta[3] = {bu, bv, bw};
+  auto lin = [=](int ix, int iy, int iz) { return (int64_t)ix + (int64_t)nx * (iy + (int64_t)ny * iz); };
+  const int64_t nCells = (int64_t)nx * ny * nz;
+  /* buildPressureSystem: rows */
+  std::vector<int> cellToRow(nCells, -1), rowIx, rowIy, rowIz;
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int ix = 0; ix < nx; ix++) {
+        if (!G.isFluid(mass, ix, iy, iz, massEps)) continue;
+        cellToRow[lin(ix, iy, iz)] = (int)rowIx.size();
+        rowIx.push_back(ix); rowIy.push_back(iy); rowIz.push_back(iz);
+      }
+  const int n = (int)rowIx.size();
+  /* rebuildPressurePattern */
+  enum { XM, XP, YM, YP, ZM, ZP };
+  std::vector<int> rowPtr(n + 1, 0), colInd, diagPos(n, -1), pos[6];
+  for (auto &v : pos) v.assign(n, -1);
+  std::vector<float> val;
+  for (int row = 0; row < n; row++) {
+    const int ix = rowIx[row], iy = rowIy[row], iz = rowIz[row];
+    struct Nbr { int col, dir; } nbrs[6];
+    int nNbr = 0;
+    auto addNbr = [&](int jx, int jy, int jz, int dir) {
+      if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) return;
+      const int col = cellToRow[lin(jx, jy, jz)];
+      if (col >= 0) nbrs[nNbr++] = {col, dir};
+    };
+    addNbr(ix - 1, iy, iz, XM); addNbr(ix + 1, iy, iz, XP);
+    addNbr(ix, iy - 1, iz, YM); addNbr(ix, iy + 1, iz, YP);
+    addNbr(ix, iy, iz - 1, ZM); addNbr(ix, iy, iz + 1, ZP);
+    for (int i = 0; i < nNbr - 1; i++)
+      for (int j = i + 1; j < nNbr; j++)
+        if (nbrs[j].col < nbrs[i].col) std::swap(nbrs[i], nbrs[j]);
+    bool diagIn = false;
+    for (int i = 0; i < nNbr; i++) {
+      if (!diagIn && nbrs[i].col > row) {
+        diagPos[row] = (int)colInd.size();
+        colInd.push_back(row); val.push_back(0.f);
+        diagIn = true;
+      }
+      pos[nbrs[i].dir][row] = (int)colInd.size();
+      colInd.push_back(nbrs[i].col); val.push_back(0.f);
+    }
+    if (!diagIn) {
+      diagPos[row] = (int)colInd.size();
+      colInd.push_back(row); val.push_back(0.f);
+    }
+    rowPtr[row + 1] = (int)colInd.size();
+  }
+  /* updatePressureNumerics */
+  std::vector<float> rhs(n), betaXm(n), betaYm(n), betaZm(n);
+  for (int row = 0; row < n; row++) {
+    const int ix = rowIx[row], iy = rowIy[row], iz = rowIz[row];
+    const float bxm = (ix > 0) ? G.getVC(sgBeta, ix, iy, iz, 0) : 0.f;
+    const float bxp = (ix + 1 < nx) ? G.getVC(sgBeta, ix + 1, iy, iz, 0) : 0.f;
+    const float bym = (iy > 0) ? G.getVC(sgBeta, ix, iy, iz, 1) : 0.f;
+    const float byp = (iy + 1 < ny) ? G.getVC(sgBeta, ix, iy + 1, iz, 1) : 0.f;
+    const float bzm = (iz > 0) ? G.getVC(sgBeta, ix, iy, iz, 2) : 0.f;
+    const float bzp = (iz + 1 < nz) ? G.getVC(sgBeta, ix, iy, iz + 1, 2) : 0.f;
+    rhs[row] = -G.getF(div, ix, iy, iz);
+    betaXm[row] = bxm; betaYm[row] = bym; betaZm[row] = bzm;
+    val[diagPos[row]] = bxm + bxp + bym + byp + bzm + bzp;
+    if (pos[XM][row] >= 0) val[pos[XM][row]] = -bxm;
+    if (pos[XP][row] >= 0) val[pos[XP][row]] = -bxp;
+    if (pos[YM][row] >= 0) val[pos[YM][row]] = -bym;
+    if (pos[YP][row] >= 0) val[pos[YP][row]] = -byp;
+    if (pos[ZM][row] >= 0) val[pos[ZM][row]] = -bzm;
+    if (pos[ZP][row] >= 0) val[pos[ZP][row]] = -bzp;
+  }
+  auto spmv = [&](const float *xv, float *q) {
+    for (int i = 0; i < n; i++) {
+      float sum = 0.f;
+      for (int j = rowPtr[i]; j < rowPtr[i + 1]; j++) sum += val[j] * xv[colInd[j]];
+      q[i] = sum;
+    }
+  };
+  auto dot = [&](const float *a, const float *b) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += (double)a[i] * (double)b[i];
+    return s;
+  };
+  /* buildMIC0Compact */
+  std::vector<float> precon(n, 0.f);
+  const float sigma = 0.25f;
+  for (int row = 0; row < n; row++) {
+    const float Adiag = val[diagPos[row]];
+    float e = Adiag;
+    const int ix = rowIx[row], iy = rowIy[row], iz = rowIz[row];
+    if (ix > 0) { const int r2 = cellToRow[lin(ix - 1, iy, iz)];
+                  if (r2 >= 0) { const float bx = betaXm[row], p = precon[r2]; e -= (bx * p) * (bx * p); } }
+    if (iy > 0) { const int r2 = cellToRow[lin(ix, iy - 1, iz)];
+                  if (r2 >= 0) { const float by = betaYm[row], p = precon[r2]; e -= (by * p) * (by * p); } }
+    if (iz > 0) { const int r2 = cellToRow[lin(ix, iy, iz - 1)];
+                  if (r2 >= 0) { const float bz = betaZm[row], p = precon[r2]; e -= (bz * p) * (bz * p); } }
+    precon[row] = 1.f / sqrtf(std::max(e, sigma * std::max(Adiag, 1e-10f)));
+  }
+  /* applyMIC0Compact */
+  auto apply = [&](const float *r, float *z) {
+    std::vector<float> q(n);
+    for (int row = 0; row < n; row++) {
+      const int ix = rowIx[row], iy = rowIy[row], iz = rowIz[row];
+      float t = r[row];
+      if (ix > 0) { const int r2 = cellToRow[lin(ix - 1, iy, iz)]; if (r2 >= 0) t += betaXm[row] * precon[r2] * q[r2]; }
+      if (iy > 0) { const int r2 = cellToRow[lin(ix, iy - 1, iz)]; if (r2 >= 0) t += betaYm[row] * precon[r2] * q[r2]; }
+      if (iz > 0) { const int r2 = cellToRow[lin(ix, iy, iz - 1)]; if (r2 >= 0) t += betaZm[row] * precon[r2] * q[r2]; }
+      q[row] = t * precon[row];
+    }
+    for (int row = n - 1; row >= 0; row--) {
+      const int ix = rowIx[row], iy = rowIy[row], iz = rowIz[row];
+      const float pi = precon[row];
+      float t = q[row];
+      if (ix + 1 < nx) { const int r2 = cellToRow[lin(ix + 1, iy, iz)]; if (r2 >= 0) t += betaXm[r2] * pi * z[r2]; }
+      if (iy + 1 < ny) { const int r2 = cellToRow[lin(ix, iy + 1, iz)]; if (r2 >= 0) t += betaYm[r2] * pi * z[r2]; }
+      if (iz + 1 < nz) { const int r2 = cellToRow[lin(ix, iy, iz + 1)]; if (r2 >= 0) t += betaZm[r2] * pi * z[r2]; }
+      z[row] = t * pi;
+    }
+  };
+  std::vector<float> x(n, 0.f), r(n), z(n), d(n), q(n);
+  for (int i = 0; i < n; i++) r[i] = rhs[i];
+  if (preconOut) std::copy(precon.begin(), precon.end(), preconOut);
+  if (zOut) {
+    std::vector<float> z0(n);
+    apply(rhs.data(), z0.data());
+    std::copy(z0.begin(), z0.end(), zOut);
+  }
+  /* solvePressureCompactPCG */
+  double rhsNormSq = 0.0;
+  for (int i = 0; i < n; i++) rhsNormSq += (double)r[i] * (double)r[i];
+  int iters = 0, converged = -1;
+  if (n > 0 && rhsNormSq > 1e-30) {
+    apply(r.data(), z.data());
+    double rho = dot(r.data(), z.data());
+    for (int i = 0; i < n; i++) d[i] = z[i];
+    const double tolSq = (double)tol * (double)tol;
+    for (int iter = 0; iter < maxIter; iter++) {
+      spmv(d.data(), q.data());
+      const double dq = dot(d.data(), q.data());
+      if (dq <= 1e-30 || rho <= 1e-30) break;
+      const float alpha = (float)(rho / dq);
+      for (int i = 0; i < n; i++) { x[i] += alpha * d[i]; r[i] -= alpha * q[i]; }
+      iters = iter + 1;
+      const double rNormSq = dot(r.data(), r.data());
+      if (rNormSq <= tolSq * rhsNormSq) { converged = iter + 1; break; }
+      apply(r.data(), z.data());
+      const double rhoNew = dot(r.data(), z.data());
+      if (rho <= 1e-30) break;
+      const float beta = (float)(rhoNew / rho);
+      for (int i = 0; i < n; i++) d[i] = z[i] + beta * d[i];
+      rho = rhoNew;
+    }
+  }
+  if (itersOut) *itersOut = converged > 0 ? converged : iters;
+  if (relOut) *relOut = rhsNormSq > 0 ? (float)std::sqrt(dot(r.data(), r.data()) / rhsNormSq) : 0.f;
+  if (xOut) {
+    std::fill(xOut, xOut + nCells, 0.f);
+    for (int row = 0; row < n; row++) xOut[lin(rowIx[row], rowIy[row], rowIz[row])] = x[row];
+  }
+  return n;
+}

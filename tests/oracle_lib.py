@@ -40,6 +40,8 @@ for name, res, args in [
     ("orc_solve_fp64", C.c_int, [P, C.c_double, C.c_int, P, C.POINTER(C.c_double)]),
     ("orc_flip_rhs", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, C.c_float, P]),
     ("orc_scene_particles", C.c_int64, [C.POINTER(Scene), P]),
+    ("orc_compact_pcg", C.c_int, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, C.c_float, C.c_int, P, P, P,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("orc_p2g", None, [C.c_int] * 4 + [P, C.c_int64, P] + [C.c_float] * 5 + [C.c_int] + [P] * 8),
     ("orc_flip_update_velocity", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, P, P, P, C.c_float,
                                                         C.c_int]),
@@ -178,10 +180,17 @@ def _f32(a):
     return np.ascontiguousarray(a, np.float32)
 
 
+def _levels(block_level, nx, ny, nz, bsx):
+    lv = np.ascontiguousarray(block_level, np.int32)
+    if lv.size != (nx // bsx) * (ny // bsx) * (nz // bsx):
+        raise ValueError("one level per block of the raster expected")
+    return lv
+
+
 def flip_rhs(block_level, bsx, mass, mass_eps, vel, dt):
     """flipRhsCB on every cell with level-0 data (arrays [z][y][x])."""
     nz, ny, nx = mass.shape
-    lv = np.ascontiguousarray(block_level, np.int32)
+    lv = _levels(block_level, nx, ny, nz, bsx)
     m, u, v, w = (_f32(a) for a in (mass, *vel))
     out = np.zeros_like(m)
     _lib.orc_flip_rhs(nx, ny, nz, bsx, _p(lv), _p(m), mass_eps, _p(u), _p(v), _p(w), dt, _p(out))
@@ -191,7 +200,7 @@ def flip_rhs(block_level, bsx, mass, mass_eps, vel, dt):
 def flip_update_velocity(block_level, bsx, mass, mass_eps, p, beta, vel, dt, negate_p):
     """Pressure-gradient update of src/flip_sim.cpp:1317-1349; returns new (u, v, w)."""
     nz, ny, nx = mass.shape
-    lv = np.ascontiguousarray(block_level, np.int32)
+    lv = _levels(block_level, nx, ny, nz, bsx)
     m, pp = _f32(mass), _f32(p)
     b = [_f32(a) for a in beta]
     out = [_f32(a).copy() for a in vel]
@@ -206,7 +215,7 @@ def p2g(block_level, nx, ny, nz, bsx, particles, rho_l=1000.0, rho_g=1.0, mass_e
     returns (dict of dense [z][y][x] outputs, per-block active flags)."""
     from msbgx import PARTICLE_DTYPE
     a = np.ascontiguousarray(particles, PARTICLE_DTYPE)
-    lv = np.ascontiguousarray(block_level, np.int32)
+    lv = _levels(block_level, nx, ny, nz, bsx)
     names = ["mass", "u", "v", "w", "beta_u", "beta_v", "beta_w"]
     out = {k: np.zeros((nz, ny, nx), np.float32) for k in names}
     act = np.zeros(len(lv), np.uint8)
@@ -222,3 +231,23 @@ def scene_particles(scene):
     out = np.zeros(n, PARTICLE_DTYPE)
     _lib.orc_scene_particles(C.byref(scene), out.ctypes.data_as(C.c_void_p))
     return out
+
+
+def compact_pcg(block_level, bsx, mass, mass_eps, beta, div, tol=1e-6, max_iter=1000, want_factor=False):
+    """Compact MIC(0)-PCG (src/flip_sim.cpp:858-1151): (x dense [z][y][x],
+    iterations, |r|/|b|, precon rows or None, M^-1 rhs rows or None)."""
+    nz, ny, nx = mass.shape
+    lv = _levels(block_level, nx, ny, nz, bsx)
+    m, d = _f32(mass), _f32(div)
+    b = [_f32(a) for a in beta]
+    x = np.zeros_like(m)
+    nrows = int(np.count_nonzero(m > mass_eps))
+    pre = np.zeros(max(nrows, 1), np.float32) if want_factor else None
+    z = np.zeros(max(nrows, 1), np.float32) if want_factor else None
+    it, rr = C.c_int(), C.c_float()
+    n = _lib.orc_compact_pcg(nx, ny, nz, bsx, _p(lv), _p(m), mass_eps, *[_p(a) for a in b], _p(d), tol, max_iter,
+                             _p(x), _p(pre) if want_factor else None, _p(z) if want_factor else None,
+                             C.byref(it), C.byref(rr))
+    if want_factor:
+        pre, z = pre[:n], z[:n]
+    return x, it.value, rr.value, pre, z
