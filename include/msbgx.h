@@ -379,7 +379,7 @@ int msbgx_p2g_to_velocity(msbgx_p2g *t, msbgx_grid *g);
  * converged iteration, or the iterations run, and |r|/|b|;
  * msbgx_last_solve_ms gives the device time. */
 int msbgx_solve_pressure_compact(msbgx_grid *g, int max_iter, float tol, int *iters, float *rel_residual);
-/* Reference: buildMIC0Compact + applyMIC0Compact (src/flip_sim.cpp:1026-1096)
+/* Reference: buildMIC0Compact + applyMIC0Compact (src/flip_sim.cpp:1027-1097)
  * on the compact system: the factor and z = M^-1 rhs per row; returns the row
  * count (cap = 0 queries it). */
 int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap);

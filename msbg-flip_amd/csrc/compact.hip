@@ -3,13 +3,13 @@
  * GPU (compiled as part of solver.hip), the reference's "paper baseline"
  * solver kind, as a comparator for the MSBG MG-PCG.
  *
- * Reference (src/flip_sim.cpp): buildPressureSystem 966-1011 (rows = fluid
+ * Reference (src/flip_sim.cpp): buildPressureSystem 953-1011 (rows = fluid
  * cells of the resolution-level-0 grid in (z, y, x) order), rebuildPressurePattern
- * 858-913 (CSR, columns ascending), updatePressureNumerics 915-950 (diag =
+ * 858-913 (CSR, columns ascending), updatePressureNumerics 915-951 (diag =
  * bxm + bxp + bym + byp + bzm + bzp, off-diagonals -beta, rhs = -div/dt),
- * buildMIC0Compact 1026-1057, applyMIC0Compact 1059-1096,
- * solvePressureCompactPCG 1100-1151, csrSpmv 285-294, compactDot 296-301,
- * scatterPressureToGrid 1013-1022.
+ * buildMIC0Compact 1027-1059, applyMIC0Compact 1061-1097,
+ * solvePressureCompactPCG 1101-1151, csrSpmv 285-294, compactDot 296-301,
+ * scatterPressureToGrid 1013-1023.
  *
  * The system is taken from a prepared grid: fluid = level-0 cell type with
  * neither SOLID nor VOID, beta = level-0 face coefficient (0 where the cell
