@@ -185,13 +185,16 @@ __global__ void k_cpt_dir(int n, const float *__restrict__ ab, const float *__re
 /* The scalar steps of one iteration (single workgroup, partials summed in
  * order).  stage 0: rho = r.z (start); 1: dq -> alpha or stop; 2: rNormSq ->
  * convergence; 3: rhoNew -> beta. */
-__global__ void k_cpt_scalar(int stage, const double *__restrict__ part, int np, double *__restrict__ sc,
-                             float *__restrict__ ab, int *__restrict__ flags, int maxIter, double tolSq)
+__global__ void __launch_bounds__(64) k_cpt_scalar(int stage, const double *__restrict__ part, int np,
+                                                   double *__restrict__ sc, float *__restrict__ ab,
+                                                   int *__restrict__ flags, int maxIter, double tolSq)
 {
-  if (threadIdx.x != 0) return;
   if (stage != 0 && flags[0]) return;
+  /* one wave: strided partial sums, then a fixed shuffle tree */
   double s = 0.0;
-  for (int i = 0; i < np; i++) s += part[i];
+  for (int i = threadIdx.x; i < np; i += 64) s += part[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (threadIdx.x != 0) return;
   switch (stage) {
   case 0:
     sc[1] = s;
