@@ -57,6 +57,10 @@ struct LevelArgs {
   const int *nbr;        // 6 face neighbours per block, -1 = domain boundary; null: raster arithmetic
   const int *probOf;     // problem of each block
   const float *omegaOf;  // smoother weight of each problem
+  /* MG0 with 32^3 level-0 blocks: per-cell coefficient records of the column
+   * smoother, two float4 per cell {D, W-x, W+x, W-y}, {W+y, W-z, W+z, 0}, at
+   * record off[bid] + slot * 1024 + (x + 32 y), slot = (z + x/8 + y) mod 32 */
+  const float4 *pack;
 };
 
 __device__ __forceinline__ float fromBits16(uint16_t v, int prec)
@@ -329,7 +333,8 @@ __device__ __forceinline__ void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0
  *
  * Thread (x, y) owns the z-column of its cell row.  The staged h*x values of
  * the column live in registers, and for B <= 16 so do all seven coefficients
- * (REGC); for B = 32 the coefficients are re-read from memory each step.  Cell
+ * (REGC); for B = 32 they are read each step from per-cell records laid
+ * out by register slot (LevelArgs::pack), one contiguous plane per step.  Cell
  * z sits in register slot (z + o) mod B with o = x/S + y, so inside a loop
  * unrolled over one period of B hyperplanes the slot a thread touches at
  * hyperplane hp is the compile-time constant hp mod B.  The four sequential
@@ -352,6 +357,8 @@ struct ColCtx {
   const float *Bv;
   float *Xg;
   const float *faces;
+  const float4 *pack;
+  const float4 *packBase;   // the thread's record in plane 0 of its block
   int64_t off;
   int tx, ty, o, fk1, fk3, fk5, mixed0, mixed2, mixed4;
   int64_t fo1, fo3, fo5;
@@ -417,7 +424,21 @@ __device__ __forceinline__ void colStep(ColRegs<B, REGC> &r, uint32_t skip, floa
   const int z = hp - c.o;
   const bool active = z >= 0 && z < B && !((skip >> sl) & 1u);
   float D = 0.f, Bb = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f, w6 = 0.f;
-  if (!REGC && active) loadCoeffs<B>(c, z, D, Bb, w1, w2, w3, w4, w5, w6);
+  if constexpr (!REGC) {
+    if (active) {
+      /* coefficient records (B = 32): every thread of the workgroup reads
+       * record plane sl, one contiguous stream per step */
+      /* byte offset of plane sl from the thread's base record, made opaque so
+       * the compiler forms it here instead of keeping 32 hoisted addresses */
+      uint32_t o32 = (uint32_t)sl * (uint32_t)(Cfg<B>::NT * 2 * sizeof(float4));
+      asm volatile("" : "+v"(o32));
+      const float4 *rp = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(c.packBase) + o32);
+      const float4 r0 = rp[0], r1 = rp[1];
+      D = r0.x; w1 = r0.y; w2 = r0.z; w3 = r0.w;
+      w4 = r1.x; w5 = r1.y; w6 = r1.z;
+      Bb = c.Bv[c.off + tx + B * ty + B * B * z];
+    }
+  }
   float *pA = pub[0], *pB = pub[1], *pC = pub[2];
   pA[tid] = r.xs[sl];
   pB[tid] = r.xs[slA];
@@ -550,6 +571,8 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
 
   ColCtx c;
   c.a = &a; c.Bv = Bv; c.Xg = X; c.faces = faces; c.off = off;
+  c.pack = REGC ? nullptr : a.pack;
+  c.packBase = REGC ? nullptr : a.pack + 2 * (off + (int64_t)threadIdx.x);
   c.tx = threadIdx.x % B; c.ty = threadIdx.x / B; c.o = c.tx / S + c.ty;
   c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
   c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
@@ -592,6 +615,39 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
     }
     if constexpr (REGC) colJacobi<B>(r, skip, colz, c);
     else colSweep<B, REGC, -1, false>(r, skip, pub, c);
+  }
+}
+
+/* Coefficient records of the 32^3 column smoother (see LevelArgs::pack): the
+ * seven values loadCoeffs returns for each cell, in the operand precision of
+ * the level, stored at the record of the register slot the smoother visits
+ * at that cell's hyperplane step. */
+__global__ void __launch_bounds__(1024) k_pack_col32(LevelArgs a, const int *__restrict__ blocks,
+                                                     const float *__restrict__ X, float4 *__restrict__ pack)
+{
+  constexpr int B = 32, S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
+  __shared__ float faces[6 * NF];
+  const int bid = blocks[blockIdx.x];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  int fkind[6];
+  int64_t foff[6];
+  stageFaces<B>(faces, a, bid, lev, 1.0f, off, X, fkind, foff);
+  ColCtx c;
+  c.a = &a; c.Bv = X; c.Xg = nullptr; c.faces = faces; c.off = off; c.pack = nullptr;
+  c.tx = threadIdx.x % B; c.ty = threadIdx.x / B; c.o = c.tx / S + c.ty;
+  c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
+  c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
+  c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
+  c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  c.om = 0.f; c.omm = 0.f; c.HR = 1.f;
+  for (int z = 0; z < B; z++) {
+    float D, Bb, w1, w2, w3, w4, w5, w6;
+    loadCoeffs<B>(c, z, D, Bb, w1, w2, w3, w4, w5, w6);
+    const int sl = (z + c.o) & (B - 1);
+    float4 *r = pack + 2 * (off + (int64_t)sl * NT + threadIdx.x);
+    r[0] = make_float4(D, w1, w2, w3);
+    r[1] = make_float4(w4, w5, w6, 0.0f);
   }
 }
 
@@ -982,6 +1038,8 @@ struct Dev {
   float *omegaOf = nullptr;
   BatchDev *batch = nullptr;
   LearnedCtx *learned = nullptr;   // buffers and graph of the learned-correction solve
+  /* 32^3 column smoother: coefficient records over the MG0 pool (LevelArgs::pack) */
+  float4 *pack32 = nullptr;
   /* FLIP caller steps: velocity channels (MG0 pool layout) and the level-0 block list */
   float *vel[3] = {nullptr, nullptr, nullptr};
   int *level0 = nullptr;
@@ -1039,6 +1097,7 @@ void deviceFree(Grid &g)
     f(d.nbr); f(d.probOf); f(d.omegaOf);
     freeLearned(d.learned);
     freeFlip(d);
+    if (d.pack32) (void)hipFree(d.pack32);
     if (BatchDev *b = d.batch) {
       if (b->graph) (void)hipGraphExecDestroy(b->graph);
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
@@ -1067,6 +1126,7 @@ static LevelArgs levelArgs(const Grid &g, int L)
   a.nbr = dd ? dd->nbr : nullptr;
   a.probOf = dd ? dd->probOf : nullptr;
   a.omegaOf = dd ? dd->omegaOf : nullptr;
+  a.pack = (L == 0 && g.bsx0 == 32 && dd) ? dd->pack32 : nullptr;
   return a;
 }
 
@@ -1104,6 +1164,33 @@ static int uploadLists(Grid &g, int L, const std::vector<char> *live)
   for (int l = 0; l < kMaxLevels; l++, si++) m.dAct[l] = DevList{m.d_lists + spans[si].first, spans[si].second};
   for (int c = 0; c < 2; c++)
     for (int l = 0; l < kMaxLevels; l++, si++) m.dRelax[c][l] = DevList{m.d_lists + spans[si].first, spans[si].second};
+  return MSBGX_OK;
+}
+
+/* Coefficient records of the 32^3 column smoother over the MG0 pool,
+ * rebuilt whenever the coefficients, their operand precision or the block
+ * neighbourhood (batched block sets) change. */
+static int buildPack32(Grid &g)
+{
+  if (g.bsx0 != 32 || !g.dev) return MSBGX_OK;
+  Dev &d = dev(g);
+  MgLevel &m = g.mg[0];
+  if (!d.pack32)
+    if (int rc = dalloc(&d.pack32, (size_t)m.ncells * 2)) return rc;
+  std::vector<int> l0;
+  for (int bid = 0; bid < g.nBlocks; bid++)
+    if (m.level[bid] == 0) l0.push_back(bid);
+  if (l0.empty()) return MSBGX_OK;
+  int *dl = nullptr;
+  if (int rc = dalloc(&dl, l0.size())) return rc;
+  HIPCHK(hipMemcpy(dl, l0.data(), l0.size() * sizeof(int), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_pack_col32, dim3((unsigned)l0.size()), dim3(1024), 0, d.stream, levelArgs(g, 0), dl,
+                     m.d_ch[C_X], d.pack32);
+  const hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(d.stream);
+  (void)hipFree(dl);
+  HIPCHK(e);
+  HIPCHK(e2);
   return MSBGX_OK;
 }
 
@@ -1175,7 +1262,8 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
     HIPCHK(hipMemcpyAsync(m0.d_ch[C_B], rhs.data(), rhs.size() * sizeof(float), hipMemcpyHostToDevice, d.stream));
   }
   HIPCHK(hipStreamSynchronize(d.stream));
-  return MSBGX_OK;
+  if (d.pack32) { (void)hipFree(d.pack32); d.pack32 = nullptr; }   // the pool layout may have changed
+  return buildPack32(g);
 }
 
 /* ---------------------------------------------------------- launch helpers */
@@ -1186,6 +1274,9 @@ static void launchRelaxB(const Grid &g, int L, const DevList &l, float *X, const
 {
   if (l.n == 0) return;
   const LevelArgs a = levelArgs(g, L);
+  if constexpr (B == 32) {
+    if (!a.pack) return;   // records are built at upload for every 32^3 MG0; never reached without them
+  }
   if constexpr (B <= 16) {
     /* LDS-tile variant (FP32 operands only); the autotuner may pick it per block size */
     if (g.smootherVariant[__builtin_ctz(g.bsx0 / B)] == 1 && a.prec == MSBGX_PREC_FP32) {
@@ -1655,6 +1746,7 @@ int deviceSetPrecision(Grid &g, int prec)
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d.stream));
+  if (int rc = buildPack32(g)) return rc;   // records hold the operands in the new precision
   deviceInvalidateGraphs(g);   // kernel arguments changed
   return MSBGX_OK;
 }
@@ -1858,6 +1950,7 @@ int deviceUploadBatch(Grid &m, const std::vector<const Grid *> &probs)
   HIPCHK(hipMemcpy(d.nbr, m.nbrTable.data(), (size_t)nb * 6 * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.probOf, m.probOf.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.omegaOf, m.probOmega.data(), m.nProb * sizeof(float), hipMemcpyHostToDevice));
+  if (int rc = buildPack32(m)) return rc;   // again, with the explicit neighbour table
   /* right-hand sides: scene problems on the device, others from their host copy */
   MgLevel &m0 = m.mg[0];
   std::vector<float> hostB;
