@@ -28,8 +28,12 @@
  * face, the 18 cells whose particles can reach it in (z, y, x) order, each
  * cell's particles in index order.  The oracle sums in the same order.
  *
- * Memory: particle array 28 B and key + index 8 B per particle; per bin
- * one count and one start (8 B; 8.6 GB at 1024^3); outputs per cell of
+ * k_p2g_permute then copies the particles in bin order into SoA arrays and
+ * k_p2g_gather makes one pass over the 27 cells around each cell, adding each
+ * particle to all three faces it reaches.
+ *
+ * Memory: particle array 28 B, key + index 8 B and the bin-ordered copy 25 B
+ * per particle; per bin one count and one start (8 B; 8.6 GB at 1024^3); outputs per cell of
  * the level-0 blocks: mass, liquid count, three velocities, three betas.
  */
 
@@ -45,6 +49,8 @@ struct P2gCtx {
   int64_t capP = 0;
   msbgx_particle *dPart = nullptr;
   uint32_t *dKey = nullptr, *dIdx = nullptr;
+  float *dS[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // bin-ordered x, y, z, u, v, w
+  uint8_t *dPh = nullptr;                                                  // bin-ordered phase
   uint32_t *dCount = nullptr, *dStart = nullptr, *dTile = nullptr;
   int64_t nCells = 0, nTiles = 0;
   int64_t nOut = 0;          // level0.size() * bsx^3
@@ -181,9 +187,25 @@ __global__ void k_p2g_sort_cells(const uint32_t *__restrict__ start, int64_t nCe
   }
 }
 
+/* Particles in bin order as SoA arrays (position, velocity, phase), so a
+ * bin's particles are contiguous for the gather. */
+__global__ void k_p2g_permute(const msbgx_particle *__restrict__ P, const uint32_t *__restrict__ idx, int64_t total,
+                              float *__restrict__ sx, float *__restrict__ sy, float *__restrict__ sz,
+                              float *__restrict__ su, float *__restrict__ sv, float *__restrict__ sw,
+                              uint8_t *__restrict__ sph)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const msbgx_particle p = P[idx[k]];
+  sx[k] = p.pos[0]; sy[k] = p.pos[1]; sz[k] = p.pos[2];
+  su[k] = p.vel[0]; sv[k] = p.vel[1]; sw[k] = p.vel[2];
+  sph[k] = (uint8_t)(p.phase == 0 ? 0 : 1);
+}
+
 struct P2gArgs {
-  const msbgx_particle *P;
-  const uint32_t *start, *idx;
+  const float *sp[6];       // bin-ordered positions and velocities
+  const uint8_t *sph;       // bin-ordered phases
+  const uint32_t *start;
   const int *slot;
   int nx, ny, nz, nbx, nby, lb;
   float rhoL, rhoG, massEps, betaMin, rhoTildeMin, phiDenom;
@@ -202,7 +224,7 @@ __device__ __forceinline__ int p2gCellPhase(const P2gArgs &a, int x, int y, int 
   int liq = 0;
   for (uint32_t k = s; k < e; k++) {
     mass += 1.0f;
-    if (a.P[a.idx[k]].phase == 0) liq++;
+    if (a.sph[k] == 0) liq++;
   }
   if (!(mass > a.massEps)) return -1;
   return liq * 2 > (int)(mass + 0.5f) ? 0 : 1;
@@ -244,49 +266,64 @@ __global__ void __launch_bounds__(256) k_p2g_gather(P2gArgs a, const int *__rest
       int liq = 0;
       for (uint32_t k = s; k < e; k++) {
         m += 1.0f;
-        if (a.P[a.idx[k]].phase == 0) liq++;
+        if (a.sph[k] == 0) liq++;
       }
       mass[o + v] = m;
       liqOut[o + v] = liq;
       if (m > a.massEps) any = 1;
     }
     const int phC = p2gCellPhase(a, f[0], f[1], f[2]);
-#pragma unroll 1
-    for (int dir = 0; dir < 3; dir++) {
-      float sv = 0.0f, sw = 0.0f, sr = 0.0f;
-      int lo[3], hi[3];
-      for (int d = 0; d < 3; d++) {
-        lo[d] = f[d] - 1;
-        hi[d] = d == dir ? f[d] : f[d] + 1;
-      }
-      for (int cz = lo[2]; cz <= hi[2]; cz++)
-        for (int cy = lo[1]; cy <= hi[1]; cy++)
-          for (int cx = lo[0]; cx <= hi[0]; cx++) {
-            const int64_t c = p2gBin(cx, cy, cz, a.nx, a.ny);   // lo >= -1, hi <= n: inside the bin raster
-            const uint32_t s = a.start[c], e = a.start[c + 1];
-            for (uint32_t k = s; k < e; k++) {
-              const msbgx_particle &p = a.P[a.idx[k]];
+    /* one pass over the 27 cells around the cell in (z, y, x) order: each
+     * particle adds to every face whose stencil it reaches, so each face's
+     * sums run over its own 18 source cells in (z, y, x) order, then by
+     * particle, as three separate 18-cell passes would */
+    float sv[3] = {0.0f, 0.0f, 0.0f}, sw[3] = {0.0f, 0.0f, 0.0f}, sr[3] = {0.0f, 0.0f, 0.0f};
+    for (int cz = f[2] - 1; cz <= f[2] + 1; cz++)
+      for (int cy = f[1] - 1; cy <= f[1] + 1; cy++)
+        for (int cx = f[0] - 1; cx <= f[0] + 1; cx++) {
+          const int64_t c = p2gBin(cx, cy, cz, a.nx, a.ny);   // f - 1 >= -1, f + 1 <= n: inside the bin raster
+          const uint32_t s = a.start[c], e = a.start[c + 1];
+          for (uint32_t k = s; k < e; k++) {
+            const float pos[3] = {a.sp[0][k], a.sp[1][k], a.sp[2][k]};
+            /* per axis: the node weights on the grid whose nodes sit on the
+             * axis (the face's own axis) and on the half-shifted grids */
+            float frOn[3], frOff[3];
+            int ddOn[3], ddOff[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              const int i0 = (int)floorf(pos[d]);
+              frOn[d] = pos[d] - (float)i0;
+              ddOn[d] = f[d] - i0;
+              const float g1 = pos[d] - 0.5f;
+              const int i1 = (int)floorf(g1);
+              frOff[d] = g1 - (float)i1;
+              ddOff[d] = f[d] - i1;
+            }
+            const float mp = a.sph[k] == 0 ? a.rhoL : a.rhoG;
+#pragma unroll
+            for (int dir = 0; dir < 3; dir++) {
               float fr[3];
               int dd[3], ok = 1;
+#pragma unroll
               for (int d = 0; d < 3; d++) {
-                const float gq = d == dir ? p.pos[d] : p.pos[d] - 0.5f;
-                const int i0 = (int)floorf(gq);
-                fr[d] = gq - (float)i0;
-                dd[d] = f[d] - i0;
+                fr[d] = d == dir ? frOn[d] : frOff[d];
+                dd[d] = d == dir ? ddOn[d] : ddOff[d];
                 if (dd[d] < 0 || dd[d] > 1) ok = 0;
               }
               if (!ok) continue;
               const float w = ((dd[0] ? fr[0] : 1.0f - fr[0]) * (dd[1] ? fr[1] : 1.0f - fr[1])) *
                               (dd[2] ? fr[2] : 1.0f - fr[2]);
-              const float mp = p.phase == 0 ? a.rhoL : a.rhoG;
-              sv = sv + p.vel[dir] * w;
-              sw = sw + w;
-              sr = sr + mp * w;
+              sv[dir] = sv[dir] + a.sp[3 + dir][k] * w;
+              sw[dir] = sw[dir] + w;
+              sr[dir] = sr[dir] + mp * w;
             }
           }
-      vel[dir][o + v] = sw > a.massEps ? sv / sw : 0.0f;
+        }
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+      vel[dir][o + v] = sw[dir] > a.massEps ? sv[dir] / sw[dir] : 0.0f;
       const int phL = p2gCellPhase(a, f[0] - (dir == 0), f[1] - (dir == 1), f[2] - (dir == 2));
-      bet[dir][o + v] = p2gBeta(a, sr, phL, phC);
+      bet[dir][o + v] = p2gBeta(a, sr[dir], phL, phC);
     }
   }
   any = __syncthreads_or(any);
@@ -342,6 +379,8 @@ void p2gDestroy(P2gCtx *c)
   if (!c) return;
   auto f = [](void *p) { if (p) (void)hipFree(p); };
   f(c->dLevel0); f(c->dSlot); f(c->dPart); f(c->dKey); f(c->dIdx); f(c->dCount); f(c->dStart); f(c->dTile);
+  f(c->dPh);
+  for (int k = 0; k < 6; k++) f(c->dS[k]);
   f(c->dMass); f(c->dLiq); f(c->dActive);
   for (int k = 0; k < 3; k++) { f(c->dVel[k]); f(c->dBeta[k]); }
   for (int k = 0; k < 3; k++)
@@ -395,11 +434,14 @@ static int p2gReserve(P2gCtx &c, int64_t n)
 {
   if (n <= c.capP) return MSBGX_OK;
   auto f = [](void *p) { if (p) (void)hipFree(p); };
-  f(c.dPart); f(c.dKey); f(c.dIdx);
-  c.dPart = nullptr; c.dKey = nullptr; c.dIdx = nullptr; c.capP = 0;
+  f(c.dPart); f(c.dKey); f(c.dIdx); f(c.dPh);
+  for (int k = 0; k < 6; k++) { f(c.dS[k]); c.dS[k] = nullptr; }
+  c.dPart = nullptr; c.dKey = nullptr; c.dIdx = nullptr; c.dPh = nullptr; c.capP = 0;
   P2GCHK(hipMalloc((void **)&c.dPart, (size_t)n * sizeof(msbgx_particle)));
   P2GCHK(hipMalloc((void **)&c.dKey, (size_t)n * sizeof(uint32_t)));
   P2GCHK(hipMalloc((void **)&c.dIdx, (size_t)n * sizeof(uint32_t)));
+  for (int k = 0; k < 6; k++) P2GCHK(hipMalloc((void **)&c.dS[k], (size_t)n * sizeof(float)));
+  P2GCHK(hipMalloc((void **)&c.dPh, (size_t)n));
   c.capP = n;
   return MSBGX_OK;
 }
@@ -424,10 +466,18 @@ static int p2gRunDevice(P2gCtx &c, int64_t n, const msbgx_p2g_params &p)
   if (n > 0) hipLaunchKernelGGL(k_p2g_place, dim3((unsigned)gP), dim3(256), 0, s, c.dKey, n, c.dStart, c.dCount, c.dIdx);
   hipLaunchKernelGGL(k_p2g_sort_cells, dim3((unsigned)((c.nCells + 255) / 256)), dim3(256), 0, s, c.dStart, c.nCells,
                      c.dIdx);
+  /* binned particles: every particle except those beyond the halo, counted by the scan */
+  uint32_t total = 0;
+  P2GCHK(hipMemcpyAsync(&total, c.dStart + c.nCells, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  P2GCHK(hipStreamSynchronize(s));
+  if (total > 0)
+    hipLaunchKernelGGL(k_p2g_permute, dim3((unsigned)(((int64_t)total + 255) / 256)), dim3(256), 0, s, c.dPart, c.dIdx,
+                       (int64_t)total, c.dS[0], c.dS[1], c.dS[2], c.dS[3], c.dS[4], c.dS[5], c.dPh);
   P2GCHK(hipGetLastError());
   P2GCHK(hipEventRecord(c.ev[1], s));
   P2gArgs a;
-  a.P = c.dPart; a.start = c.dStart; a.idx = c.dIdx; a.slot = c.dSlot;
+  for (int k = 0; k < 6; k++) a.sp[k] = c.dS[k];
+  a.sph = c.dPh; a.start = c.dStart; a.slot = c.dSlot;
   a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nbx = c.nbx; a.nby = c.nby; a.lb = c.lb;
   a.rhoL = p.rho_l; a.rhoG = p.rho_g; a.massEps = p.mass_eps; a.betaMin = p.beta_min;
   /* the reference's phi parameters (src/flip_sim.cpp:744-748), same float operations */
