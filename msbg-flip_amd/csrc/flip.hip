@@ -163,21 +163,35 @@ __global__ void __launch_bounds__(256) k_flip_update(FlipArgs a, const int *__re
   for (int v = threadIdx.x; v < n; v += blockDim.x) {
     const int c[3] = {v & (B - 1), (v >> a.lb) & (B - 1), v >> (2 * a.lb)};
     const bool fR = flipFluid(a.flags[o + v]);
-    const float pOwn = fR ? (negate ? -P[o + v] : P[o + v]) : 0.0f;   // a non-fluid side reads +0
+    /* every load of the cell is issued up front (addresses do not depend on
+     * loaded values; a face without a -side cell reads the cell itself and
+     * discards it), so a cell waits for one memory latency, not a chain */
+    const float pO = P[o + v];
+    int64_t q[3];
+    bool has[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      float nv = 0.0f;
-      if (cb[d] > 0 || c[d] > 0) {
-        const int64_t q = flipNeighbour(o, v, c[d], d, -1, B, strides[d], nbo);
-        const bool fL = q >= 0 && flipFluid(a.flags[q]);
-        if (fL || fR) {
-          const float pL = fL ? (negate ? -P[q] : P[q]) : 0.0f, pR = pOwn;
-          const float bt = beta[d][o + v];
-          const float t = vel[d][o + v] - dt * bt * (pR - pL);
-          nv = isfinite(t) ? t : 0.0f;
-        }
-      }
-      vel[d][o + v] = nv;
+      const int64_t qq = (cb[d] > 0 || c[d] > 0) ? flipNeighbour(o, v, c[d], d, -1, B, strides[d], nbo) : -1;
+      has[d] = qq >= 0;
+      q[d] = has[d] ? qq : o + v;
+    }
+    uint16_t fl[3];
+    float pq[3], bt[3], vv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      fl[d] = a.flags[q[d]];
+      pq[d] = P[q[d]];
+      bt[d] = beta[d][o + v];
+      vv[d] = vel[d][o + v];
+    }
+    const float pOwn = fR ? (negate ? -pO : pO) : 0.0f;   // a non-fluid side reads +0
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const bool fL = has[d] && flipFluid(fl[d]);
+      const float pL = fL ? (negate ? -pq[d] : pq[d]) : 0.0f;
+      const float t = vv[d] - dt * bt[d] * (pOwn - pL);
+      const bool face = (cb[d] > 0 || c[d] > 0) && (fL || fR);
+      vel[d][o + v] = face ? (isfinite(t) ? t : 0.0f) : 0.0f;
     }
   }
 }
