@@ -240,13 +240,36 @@ def learned_runs(msbgx, args, log, mode="fine"):
     if mode == "fine":
         sched = [(args.learned_steps, 3e-3, 64)]
     else:   # random tile origins make the coarse samples harder: a longer, decaying schedule
-        sched = [(2000, 3e-3, 16), (2000, 1e-3, 16), (4000, 1e-3, 32), (4000, 3e-4, 32)]
+        sched = [(2000, 3e-3, 16), (2000, 1e-3, 16), (4000, 1e-3, 32), (4000, 3e-4, 32), (4000, 3e-4, 64),
+                 (4000, 1e-4, 64)]
     train_ms, first, last = 0.0, None, None
+    # coarse form: after each schedule stage, validate on a held-out dam-break
+    # seed at 384^3 (not a training seed, not an evaluation seed) and keep the
+    # weights with the fewest iterations
+    gv = None
+    if mode == "coarse":
+        vc = msbgx.config_c4(384, seed=900)
+        gv = msbgx.build_grid(vc)
+        gv.prepare_pressure_solve_flip(scene=vc.scene)
+    selection, cum = [], 0
     for k, (steps, lr, bs) in enumerate(sched):
         a, b = net.train(steps=steps, batch=bs, lr=lr, seed=7 + k)
         first = a if first is None else first
         last = b
         train_ms += net.last_train_ms()
+        cum += steps
+        if gv is not None:
+            gv.zero_channel(msbgx.CH_PRESSURE, 0)
+            it_v, rr_v, fb_v = net.solve(gv, max_iter=args.max_iter, tol=args.tol)
+            selection.append({"steps": cum, "val_iterations": it_v, "val_fallback": fb_v, "weights": net.weights})
+            log(f"learned ({mode}): after {cum} steps, validation (dam break 384^3, seed 900): {it_v} iterations")
+    chosen = None
+    if selection:
+        best = min(range(len(selection)), key=lambda i: (selection[i]["val_iterations"], -selection[i]["steps"]))
+        chosen = selection[best]["steps"]
+        net.weights = selection[best]["weights"]
+        gv.destroy()
+        log(f"learned ({mode}): keeping the weights after {chosen} steps")
     steps_total = sum(st for st, _, _ in sched)
     log(f"learned ({mode}): {steps_total} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
     evals = []
@@ -292,7 +315,9 @@ def learned_runs(msbgx, args, log, mode="fine"):
             "train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
                       "samples": net_samples, "collect_s": round(t_collect, 1), "steps": steps_total,
                       "schedule": [{"steps": st, "lr": lr, "batch": bs} for st, lr, bs in sched],
-                      "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1)},
+                      "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1),
+                      "validation": [{k: v for k, v in e.items() if k != "weights"} for e in selection],
+                      "kept_weights_after_steps": chosen},
             "eval": evals, "conv16_roofline": conv, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
 
 
