@@ -101,3 +101,29 @@ def test_scene_particles_at_256():
     g.flip_rhs_from_velocity(cfg.scene.dt)
     b_p2g = np.concatenate([g.get_block(msbgx.CH_DIVERGENCE, 0, int(b), cfg.bsx) for b in blocks])
     assert np.max(np.abs(b_p2g - b_scene)) <= 1e-5 * np.max(np.abs(b_scene))
+
+
+def test_p2g_rejects_bad_input():
+    g = msbgx.MultiresSparseGrid.create(32, 32, 32, 16, 2)
+    t = msbgx.ParticleToGrid(g)
+    with pytest.raises(msbgx.MsbgxError) as e:
+        t.get()                                           # nothing transferred yet
+    assert e.value.rc == msbgx.ESTATE
+    p = np.zeros(4, PARTICLE_DTYPE)
+    p["pos"] = 3.5
+    p["pos"][2, 1] = np.nan
+    with pytest.raises(msbgx.MsbgxError) as e:
+        t.run(p)                                          # non-finite position
+    assert e.value.rc == msbgx.EINVAL
+    with pytest.raises(msbgx.MsbgxError) as e:
+        t.run(p[:2], rho_g=0.0)                           # invalid density
+    assert e.value.rc == msbgx.EINVAL
+    t.run(p[:2])
+    other = msbgx.MultiresSparseGrid.create(32, 32, 32, 16, 2)
+    other.set_refinement_map(np.array([0] + [1] * 7, np.int32))
+    flags = np.zeros((32, 32, 32), np.uint16)
+    w = [np.ones((32, 32, 32), np.float32) for _ in range(3)]
+    other.prepare_pressure_solve_flip(dense=(flags, *w, np.zeros((32, 32, 32), np.float32)))
+    with pytest.raises(msbgx.MsbgxError) as e:
+        t.to_velocity(other)                              # level map differs from the transfer's
+    assert e.value.rc == msbgx.EINVAL
