@@ -439,6 +439,45 @@ def compact_runs(msbgx, g, cfg, args, log):
     return out
 
 
+def pde_runs(msbgx, args, log):
+    """msbg_demo's mean-curvature smoothing (5 iterations, rounded up to 6, 8
+    colours, time step 0.05) of a 256^3 render density (a sphere of radius
+    0.3 with a tanh profile) over the blocks that hold its interface; the
+    oracle on the same input (1 thread)."""
+    import numpy as np
+    n, bsx = 256, 16
+    g = msbgx.MultiresSparseGrid.create(n, n, n, bsx, 3)
+    x = (np.arange(n, dtype=np.float32) + 0.5) / n
+    r = np.sqrt((x[None, None, :] - 0.5) ** 2 + (x[None, :, None] - 0.5) ** 2 + (x[:, None, None] - 0.45) ** 2)
+    rho = np.round(np.clip(0.5 - 0.5 * np.tanh((r - 0.3) * n / 2.5), 0, 1) * 65535).astype(np.uint16)
+    nb = n // bsx
+    blk = rho.reshape(nb, bsx, nb, bsx, nb, bsx)
+    interface = ((blk > 0) & (blk < 65535)).any(axis=(1, 3, 5))        # [bz][by][bx]
+    active = np.nonzero(interface.reshape(-1))[0].astype(np.int32)      # bid = bx + nb (by + nb bz)
+    g.pde_mean_curvature(rho, active, 5, 0.05)                          # warm-up
+    out, ms = g.pde_mean_curvature(rho, active, 5, 0.05)
+    vox = len(active) * bsx ** 3
+    rec = {"workload": "sphere r=0.3 render density 256^3 (uint16), 16^3 blocks, interface blocks listed",
+           "active_blocks": int(len(active)), "iterations": 6, "gpu_ms": round(ms, 3),
+           "Gvoxel_updates_per_s": round(vox * 6 / (ms * 1e-3) / 1e9, 2),
+           "GBs_algorithmic": round(vox * 6 * 4 / (ms * 1e-3) / 1e9, 1)}
+    log(f"pde: {len(active)} blocks, 6 iterations, {ms:.2f} ms")
+    if not args.no_cpu:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+            t0 = time.time()
+            ref = oracle_lib.pde_mean_curvature(np.zeros(nb ** 3, np.int32), bsx, rho, active, 5, 0.05)
+            rec["cpu_oracle_s"] = round(time.time() - t0, 3)
+            rec["cpu_oracle_threads"] = 1
+            rec["bit_identical_to_oracle"] = bool(np.array_equal(ref, out))
+            log(f"pde cpu oracle: {rec['cpu_oracle_s']:.2f} s (1 thread), identical: {rec['bit_identical_to_oracle']}")
+        except Exception as e:
+            log(f"pde cpu oracle failed: {e}")
+    g.destroy()
+    return rec
+
+
 def p2g_runs(msbgx, args, log):
     """Particle-to-grid transfer of the full-scale particle scene (dam break +
     sphere, 8 particles per liquid cell) on the GPU, and GPU vs the CPU oracle
@@ -598,6 +637,7 @@ def main():
         line["flip_caller"] = flip_runs(msbgx, r["g"], cfg, args, log)
         line["particle_to_grid"] = p2g_runs(msbgx, args, log)
         line["compact_mic0_pcg"] = compact_runs(msbgx, r["g"], cfg, args, log)
+        line["pde_demo"] = pde_runs(msbgx, args, log)
     if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
     if not args.no_learned:

@@ -384,6 +384,19 @@ int msbgx_solve_pressure_compact(msbgx_grid *g, int max_iter, float tol, int *it
  * count (cap = 0 queries it). */
 int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap);
 
+/* Reference: MultiresSparseGrid::applyChannelPdeFast<RenderDensity>
+ * (src/msbg4.cpp:60-960) as the msbg_demo solver entry calls it
+ * (src/msbg_demo.cpp:716-727): mean-curvature smoothing of the uint16 render
+ * density (RSURF_16_BIT) with the 8-colour in-place block scheme over the
+ * listed blocks of level 0, one-voxel halo blocks with Neumann values where
+ * no level-0 block holds the voxel (src/halo.cpp:267-480), values clamped to
+ * [0, 1], num_iter rounded up to even, time step time_step.  density is the
+ * dense sx0*sy0*sz0 channel (x fastest), smoothed in place on the GPU; only
+ * voxels of level-0 blocks are read or written.  Needs the level map, not a
+ * prepare; block edge <= 16.  *ms (may be NULL) receives the device time. */
+int msbgx_pde_mean_curvature(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active, int num_iter,
+                             float time_step, double *ms);
+
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
  * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);

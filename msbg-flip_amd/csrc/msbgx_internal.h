@@ -157,6 +157,9 @@ int p2gToVelocity(P2gCtx &c, Grid &g);
 /* compact MIC(0)-PCG comparator (compact.hip) */
 int deviceSolveCompact(Grid &g, int maxIter, float tol, int *iters, float *rel, double *ms);
 int deviceCompactApply(Grid &g, float *preconOut, float *zOut, int cap, int *nRows);
+/* msbg_demo mean-curvature smoothing (pde.hip) */
+int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter, float dt,
+                           double *ms);
 int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst);
 int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst);
 int deviceProlongate(Grid &g, int Lf, int ch);

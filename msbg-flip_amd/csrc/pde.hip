@@ -1,0 +1,207 @@
+/*
+ * pde.hip -- the msbg_demo solver entry on the GPU (compiled as part of
+ * solver.hip): mean-curvature smoothing of a 16-bit render-density channel
+ * with the 8-colour in-place block scheme.
+ *
+ * Reference: MultiresSparseGrid::applyChannelPdeFast<RenderDensity>
+ * (src/msbg4.cpp:60-960) as src/msbg_demo.cpp:716-727 calls it
+ * (PDE_MEAN_CURVATURE + OPT_8_COLOR_SCHEME, doConstrZeroOne, no density
+ * constraints, grid spacing 1), with its halo blocks
+ * (HaloBlockSet::fillHaloBlock_ with OPT_BC_NEUMANN, src/halo.cpp:267-480,
+ * getOutOfBlockValue src/halo.h:267-290); RenderDensity is uint16
+ * (RSURF_16_BIT, src/msbgcell.h:68):
+ *   iterations: numIter rounded up to even (src/msbg4.cpp:156-157);
+ *   per iteration, colours 0..7 (colour = (bx&1) + 2(by&1) + 4(bz&1),
+ *   src/msbg.h:642-646), per listed level-0 block of that colour:
+ *     halo (one voxel) as float v / 65535: a voxel of a level-0 block from
+ *     the channel, any other voxel (outside the domain, or in a block of
+ *     another level) the block's own nearest voxel (Neumann);
+ *     per voxel: central differences, H = (fy^2+fz^2) fxx + (fx^2+fz^2) fyy +
+ *     (fx^2+fy^2) fzz - 2 (fx fy fyx + fx fz fzx + fy fz fzy), divided by
+ *     |grad|^2 when that exceeds 1e-7 (else 0), D = clamp(dt H, +-0.1),
+ *     phi = clamp(phi + D, 0, 1), stored as round-to-nearest-even(phi 65535).
+ * Same-colour blocks are never neighbours (not even diagonally), so one
+ * colour's blocks update in place from halos that no launch of that colour
+ * writes.
+ *
+ * Data: the channel over the level-0 blocks in ascending block order, B^3
+ * uint16 per block (a level-0 block the caller leaves empty reads as its
+ * values); one workgroup per block stages the (B+2)^3 halo in LDS (B <= 16).
+ * Algorithmic bytes per voxel and iteration: 2 read + 2 written.
+ */
+
+namespace msbgx {
+
+struct PdeArgs {
+  const int *slot;   // per block: rank among the level-0 blocks, -1 otherwise
+  int nbx, nby, nbz, lb;
+  int n[3];
+  float dt;
+};
+
+template <int B>
+__global__ void __launch_bounds__(256) k_pde_mc(PdeArgs a, const int *__restrict__ blocks, uint16_t *__restrict__ rho)
+{
+  constexpr int H = B + 2, H2 = H * H, N = B * B * B;
+  __shared__ float halo[H * H * H];
+  const int bid = blocks[blockIdx.x];
+  const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  const int64_t own = (int64_t)a.slot[bid] * N;
+  const int g0[3] = {bx * B, by * B, bz * B};
+  const float inv = 1.f / 65535;   // 1.f / numeric_limits<uint16_t>::max()
+  for (int h = threadIdx.x; h < H * H * H; h += blockDim.x) {
+    const int hx = h % H, hy = (h / H) % H, hz = h / H2;
+    const int g[3] = {g0[0] + hx - 1, g0[1] + hy - 1, g0[2] + hz - 1};
+    int64_t src = -1;
+    if (g[0] >= 0 && g[1] >= 0 && g[2] >= 0 && g[0] < a.n[0] && g[1] < a.n[1] && g[2] < a.n[2]) {
+      const int nb = (g[0] >> a.lb) + a.nbx * ((g[1] >> a.lb) + a.nby * (g[2] >> a.lb));
+      const int s = a.slot[nb];
+      if (s >= 0) src = (int64_t)s * N + (g[0] & (B - 1)) + B * ((g[1] & (B - 1)) + B * (g[2] & (B - 1)));
+    }
+    if (src < 0) {   // Neumann: the block's own nearest voxel
+      const int cx = min(max(hx - 1, 0), B - 1), cy = min(max(hy - 1, 0), B - 1), cz = min(max(hz - 1, 0), B - 1);
+      src = own + cx + B * (cy + B * cz);
+    }
+    halo[h] = (float)rho[src] * inv;
+  }
+  __syncthreads();
+  const float dt = a.dt;
+  for (int v = threadIdx.x; v < N; v += blockDim.x) {
+    const int vx = v % B, vy = (v / B) % B, vz = v / (B * B);
+    const float *ph = halo + (vx + 1) + H * (vy + 1) + H2 * (vz + 1);
+    auto F = [&](int dx, int dy, int dz) { return ph[dx + H * dy + H2 * dz]; };
+    const float f0 = F(0, 0, 0), f1 = F(-1, 0, 0), f2 = F(1, 0, 0), f3 = F(0, -1, 0), f4 = F(0, 1, 0),
+                f5 = F(0, 0, -1), f6 = F(0, 0, 1);
+    const float fx = 0.5f * (f2 - f1), fy = 0.5f * (f4 - f3), fz = 0.5f * (f6 - f5);
+    const float fxx = f1 + f2 - 2 * f0, fyy = f3 + f4 - 2 * f0, fzz = f5 + f6 - 2 * f0;
+    const float fy_right = (F(1, 1, 0) - F(1, -1, 0)) * 0.5f, fy_left = (F(-1, 1, 0) - F(-1, -1, 0)) * 0.5f,
+                fz_right = (F(1, 0, 1) - F(1, 0, -1)) * 0.5f, fz_left = (F(-1, 0, 1) - F(-1, 0, -1)) * 0.5f,
+                fz_up = (F(0, 1, 1) - F(0, 1, -1)) * 0.5f, fz_down = (F(0, -1, 1) - F(0, -1, -1)) * 0.5f,
+                fyx = (fy_right - fy_left) * 0.5f, fzx = (fz_right - fz_left) * 0.5f, fzy = (fz_up - fz_down) * 0.5f;
+    float Hc = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
+               2.0f * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
+    const float gradMagSq = fx * fx + fy * fy + fz * fz;
+    Hc = gradMagSq > 1e-7f ? Hc / gradMagSq : 0.0f;
+    float D = dt * Hc;
+    D = fminf(fmaxf(D, -0.1f), 0.1f);
+    float phi = f0 + D;
+    phi = fminf(fmaxf(phi, 0.0f), 1.0f);
+    rho[own + v] = (uint16_t)(int)rintf(phi * 65535);
+  }
+}
+
+/* Host: upload the dense channel, run the colour passes, download. */
+int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter, float dt,
+                           double *ms)
+{
+  const int B = g.bsx0, lb = __builtin_ctz(B);
+  if (B > 16) {
+    setError("mean-curvature smoothing: block edge above 16 is not supported (the halo block is staged in LDS)");
+    return MSBGX_EINVAL;
+  }
+  const MgLevel &m = g.mg[0];
+  std::vector<int> slot(g.nBlocks, -1), level0;
+  for (int b = 0; b < g.nBlocks; b++)
+    if (g.map[b] == 0) {
+      slot[b] = (int)level0.size();
+      level0.push_back(b);
+    }
+  (void)m;
+  /* colour lists of the listed level-0 blocks, in list order */
+  std::vector<int> lists[8];
+  for (int i = 0; i < nActive; i++) {
+    const int b = active[i];
+    if (b < 0 || b >= g.nBlocks) {
+      setError("mean-curvature smoothing: active block id out of range");
+      return MSBGX_EINVAL;
+    }
+    if (slot[b] < 0) continue;   // only level-0 blocks are smoothed (src/msbg4.cpp:447)
+    int bx, by, bz;
+    g.blockCoords(b, bx, by, bz);
+    lists[(bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2)].push_back(b);
+  }
+  const int nIter = (numIter % 2 == 1) ? numIter + 1 : numIter;
+  const size_t nv = level0.size() * (size_t)B * B * B;
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct Res {
+    hipStream_t s;
+    std::vector<void *> p;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Res()
+    {
+      for (void *q : p) (void)hipFree(q);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      (void)hipStreamDestroy(s);
+    }
+  } res{s, {}};
+  uint16_t *dRho = nullptr;
+  int *dSlot = nullptr, *dLists = nullptr;
+  HIPCHK(hipMalloc((void **)&dRho, std::max<size_t>(nv, 1) * sizeof(uint16_t)));
+  res.p.push_back(dRho);
+  HIPCHK(hipMalloc((void **)&dSlot, slot.size() * sizeof(int)));
+  res.p.push_back(dSlot);
+  size_t nl = 0;
+  for (auto &l : lists) nl += l.size();
+  HIPCHK(hipMalloc((void **)&dLists, std::max<size_t>(nl, 1) * sizeof(int)));
+  res.p.push_back(dLists);
+  HIPCHK(hipEventCreate(&res.e0));
+  HIPCHK(hipEventCreate(&res.e1));
+  /* dense -> natural level-0 layout */
+  std::vector<uint16_t> nat(nv);
+#pragma omp parallel for schedule(static)
+  for (int li = 0; li < (int)level0.size(); li++) {
+    int bx, by, bz;
+    g.blockCoords(level0[li], bx, by, bz);
+    for (int vz = 0; vz < B; vz++)
+      for (int vy = 0; vy < B; vy++) {
+        const uint16_t *src = dense + ((size_t)(bz * B + vz) * g.sy0 + (size_t)(by * B + vy)) * g.sx0 + (size_t)bx * B;
+        std::copy(src, src + B, nat.data() + (size_t)li * B * B * B + (size_t)B * (vy + (size_t)B * vz));
+      }
+  }
+  HIPCHK(hipMemcpy(dRho, nat.data(), nv * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dSlot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> all;
+  int start[9] = {0};
+  for (int c = 0; c < 8; c++) {
+    start[c] = (int)all.size();
+    all.insert(all.end(), lists[c].begin(), lists[c].end());
+  }
+  start[8] = (int)all.size();
+  if (!all.empty()) HIPCHK(hipMemcpy(dLists, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  PdeArgs a;
+  a.slot = dSlot; a.nbx = g.nbx; a.nby = g.nby; a.nbz = g.nbz; a.lb = lb;
+  a.n[0] = g.sx0; a.n[1] = g.sy0; a.n[2] = g.sz0; a.dt = dt;
+  HIPCHK(hipEventRecord(res.e0, s));
+  for (int it = 0; it < nIter; it++)
+    for (int c = 0; c < 8; c++) {
+      const int cnt = start[c + 1] - start[c];
+      if (cnt == 0) continue;
+      if (B == 16) hipLaunchKernelGGL(k_pde_mc<16>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
+      else if (B == 8) hipLaunchKernelGGL(k_pde_mc<8>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
+      else if (B == 4) hipLaunchKernelGGL(k_pde_mc<4>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
+      else hipLaunchKernelGGL(k_pde_mc<2>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
+    }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(res.e1, s));
+  HIPCHK(hipEventSynchronize(res.e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, res.e0, res.e1));
+  if (ms) *ms = t;
+  HIPCHK(hipMemcpy(nat.data(), dRho, nv * sizeof(uint16_t), hipMemcpyDeviceToHost));
+#pragma omp parallel for schedule(static)
+  for (int li = 0; li < (int)level0.size(); li++) {
+    int bx, by, bz;
+    g.blockCoords(level0[li], bx, by, bz);
+    for (int vz = 0; vz < B; vz++)
+      for (int vy = 0; vy < B; vy++) {
+        uint16_t *dst = dense + ((size_t)(bz * B + vz) * g.sy0 + (size_t)(by * B + vy)) * g.sx0 + (size_t)bx * B;
+        const uint16_t *src = nat.data() + (size_t)li * B * B * B + (size_t)B * (vy + (size_t)B * vz);
+        std::copy(src, src + B, dst);
+      }
+  }
+  return MSBGX_OK;
+}
+
+}  // namespace msbgx

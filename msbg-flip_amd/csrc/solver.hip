@@ -2305,6 +2305,7 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
 #include "flip.hip"
 #include "p2g.hip"
 #include "compact.hip"
+#include "pde.hip"
 
 namespace msbgx {
 /* Captured graphs bake in launch shapes and kernel choices: drop them. */

@@ -225,6 +225,7 @@ def _load():
         "msbgx_p2g_to_velocity": (C.c_int, [P, P]),
         "msbgx_solve_pressure_compact": (C.c_int, [P, C.c_int, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
         "msbgx_compact_mic0": (C.c_int, [P, P, P, C.c_int]),
+        "msbgx_pde_mean_curvature": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_double)]),
         "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
         "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
         "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
@@ -258,7 +259,7 @@ EXPORTED = [
     "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity", "msbgx_default_p2g_params", "msbgx_p2g_create",
     "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
     "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
-    "msbgx_solve_pressure_compact", "msbgx_compact_mic0",
+    "msbgx_solve_pressure_compact", "msbgx_compact_mic0", "msbgx_pde_mean_curvature",
 ]
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
@@ -377,6 +378,15 @@ class MultiresSparseGrid:
         applies it to -p (the MSBG path solves A p = +div/dt)."""
         mode = FLIP_SIGN_CORRECTED if corrected else FLIP_SIGN_REFERENCE
         _ck(lib.msbgx_flip_update_velocity(self._h, dt, mode), "flip_update_velocity")
+
+    def pde_mean_curvature(self, density, active_blocks, num_iter=5, time_step=0.05):
+        """msbg_demo's mean-curvature smoothing (src/msbg4.cpp:60-960) of a
+        uint16 [z][y][x] render-density array; returns (smoothed copy, device ms)."""
+        d = np.ascontiguousarray(density, np.uint16).copy()
+        a = np.ascontiguousarray(active_blocks, np.int32)
+        ms = C.c_double()
+        _ck(lib.msbgx_pde_mean_curvature(self._h, _ptr(d), _ptr(a), len(a), num_iter, time_step, C.byref(ms)), "pde")
+        return d, ms.value
 
     def solve_pressure_compact(self, max_iter=1000, tol=1e-6):
         """Compact MIC(0)-PCG comparator (src/flip_sim.cpp:858-1151); the

@@ -478,3 +478,71 @@ extern "C" int orc_compact_pcg(int nx, int ny, int nz, int bsx, const int *block
   }
   return n;
 }
+
+/* ------------------------------------------------------------------------
+ * Mean-curvature smoothing of a uint16 render-density channel, 8-colour
+ * in-place block scheme: applyChannelPdeFast<RenderDensity> as msbg_demo
+ * calls it (src/msbg4.cpp:60-960, src/msbg_demo.cpp:716-727), halo blocks
+ * with OPT_BC_NEUMANN (src/halo.cpp:267-480, src/halo.h:267-290),
+ * block colours (src/msbg.h:642-646).  rho is the dense nx*ny*nz channel
+ * (x fastest); only voxels of level-0 blocks hold data.
+ * ---------------------------------------------------------------------- */
+extern "C" void orc_pde_mean_curvature(int nx, int ny, int nz, int bsx, const int *blockLevel, uint16_t *rho,
+                                       const int *active, int nActive, int numIter, float dt)
+{
+  const int nbx = nx / bsx, nby = ny / bsx;
+  const int hsx = bsx + 2, hsx2 = hsx * hsx;
+  auto idx = [&](int x, int y, int z) { return ((int64_t)z * ny + y) * nx + x; };
+  auto hasData = [&](int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return false;
+    return blockLevel[x / bsx + nbx * (y / bsx + nby * (z / bsx))] == 0;
+  };
+  int nMaxIter = numIter;
+  if ((nMaxIter % 2) == 1) nMaxIter++;
+  std::vector<float> halo((size_t)hsx * hsx * hsx);
+  for (int iIter = 0; iIter < nMaxIter; iIter++)
+    for (int pass = 0; pass < 8; pass++)
+      for (int t = 0; t < nActive; t++) {
+        const int bid = active[t];
+        const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+        const int color = (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
+        if (color != pass) continue;
+        if (blockLevel[bid] != 0) continue;
+        const int x0 = bx * bsx, y0 = by * bsx, z0 = bz * bsx;
+        /* fillHaloBlock_ with OPT_BC_NEUMANN */
+        for (int z = z0 - 1; z <= z0 + bsx; z++)
+          for (int y = y0 - 1; y <= y0 + bsx; y++)
+            for (int x = x0 - 1; x <= x0 + bsx; x++) {
+              uint16_t val;
+              if (hasData(x, y, z)) val = rho[idx(x, y, z)];
+              else val = rho[idx(std::min(std::max(x, x0), x0 + bsx - 1), std::min(std::max(y, y0), y0 + bsx - 1),
+                                 std::min(std::max(z, z0), z0 + bsx - 1))];
+              halo[(size_t)(x - x0 + 1) + hsx * (y - y0 + 1) + hsx2 * (z - z0 + 1)] = val * (1.f / 65535);
+            }
+        for (int vz = 0; vz < bsx; vz++)
+          for (int vy = 0; vy < bsx; vy++)
+            for (int vx = 0; vx < bsx; vx++) {
+              const float *ph = halo.data() + (vx + 1) + hsx * (vy + 1) + hsx2 * (vz + 1);
+              auto F_ = [&](int dx, int dy, int dz) { return ph[dx + dy * hsx + dz * hsx2]; };
+              float phi = F_(0, 0, 0);
+              const float f0 = F_(0, 0, 0), f1 = F_(-1, 0, 0), f2 = F_(1, 0, 0), f3 = F_(0, -1, 0),
+                          f4 = F_(0, 1, 0), f5 = F_(0, 0, -1), f6 = F_(0, 0, 1);
+              const float fx = 0.5f * (f2 - f1), fy = 0.5f * (f4 - f3), fz = 0.5f * (f6 - f5);
+              const float fxx = f1 + f2 - 2 * f0, fyy = f3 + f4 - 2 * f0, fzz = f5 + f6 - 2 * f0;
+              const float fy_right = (F_(1, 1, 0) - F_(1, -1, 0)) * 0.5f, fy_left = (F_(-1, 1, 0) - F_(-1, -1, 0)) * 0.5f,
+                          fz_right = (F_(1, 0, 1) - F_(1, 0, -1)) * 0.5f, fz_left = (F_(-1, 0, 1) - F_(-1, 0, -1)) * 0.5f,
+                          fz_up = (F_(0, 1, 1) - F_(0, 1, -1)) * 0.5f, fz_down = (F_(0, -1, 1) - F_(0, -1, -1)) * 0.5f,
+                          fyx = (fy_right - fy_left) * 0.5f, fzx = (fz_right - fz_left) * 0.5f,
+                          fzy = (fz_up - fz_down) * 0.5f;
+              float H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
+                        2.0f * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
+              const float gradMagSq = fx * fx + fy * fy + fz * fz;
+              H = gradMagSq > 1e-7f ? H / gradMagSq : 0.0f;
+              float D = dt * H;
+              D = std::min(std::max(D, -0.1f), 0.1f);
+              phi += D;
+              phi = std::min(std::max(phi, 0.0f), 1.0f);
+              rho[idx(x0 + vx, y0 + vy, z0 + vz)] = (uint16_t)(int)std::nearbyint(phi * 65535);
+            }
+      }
+}

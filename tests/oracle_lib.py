@@ -40,6 +40,7 @@ for name, res, args in [
     ("orc_solve_fp64", C.c_int, [P, C.c_double, C.c_int, P, C.POINTER(C.c_double)]),
     ("orc_flip_rhs", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, C.c_float, P]),
     ("orc_scene_particles", C.c_int64, [C.POINTER(Scene), P]),
+    ("orc_pde_mean_curvature", None, [C.c_int] * 4 + [P, P, P, C.c_int, C.c_int, C.c_float]),
     ("orc_compact_pcg", C.c_int, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, C.c_float, C.c_int, P, P, P,
                                                    C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("orc_p2g", None, [C.c_int] * 4 + [P, C.c_int64, P] + [C.c_float] * 5 + [C.c_int] + [P] * 8),
@@ -251,3 +252,13 @@ def compact_pcg(block_level, bsx, mass, mass_eps, beta, div, tol=1e-6, max_iter=
     if want_factor:
         pre, z = pre[:n], z[:n]
     return x, it.value, rr.value, pre, z
+
+
+def pde_mean_curvature(block_level, bsx, density, active, num_iter=5, dt=0.05):
+    """applyChannelPdeFast (mean curvature, 8 colours) on a uint16 [z][y][x] array; returns a copy."""
+    nz, ny, nx = density.shape
+    lv = _levels(block_level, nx, ny, nz, bsx)
+    d = np.ascontiguousarray(density, np.uint16).copy()
+    a = np.ascontiguousarray(active, np.int32)
+    _lib.orc_pde_mean_curvature(nx, ny, nz, bsx, _p(lv), _p(d), _p(a), len(a), num_iter, dt)
+    return d
