@@ -340,10 +340,13 @@ __device__ __forceinline__ void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0
  * hyperplane hp is the compile-time constant hp mod B.  The four sequential
  * local sweeps are hyperplane wavefronts over seg + y + z, forward and
  * reverse as in src/msbg3.cpp:1157-1175; the fifth is the Jacobi pass whose
- * result /h is stored (src/msbg3.cpp:1341-1349).  Per step each thread
- * publishes three values to LDS -- old value at z, old value one step ahead,
- * newest value one step behind -- which is exactly what its x/y neighbours
- * read at that step; LDS is double-buffered, one barrier per step. */
+ * result /h is stored (src/msbg3.cpp:1341-1349).  One barrier per step.
+ * x/y neighbours come through LDS: for B <= 16 the whole block with its x/y
+ * halo ring stays in LDS (Halo3) and each step writes back the cell it
+ * updated; for B = 32 (no room for that) each thread publishes three values
+ * per step -- old value at z, old value one step ahead, newest value one step
+ * behind -- which is exactly what its x/y neighbours read at that step, in a
+ * double-buffered LDS array. */
 template <int B, bool REGC> struct ColRegs;
 template <int B> struct ColRegs<B, true> {
   float xs[B], cd[B], cb[B], w1[B], w2[B], w3[B], w4[B], w5[B];
@@ -363,6 +366,14 @@ struct ColCtx {
   int tx, ty, o, fk1, fk3, fk5, mixed0, mixed2, mixed4;
   int64_t fo1, fo3, fo5;
   float w6top, om, omm, HR;
+  int h3;                   // B <= 16: the thread's cell at z = 0 in the halo array (Halo3)
+  float fz0, fz1;           // B <= 16: the thread's -z and +z halo face values
+};
+
+/* B <= 16: the block's current h*x values in LDS with the x and y halo faces
+ * as a one-cell ring, (B+2)^2 * B floats, x fastest. */
+template <int B> struct Halo3 {
+  static constexpr int P = B + 2, P2 = P * P, N = P2 * B;
 };
 
 /* The seven coefficients of cell z of the thread's column; P < 0 reads the
@@ -498,36 +509,89 @@ struct PeriodUnroll<B, REGC, DIR, UPDATE, B> {
   }
 };
 
-/* One local sweep: ceil(T/B) periods of B hyperplanes.  B is even, so the
- * double-buffer index I & 1 alternates across period boundaries too. */
-/* The fifth local sweep reads only the fourth sweep's values (a Jacobi
- * pass, src/msbg3.cpp:1341-1349), so with the coefficients in registers it
- * needs no wavefront: every thread publishes its whole column to LDS once and
- * then computes its B cells independently, reading each neighbour at the
- * cell's own z (or z -+ 1) exactly as the wavefront form does. */
-template <int B>
-__device__ __forceinline__ void colJacobi(ColRegs<B, true> &r, uint32_t skip, float *colz, const ColCtx &c)
+/* B <= 16: one hyperplane step over the halo array (Halo3).  Each active
+ * thread reads the four in-plane neighbours of its cell at fixed offsets from
+ * one address, updates the cell and writes the new value back; one barrier
+ * per step.  The values read are the ones the wavefront order prescribes:
+ * neighbours one hyperplane behind were written at the previous step, those
+ * one ahead are not written yet, and the only neighbours on the same
+ * hyperplane -- x -+ 1 inside the cell's SIMD segment, updated together as in
+ * src/msbg3.cpp:1700-1743 -- are written by this step only after the writer
+ * has read its own neighbours, which the old values it reads must precede
+ * (the write depends on those reads, and both lanes are in one wave, whose
+ * LDS operations execute in issue order). */
+template <int B, int DIR, int I>
+__device__ __forceinline__ void colStepH(ColRegs<B, true> &r, uint32_t skip, float *h3, const ColCtx &c, int hp)
 {
-  constexpr int NT = Cfg<B>::NT, NF = Cfg<B>::NF;
-  const int tid = threadIdx.x, tx = c.tx, ty = c.ty;
-#pragma unroll
-  for (int sl = 0; sl < B; sl++) {
-    const int z = (sl - c.o + 2 * B) & (B - 1);
-    colz[z * NT + tid] = r.xs[sl];
+  constexpr int T = Cfg<B>::T, P = Halo3<B>::P, P2 = Halo3<B>::P2;
+  constexpr int R0 = (T - 1) % B;
+  constexpr int sl = DIR > 0 ? I : ((R0 - I) % B + B) % B;
+  constexpr int slM = ((sl - 1) % B + B) % B;
+  constexpr int slP = ((sl + 1) % B + B) % B;
+  const int z = hp - c.o;
+  const bool active = z >= 0 && z < B && !((skip >> sl) & 1u);
+  if (active) {
+    float *p = h3 + c.h3 + z * P2;
+    const float F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F0 = r.xs[sl];
+    const float F5 = z > 0 ? r.xs[slM] : c.fz0;
+    const float F6 = z < B - 1 ? r.xs[slP] : c.fz1;
+    const float w6 = z < B - 1 ? r.w5[slP] : c.w6top;
+    const float D = r.cd[sl];
+    float Sm = F1 * r.w1[sl];
+    Sm = fmaf(F2, r.w2[sl], Sm);
+    Sm = fmaf(F3, r.w3[sl], Sm);
+    Sm = fmaf(F4, r.w4[sl], Sm);
+    Sm = fmaf(F5, r.w5[sl], Sm);
+    Sm = fmaf(F6, w6, Sm);
+    const float Y = D != 0.0f ? fmaf(c.om * (r.cb[sl] + Sm), D, c.omm * F0) : F0;
+    r.xs[sl] = Y;
+    *p = Y;
   }
   ldsBarrier();
+}
+
+template <int B, int DIR, int I>
+struct PeriodUnrollH {
+  __device__ __forceinline__ static void run(ColRegs<B, true> &r, uint32_t skip, float *h3, const ColCtx &c, int base)
+  {
+    constexpr int T = Cfg<B>::T;
+    const int step = base + I;
+    if (step < T) colStepH<B, DIR, I>(r, skip, h3, c, DIR > 0 ? step : T - 1 - step);
+    PeriodUnrollH<B, DIR, I + 1>::run(r, skip, h3, c, base);
+  }
+};
+template <int B, int DIR>
+struct PeriodUnrollH<B, DIR, B> {
+  __device__ __forceinline__ static void run(ColRegs<B, true> &, uint32_t, float *, const ColCtx &, int) {}
+};
+
+template <int B, int DIR>
+__device__ __forceinline__ void colSweepH(ColRegs<B, true> &r, uint32_t skip, float *h3, const ColCtx &c)
+{
+  constexpr int T = Cfg<B>::T;
+#pragma unroll 1
+  for (int base = 0; base < T; base += B) PeriodUnrollH<B, DIR, 0>::run(r, skip, h3, c, base);
+}
+
+/* The fifth local sweep reads only the fourth sweep's values (a Jacobi
+ * pass, src/msbg3.cpp:1341-1349), which the halo array holds once the fourth
+ * sweep's last barrier has passed: no wavefront, each thread computes its B
+ * cells independently and stores the result /h. */
+template <int B>
+__device__ __forceinline__ void colJacobiH(ColRegs<B, true> &r, uint32_t skip, const float *h3, const ColCtx &c)
+{
+  constexpr int P = Halo3<B>::P, P2 = Halo3<B>::P2;
 #pragma unroll
   for (int sl = 0; sl < B; sl++) {
     if ((skip >> sl) & 1u) continue;
     const int z = (sl - c.o + 2 * B) & (B - 1);
     const int slM = (sl + B - 1) & (B - 1), slP = (sl + 1) & (B - 1);
+    const float *p = h3 + c.h3 + z * P2;
     const float F0 = r.xs[sl];
-    const float F1 = tx > 0 ? colz[z * NT + tid - 1] : c.faces[0 * NF + ty + B * z];
-    const float F2 = tx < B - 1 ? colz[z * NT + tid + 1] : c.faces[1 * NF + ty + B * z];
-    const float F3 = ty > 0 ? colz[z * NT + tid - B] : c.faces[2 * NF + tx + B * z];
-    const float F4 = ty < B - 1 ? colz[z * NT + tid + B] : c.faces[3 * NF + tx + B * z];
-    const float F5 = z > 0 ? r.xs[slM] : c.faces[4 * NF + tx + B * ty];
-    const float F6 = z < B - 1 ? r.xs[slP] : c.faces[5 * NF + tx + B * ty];
+    const float F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F5 = z > 0 ? r.xs[slM] : c.fz0;
+    const float F6 = z < B - 1 ? r.xs[slP] : c.fz1;
     const float D = r.cd[sl], Bb = r.cb[sl];
     const float w6 = z < B - 1 ? r.w5[slP] : c.w6top;
     float Sm = F1 * r.w1[sl];
@@ -537,10 +601,12 @@ __device__ __forceinline__ void colJacobi(ColRegs<B, true> &r, uint32_t skip, fl
     Sm = fmaf(F5, r.w5[sl], Sm);
     Sm = fmaf(F6, w6, Sm);
     const float Y = D != 0.0f ? fmaf(c.om * (Bb + Sm), D, c.omm * F0) : F0;
-    c.Xg[c.off + tx + B * ty + B * B * z] = Y * c.HR;
+    c.Xg[c.off + c.tx + B * c.ty + B * B * z] = Y * c.HR;
   }
 }
 
+/* B = 32: one local sweep of ceil(T/B) periods of B hyperplanes.  B is even,
+ * so the double-buffer index I & 1 alternates across period boundaries too. */
 template <int B, bool REGC, int DIR, bool UPDATE>
 __device__ __forceinline__ void colSweep(ColRegs<B, REGC> &r, uint32_t skip, float (*pub)[3][Cfg<B>::NT],
                                          const ColCtx &c)
@@ -558,9 +624,10 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   if (*done) return;
   constexpr bool REGC = B <= 16;
   constexpr int S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
+  constexpr int P = Halo3<B>::P, P2 = Halo3<B>::P2;
   __shared__ float faces[6 * NF];
-  __shared__ float pub[2][3][NT];
-  __shared__ float colz[REGC ? B * NT : 1];   // one-shot Jacobi sweep: whole columns
+  __shared__ float pub[REGC ? 1 : 2][3][REGC ? 1 : NT];   // B = 32: double-buffered step values
+  __shared__ float h3[REGC ? Halo3<B>::N : 1];            // B <= 16: block values with x/y halo ring
   const int bid = blocks[blockIdx.x];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
@@ -587,6 +654,9 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
     if (a.prec == MSBGX_PREC_FP16) skip = loadColumn<B, MSBGX_PREC_FP16>(r, c, X, h);
     else if (a.prec == MSBGX_PREC_BF16) skip = loadColumn<B, MSBGX_PREC_BF16>(r, c, X, h);
     else skip = loadColumn<B, MSBGX_PREC_FP32>(r, c, X, h);
+    c.h3 = (c.ty + 1) * P + c.tx + 1;
+#pragma unroll
+    for (int sl = 0; sl < B; sl++) h3[c.h3 + ((sl - c.o + 2 * B) & (B - 1)) * P2] = r.xs[sl];
   } else {
 #pragma unroll
     for (int sl = 0; sl < B; sl++) {
@@ -598,23 +668,48 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
     }
   }
   __syncthreads();
-  /* sweep k is reversed iff reverse == (k & 1) (src/msbg3.cpp:1157) */
-  if (reverse) {
+  if constexpr (REGC) {
+    /* x/y halo faces into the ring of the halo array: thread (u, w) = (tx, ty)
+     * moves face value u + B*w of each face (u along the face, w = z) */
+    const int u = c.tx, w = c.ty;
+    h3[w * P2 + (u + 1) * P] = faces[0 * NF + u + B * w];
+    h3[w * P2 + (u + 1) * P + B + 1] = faces[1 * NF + u + B * w];
+    h3[w * P2 + u + 1] = faces[2 * NF + u + B * w];
+    h3[w * P2 + (B + 1) * P + u + 1] = faces[3 * NF + u + B * w];
+    c.fz0 = faces[4 * NF + c.tx + B * c.ty];
+    c.fz1 = faces[5 * NF + c.tx + B * c.ty];
+    __syncthreads();
+    /* sweep k is reversed iff reverse == (k & 1) (src/msbg3.cpp:1157) */
+    if (reverse) {
 #pragma unroll 1
-    for (int pair = 0; pair < 2; pair++) {
-      colSweep<B, REGC, +1, true>(r, skip, pub, c);
-      colSweep<B, REGC, -1, true>(r, skip, pub, c);
+      for (int pair = 0; pair < 2; pair++) {
+        colSweepH<B, +1>(r, skip, h3, c);
+        colSweepH<B, -1>(r, skip, h3, c);
+      }
+    } else {
+#pragma unroll 1
+      for (int pair = 0; pair < 2; pair++) {
+        colSweepH<B, -1>(r, skip, h3, c);
+        colSweepH<B, +1>(r, skip, h3, c);
+      }
     }
-    if constexpr (REGC) colJacobi<B>(r, skip, colz, c);
-    else colSweep<B, REGC, +1, false>(r, skip, pub, c);
+    colJacobiH<B>(r, skip, h3, c);
   } else {
+    if (reverse) {
 #pragma unroll 1
-    for (int pair = 0; pair < 2; pair++) {
-      colSweep<B, REGC, -1, true>(r, skip, pub, c);
-      colSweep<B, REGC, +1, true>(r, skip, pub, c);
+      for (int pair = 0; pair < 2; pair++) {
+        colSweep<B, REGC, +1, true>(r, skip, pub, c);
+        colSweep<B, REGC, -1, true>(r, skip, pub, c);
+      }
+      colSweep<B, REGC, +1, false>(r, skip, pub, c);
+    } else {
+#pragma unroll 1
+      for (int pair = 0; pair < 2; pair++) {
+        colSweep<B, REGC, -1, true>(r, skip, pub, c);
+        colSweep<B, REGC, +1, true>(r, skip, pub, c);
+      }
+      colSweep<B, REGC, -1, false>(r, skip, pub, c);
     }
-    if constexpr (REGC) colJacobi<B>(r, skip, colz, c);
-    else colSweep<B, REGC, -1, false>(r, skip, pub, c);
   }
 }
 
