@@ -134,6 +134,20 @@ __device__ __forceinline__ bool segmentAllNonFluid(bool nonFluid)
   return segBits == ((1ull << S) - 1);
 }
 
+/* Face neighbour of block bid across face (0..5 = -x, +x, -y, +y, -z, +z),
+ * -1 outside the domain. */
+__device__ __forceinline__ int faceNeighbour(const LevelArgs &a, int bid, int face)
+{
+  if (a.nbr) return a.nbr[6 * bid + face];   // batched block sets: explicit face-neighbour table
+  const int dir = face >> 1, side = face & 1;
+  int nbxx = bid % a.nbx, nbyy = (bid / a.nbx) % a.nby, nbzz = bid / (a.nbx * a.nby);
+  if (dir == 0) nbxx += side ? 1 : -1;
+  else if (dir == 1) nbyy += side ? 1 : -1;
+  else nbzz += side ? 1 : -1;
+  return (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz)
+             ? -1 : nbxx + a.nbx * (nbyy + a.nby * nbzz);
+}
+
 /* Halo faces of a block into faces[6][B*B] (h*x of same-level neighbours,
  * mixed-resolution aggregates, 0 at the domain boundary) plus per-face kinds
  * and neighbour slot offsets (src/msbg3.cpp:665-1044, 46-161). */
@@ -150,17 +164,7 @@ __device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int
 #pragma unroll
   for (int face = 0; face < 6; face++) {
     const int dir = face >> 1, side = face & 1;
-    int nb;
-    if (a.nbr) {
-      nb = a.nbr[6 * bid + face];   // batched block sets: explicit face-neighbour table
-    } else {
-      int nbxx = bid % a.nbx, nbyy = (bid / a.nbx) % a.nby, nbzz = bid / (a.nbx * a.nby);
-      if (dir == 0) nbxx += side ? 1 : -1;
-      else if (dir == 1) nbyy += side ? 1 : -1;
-      else nbzz += side ? 1 : -1;
-      nb = (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz)
-               ? -1 : nbxx + a.nbx * (nbyy + a.nby * nbzz);
-    }
+    const int nb = faceNeighbour(a, bid, face);
     float val = 0.0f;
     int kind;
     int64_t off2 = 0;
@@ -324,6 +328,53 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
   }
 }
 
+/* The halo faces of stageFaces, into registers, for a block whose six face
+ * neighbours all have its own level or lie outside the domain (every block
+ * of a single-level grid, the interior of a level): the six neighbour levels
+ * and offsets are read in one group and then the six face values in one
+ * group, instead of one dependent round trip per face.  Thread (u, w) =
+ * (tx, ty) gets x of face cell u + B*w of each face; the face value stageFaces
+ * writes is that times h for a same-level face and 0 at the domain boundary
+ * (sameLevelFaceValue), applied only where the value is used so that the
+ * loads stay in flight meanwhile.  Returns false, loading no face value,
+ * when a neighbour has another level. */
+template <int B>
+__device__ __forceinline__ bool sameLevelFaces(const LevelArgs &a, int bid, int lev, float h, int64_t off,
+                                               const float *__restrict__ X, float *fv, int *fkind, int64_t *foff)
+{
+  const int u = threadIdx.x % B, w = threadIdx.x / B;
+  int nb[6], lev2[6];
+  int64_t off2[6];
+#pragma unroll
+  for (int face = 0; face < 6; face++) nb[face] = faceNeighbour(a, bid, face);
+#pragma unroll
+  for (int face = 0; face < 6; face++) {
+    const int n = nb[face] < 0 ? bid : nb[face];
+    lev2[face] = a.level[n];
+    off2[face] = a.off[n];
+  }
+  bool same = true;
+#pragma unroll
+  for (int face = 0; face < 6; face++) same = same && (nb[face] < 0 || lev2[face] == lev);
+  if (!same) return false;
+#pragma unroll
+  for (int face = 0; face < 6; face++) {
+    const int dir = face >> 1, side = face & 1;
+    const int e = side ? 0 : B - 1;
+    const int src = dir == 0 ? e + B * u + B * B * w : (dir == 1 ? u + B * e + B * B * w : u + B * w + B * B * e);
+    const bool in = nb[face] >= 0;
+    fv[face] = X[(in ? off2[face] : off) + src];   // own block stands in outside the domain
+    fkind[face] = in ? FK_SAME : FK_DOMAIN;
+    foff[face] = in ? off2[face] : 0;
+  }
+  return true;
+}
+
+__device__ __forceinline__ float sameLevelFaceValue(float x, int kind, float h)
+{
+  return kind == FK_SAME ? x * h : 0.0f;
+}
+
 /* Workgroup barrier that orders LDS only: outstanding global loads stay in
  * flight across it (all cross-thread exchange in these kernels is via LDS). */
 __device__ __forceinline__ void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -367,6 +418,7 @@ struct ColCtx {
   int64_t fo1, fo3, fo5;
   float w6top, om, omm, HR;
   int h3;                   // B <= 16: the thread's cell at z = 0 in the halo array (Halo3)
+  int h3o;                  // B <= 16: h3 - o * (B+2)^2, its cell at z = hp - o is h3o + hp * (B+2)^2
   float fz0, fz1;           // B <= 16: the thread's -z and +z halo face values
 };
 
@@ -398,24 +450,63 @@ __device__ __forceinline__ void loadCoeffs(const ColCtx &c, int z, float &D, flo
   w6 = z < B - 1 ? co(2, q + B * B) : c.w6top;
 }
 
-/* Register-resident prologue: h*x, skip bits and the coefficients of every
- * slot of the thread's column, with the operand precision fixed. */
+/* loadCoeffs without branches: every load is issued whatever the face kinds
+ * (a cell of the own block stands in for a neighbour that is not read) and
+ * the face rules are applied to the loaded values, so a run of these loads
+ * stays in flight and each value is waited for only where it is first used. */
 template <int B, int P>
-__device__ __forceinline__ uint32_t loadColumn(ColRegs<B, true> &r, const ColCtx &c, const float *__restrict__ X,
-                                               float h)
+__device__ __forceinline__ void loadCoeffsFlat(const ColCtx &c, int z, float &D, float &Bb, float &w1, float &w2,
+                                               float &w3, float &w4, float &w5)
+{
+  const LevelArgs &a = *c.a;
+  const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
+  const bool x1same = c.tx < B - 1 || c.fk1 == FK_SAME, y1same = c.ty < B - 1 || c.fk3 == FK_SAME;
+  const int64_t q2 = c.tx < B - 1 ? q + 1 : (c.fk1 == FK_SAME ? c.fo1 + B * c.ty + B * B * z : q);
+  const int64_t q4 = c.ty < B - 1 ? q + B : (c.fk3 == FK_SAME ? c.fo3 + c.tx + B * B * z : q);
+  D = smCoefP<P>(a, 3, q);
+  Bb = c.Bv[q];
+  const float v1 = smCoefP<P>(a, 0, q), v2 = smCoefP<P>(a, 0, q2);
+  const float v3 = smCoefP<P>(a, 1, q), v4 = smCoefP<P>(a, 1, q4);
+  const float v5 = smCoefP<P>(a, 2, q);
+  w1 = (c.tx == 0 && c.mixed0) ? 1.0f : v1;
+  w3 = (c.ty == 0 && c.mixed2) ? 1.0f : v3;
+  w5 = (z == 0 && c.mixed4) ? 1.0f : v5;
+  w2 = x1same ? v2 : (c.fk1 == FK_DOMAIN ? a.bcW[0] : 1.0f);
+  w4 = y1same ? v4 : (c.fk3 == FK_DOMAIN ? a.bcW[1] : 1.0f);
+}
+
+/* Register-resident prologue: h*x, skip bits and the coefficients of every
+ * slot of the thread's column plus the +z face coefficient, with the operand
+ * precision fixed.  Flags and x of all slots are loaded first, then the
+ * coefficients, with no branch between the loads and nothing that waits for
+ * one before the last is issued: the column costs about one memory round
+ * trip, not one per slot. */
+template <int B, int P>
+__device__ __forceinline__ uint32_t loadColumn(ColRegs<B, true> &r, ColCtx &c, const float *__restrict__ X, float h)
 {
   constexpr int S = Cfg<B>::S;
   const LevelArgs &a = *c.a;
-  uint32_t skip = 0;
+  uint16_t fl[B];
 #pragma unroll
   for (int sl = 0; sl < B; sl++) {
     const int z = (sl - c.o + 2 * B) & (B - 1);
     const int64_t q = c.off + c.tx + B * c.ty + B * B * z;
-    const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
+    fl[sl] = a.flags[q];
+    r.xs[sl] = X[q];
+  }
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    const int z = (sl - c.o + 2 * B) & (B - 1);
+    loadCoeffsFlat<B, P>(c, z, r.cd[sl], r.cb[sl], r.w1[sl], r.w2[sl], r.w3[sl], r.w4[sl], r.w5[sl]);
+  }
+  const float w6v = smCoefP<P>(a, 2, (c.fk5 == FK_SAME ? c.fo5 : c.off) + c.tx + B * c.ty);
+  c.w6top = c.fk5 == FK_SAME ? w6v : (c.fk5 == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  uint32_t skip = 0;
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    const bool sk = segmentAllNonFluid<S>((fl[sl] & (CELL_SOLID | CELL_VOID)) != 0);
     skip |= (sk ? 1u : 0u) << sl;
-    r.xs[sl] = sk ? 0.0f : X[q] * h;
-    float w6;
-    loadCoeffs<B, P>(c, z, r.cd[sl], r.cb[sl], r.w1[sl], r.w2[sl], r.w3[sl], r.w4[sl], r.w5[sl], w6);
+    r.xs[sl] = sk ? 0.0f : r.xs[sl] * h;
   }
   return skip;
 }
@@ -531,7 +622,7 @@ __device__ __forceinline__ void colStepH(ColRegs<B, true> &r, uint32_t skip, flo
   const int z = hp - c.o;
   const bool active = z >= 0 && z < B && !((skip >> sl) & 1u);
   if (active) {
-    float *p = h3 + c.h3 + z * P2;
+    float *p = h3 + c.h3o + hp * P2;   // cell z = hp - o; hp * P2 is uniform
     const float F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
     const float F0 = r.xs[sl];
     const float F5 = z > 0 ? r.xs[slM] : c.fz0;
@@ -634,7 +725,10 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   const float h = (float)(1 << lev);
   int fkind[6];
   int64_t foff[6];
-  stageFaces<B>(faces, a, bid, lev, h, off, X, fkind, foff);
+  float fv[6];            // B <= 16, same-level faces: x of this thread's face cells
+  bool fast = false;
+  if constexpr (REGC) fast = sameLevelFaces<B>(a, bid, lev, h, off, X, fv, fkind, foff);
+  if (!fast) stageFaces<B>(faces, a, bid, lev, h, off, X, fkind, foff);
 
   ColCtx c;
   c.a = &a; c.Bv = Bv; c.Xg = X; c.faces = faces; c.off = off;
@@ -644,7 +738,6 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
   c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
   c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
-  c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
   if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
   c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
 
@@ -655,9 +748,23 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
     else if (a.prec == MSBGX_PREC_BF16) skip = loadColumn<B, MSBGX_PREC_BF16>(r, c, X, h);
     else skip = loadColumn<B, MSBGX_PREC_FP32>(r, c, X, h);
     c.h3 = (c.ty + 1) * P + c.tx + 1;
+    c.h3o = c.h3 - c.o * P2;
 #pragma unroll
     for (int sl = 0; sl < B; sl++) h3[c.h3 + ((sl - c.o + 2 * B) & (B - 1)) * P2] = r.xs[sl];
+    if (fast) {
+      /* x/y halo faces into the ring of the halo array: thread (u, w) = (tx, ty)
+       * holds face value u + B*w of each face (u along the face, w = z) */
+      const int u = c.tx, w = c.ty;
+      h3[w * P2 + (u + 1) * P] = sameLevelFaceValue(fv[0], fkind[0], h);
+      h3[w * P2 + (u + 1) * P + B + 1] = sameLevelFaceValue(fv[1], fkind[1], h);
+      h3[w * P2 + u + 1] = sameLevelFaceValue(fv[2], fkind[2], h);
+      h3[w * P2 + (B + 1) * P + u + 1] = sameLevelFaceValue(fv[3], fkind[3], h);
+      c.fz0 = sameLevelFaceValue(fv[4], fkind[4], h);
+      c.fz1 = sameLevelFaceValue(fv[5], fkind[5], h);
+    }
   } else {
+    c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty)
+                                  : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
 #pragma unroll
     for (int sl = 0; sl < B; sl++) {
       const int z = (sl - c.o + 2 * B) & (B - 1);
@@ -669,16 +776,16 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   }
   __syncthreads();
   if constexpr (REGC) {
-    /* x/y halo faces into the ring of the halo array: thread (u, w) = (tx, ty)
-     * moves face value u + B*w of each face (u along the face, w = z) */
-    const int u = c.tx, w = c.ty;
-    h3[w * P2 + (u + 1) * P] = faces[0 * NF + u + B * w];
-    h3[w * P2 + (u + 1) * P + B + 1] = faces[1 * NF + u + B * w];
-    h3[w * P2 + u + 1] = faces[2 * NF + u + B * w];
-    h3[w * P2 + (B + 1) * P + u + 1] = faces[3 * NF + u + B * w];
-    c.fz0 = faces[4 * NF + c.tx + B * c.ty];
-    c.fz1 = faces[5 * NF + c.tx + B * c.ty];
-    __syncthreads();
+    if (!fast) {
+      const int u = c.tx, w = c.ty;
+      h3[w * P2 + (u + 1) * P] = faces[0 * NF + u + B * w];
+      h3[w * P2 + (u + 1) * P + B + 1] = faces[1 * NF + u + B * w];
+      h3[w * P2 + u + 1] = faces[2 * NF + u + B * w];
+      h3[w * P2 + (B + 1) * P + u + 1] = faces[3 * NF + u + B * w];
+      c.fz0 = faces[4 * NF + c.tx + B * c.ty];
+      c.fz1 = faces[5 * NF + c.tx + B * c.ty];
+      __syncthreads();
+    }
     /* sweep k is reversed iff reverse == (k & 1) (src/msbg3.cpp:1157) */
     if (reverse) {
 #pragma unroll 1
