@@ -111,16 +111,19 @@ def gpu_run(msbgx, cfg, args, log):
 
 def pmc_calibration(msbgx, cfg, log):
     """Workload for the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: one copy
-    of the MG0 fluid blocks (k_copy_blocks: reads and writes 4 B per cell) and
-    ten smoother colour passes over the largest same-size block list."""
+    of the MG0 fluid blocks (k_copy_blocks: reads and writes 4 B per cell), ten
+    smoother colour passes over the largest same-size block list and ten
+    residual launches (k_multiply) over the level-0 fluid blocks."""
     g = msbgx.build_grid(cfg)
     g.prepare_pressure_solve_flip(scene=cfg.scene)
     g.copy_channel(msbgx.CH_DIVERGENCE, msbgx.CH_R)
     ms, by, cells = g.time_kernel(0, reps=10)
+    ms_res, by_res, _ = g.time_kernel(1, reps=10)
     lv, fl = g.block_info(0)
     copy_cells = int(sum((cfg.bsx >> int(l)) ** 3 for l, f in zip(lv, fl) if not f & msbgx.BLK_NO_FLUID))
     log(f"pmc calibration: copy of {copy_cells} cells ({copy_cells * 4} B read, {copy_cells * 4} B written); "
-        f"smoother pass: {cells} cells, {by:.0f} algorithmic bytes, {ms:.4f} ms")
+        f"smoother pass: {cells} cells, {by:.0f} algorithmic bytes, {ms:.4f} ms; "
+        f"residual: {by_res:.0f} algorithmic bytes, {ms_res:.4f} ms")
 
 
 def precision_runs(msbgx, g, cfg, args, log):

@@ -867,6 +867,107 @@ __device__ __forceinline__ double blockReduceSum(double v, double *scratch)
   return r;
 }
 
+/* k_multiply for B <= 16: thread (x, y) computes its z-column.  Every load of
+ * the column (flags, x, the coefficients of the cell and of its +x/+y/+z
+ * neighbours, b) is issued before any is waited for; h*x goes into the halo
+ * array (Halo3) for the in-plane neighbours and stays in registers for the z
+ * neighbours.  Cells, operation order and the FP64 partial are those of
+ * cellStencil and the B = 32 loop (src/msbg3.cpp:1192-1335, 1282-1321). */
+template <int B>
+__device__ __forceinline__ double multiplyColumn(const LevelArgs &a, int bid, int lev, float h, int64_t off,
+                                                 const float *__restrict__ X, const float *__restrict__ Bv,
+                                                 float *__restrict__ Y, int residual, float *h3, float *faces,
+                                                 int *fkind, int64_t *foff)
+{
+  constexpr int S = Cfg<B>::S, NF = Cfg<B>::NF, P = Halo3<B>::P, P2 = Halo3<B>::P2;
+  const int x = threadIdx.x % B, y = threadIdx.x / B;
+  float fv[6];
+  const bool fast = sameLevelFaces<B>(a, bid, lev, h, off, X, fv, fkind, foff);
+  if (!fast) stageFaces<B>(faces, a, bid, lev, h, off, X, fkind, foff);
+  const int64_t q0 = off + x + B * y;
+  const float *bsrc = residual ? Bv : X;   // X stands in when b is not read
+  uint16_t fl[B];
+  float xv[B], w0[B], w1[B], w2[B], dd[B], w0x[B], w1y[B], bb[B];
+#pragma unroll
+  for (int z = 0; z < B; z++) {
+    fl[z] = a.flags[q0 + B * B * z];
+    xv[z] = X[q0 + B * B * z];
+  }
+#pragma unroll
+  for (int z = 0; z < B; z++) {
+    const int64_t q = q0 + B * B * z;
+    w0[z] = a.W0[q];
+    w1[z] = a.W1[q];
+    w2[z] = a.W2[q];
+    dd[z] = a.D[q];
+    w0x[z] = a.W0[x < B - 1 ? q + 1 : (fkind[1] == FK_SAME ? foff[1] + B * y + B * B * z : q)];
+    w1y[z] = a.W1[y < B - 1 ? q + B : (fkind[3] == FK_SAME ? foff[3] + x + B * B * z : q)];
+    bb[z] = bsrc[q];
+  }
+  const float w6t = a.W2[fkind[5] == FK_SAME ? foff[5] + x + B * y : q0];
+  uint32_t skip = 0;
+  const int hc = (y + 1) * P + x + 1;
+#pragma unroll
+  for (int z = 0; z < B; z++) {
+    const bool sk = segmentAllNonFluid<S>((fl[z] & (CELL_SOLID | CELL_VOID)) != 0);
+    skip |= (sk ? 1u : 0u) << z;
+    xv[z] = sk ? 0.0f : xv[z] * h;
+    h3[hc + z * P2] = xv[z];
+  }
+  float fz0, fz1;
+  if (fast) {
+    h3[y * P2 + (x + 1) * P] = sameLevelFaceValue(fv[0], fkind[0], h);
+    h3[y * P2 + (x + 1) * P + B + 1] = sameLevelFaceValue(fv[1], fkind[1], h);
+    h3[y * P2 + x + 1] = sameLevelFaceValue(fv[2], fkind[2], h);
+    h3[y * P2 + (B + 1) * P + x + 1] = sameLevelFaceValue(fv[3], fkind[3], h);
+    fz0 = sameLevelFaceValue(fv[4], fkind[4], h);
+    fz1 = sameLevelFaceValue(fv[5], fkind[5], h);
+  }
+  __syncthreads();
+  if (!fast) {
+    h3[y * P2 + (x + 1) * P] = faces[0 * NF + x + B * y];
+    h3[y * P2 + (x + 1) * P + B + 1] = faces[1 * NF + x + B * y];
+    h3[y * P2 + x + 1] = faces[2 * NF + x + B * y];
+    h3[y * P2 + (B + 1) * P + x + 1] = faces[3 * NF + x + B * y];
+    fz0 = faces[4 * NF + x + B * y];
+    fz1 = faces[5 * NF + x + B * y];
+    __syncthreads();
+  }
+  const bool m0 = x == 0 && fkind[0] >= FK_COARSER, m2 = y == 0 && fkind[2] >= FK_COARSER;
+  const bool m4 = fkind[4] >= FK_COARSER;
+  const float xOut = fkind[1] == FK_DOMAIN ? a.bcW[0] : 1.0f, yOut = fkind[3] == FK_DOMAIN ? a.bcW[1] : 1.0f;
+  const float zOut = fkind[5] == FK_SAME ? w6t : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  const bool xIn = x < B - 1 || fkind[1] == FK_SAME, yIn = y < B - 1 || fkind[3] == FK_SAME;
+  double acc = 0.0;
+#pragma unroll
+  for (int z = 0; z < B; z++) {
+    if ((skip >> z) & 1u) continue;
+    const float *p = h3 + hc + z * P2;
+    const float F0 = xv[z];
+    const float F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F5 = z > 0 ? xv[z > 0 ? z - 1 : 0] : fz0;
+    const float F6 = z < B - 1 ? xv[z < B - 1 ? z + 1 : 0] : fz1;
+    const float W1 = m0 ? 1.0f : w0[z];
+    const float W3 = m2 ? 1.0f : w1[z];
+    const float W5 = (z == 0 && m4) ? 1.0f : w2[z];
+    const float W2 = xIn ? w0x[z] : xOut;
+    const float W4 = yIn ? w1y[z] : yOut;
+    const float W6 = z < B - 1 ? w2[z < B - 1 ? z + 1 : 0] : zOut;
+    float s = F1 * W1;
+    s = fmaf(F2, W2, s);
+    s = fmaf(F3, W3, s);
+    s = fmaf(F4, W4, s);
+    s = fmaf(F5, W5, s);
+    s = fmaf(F6, W6, s);
+    const float D = dd[z];
+    float r = D != 0.0f ? fmaf(1.0f / D, F0, -s) : 0.0f;
+    if (residual) r = bb[z] - r;
+    Y[q0 + B * B * z] = r;
+    acc += (double)F0 * (double)r;
+  }
+  return acc;
+}
+
 /* y = A x (or r = b - A x); non-fluid segments are not written. */
 template <int B>
 __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int *__restrict__ blocks,
@@ -875,19 +976,25 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
                                                          int partialBase, const int *done)
 {
   if (*done) return;
-  __shared__ float tile[Cfg<B>::NIN + 6 * Cfg<B>::NF];
+  constexpr bool COL = B <= 16;
+  __shared__ float tile[COL ? 1 : Cfg<B>::NIN + 6 * Cfg<B>::NF];
+  __shared__ float h3[COL ? Halo3<B>::N : 1];
+  __shared__ float faces[COL ? 6 * Cfg<B>::NF : 1];
   __shared__ double red[16];
   const int bid = blocks[blockIdx.x];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
   const float h = (float)(1 << lev);
-  uint32_t skipmask;
   int fkind[6];
   int64_t foff[6];
+  double acc = 0.0;
+  if constexpr (COL) {
+    acc = multiplyColumn<B>(a, bid, lev, h, off, X, Bv, Y, residual, h3, faces, fkind, foff);
+  } else {
+  uint32_t skipmask;
   stageTile<B>(tile, skipmask, a, bid, lev, h, off, X, fkind, foff);
   __syncthreads();
   const int x = threadIdx.x % B, y = threadIdx.x / B;
-  double acc = 0.0;
   for (int z = 0; z < B; z++) {
     if ((skipmask >> z) & 1u) continue;
     float F0, Sm, D;
@@ -897,6 +1004,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
     if (residual) r = Bv[c] - r;
     Y[c] = r;
     acc += (double)F0 * (double)r;
+  }
   }
   if (partials) {
     const double s = blockReduceSum(lev == 0 ? acc : 0.0, red);
