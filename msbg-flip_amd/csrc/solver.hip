@@ -5,12 +5,17 @@
  * prolongation, the V-cycle and the MG-preconditioned CG.
  *
  * Kernel map (one workgroup per sparse block unless noted):
- *   k_relax<B>      red-black block relaxation pass; the block and its
- *                   one-cell halo live in LDS; 5 local sweeps, the first four
- *                   as hyperplane wavefronts, the last a Jacobi pass
- *                   (src/msbg3.cpp:419-1506, 2171-2240)
- *   k_multiply<B>   y = A x or r = b - A x, optional FP64 partial of <x, y>
- *                   (src/msbg3.cpp:1282-1321, 1757-1912)
+ *   k_relax_col<B>  red-black block relaxation pass (the default): thread
+ *                   (x, y) keeps its z-column and coefficients in registers;
+ *                   5 local sweeps, the first four as hyperplane wavefronts,
+ *                   the last a Jacobi pass; for B <= 16 the block and its
+ *                   x/y halo ring stay in LDS (src/msbg3.cpp:419-1506,
+ *                   2171-2240)
+ *   k_relax<B>      the same pass with the whole block tile in LDS (an
+ *                   autotuner alternative, FP32 operands)
+ *   k_multiply<B>   y = A x or r = b - A x, optional FP64 partial of <x, y>;
+ *                   column form for B <= 16 (src/msbg3.cpp:1282-1321,
+ *                   1757-1912)
  *   k_restrict      2x2x2 mean / block copy / zero (src/msbg4.cpp:1513-1915)
  *   k_prolong       piecewise-constant copy-add (src/msbg4.cpp:1921-2029)
  *   k_update_xr, k_dot, k_axpy, k_finalize_*: PCG vector work with FP64
