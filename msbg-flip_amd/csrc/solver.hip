@@ -380,6 +380,18 @@ __device__ __forceinline__ float sameLevelFaceValue(float x, int kind, float h)
   return kind == FK_SAME ? x * h : 0.0f;
 }
 
+/* List index of this workgroup, XCD-aware: workgroups are dispatched to the
+ * eight XCDs round robin (workgroup i on XCD i mod 8), so each XCD is given a
+ * contiguous run of the block list instead of every eighth entry -- lists are
+ * in block order, so spatial neighbours, whose halo faces a block reads, share
+ * that XCD's L2.  A bijection on [0, gridDim.x). */
+__device__ __forceinline__ int xcdListIndex()
+{
+  constexpr int NX = 8;
+  const int i = blockIdx.x, n = gridDim.x, q = n / NX, r = n % NX, x = i % NX;
+  return x * q + (x < r ? x : r) + i / NX;
+}
+
 /* Workgroup barrier that orders LDS only: outstanding global loads stay in
  * flight across it (all cross-thread exchange in these kernels is via LDS). */
 __device__ __forceinline__ void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -724,7 +736,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
   __shared__ float faces[6 * NF];
   __shared__ float pub[REGC ? 1 : 2][3][REGC ? 1 : NT];   // B = 32: double-buffered step values
   __shared__ float h3[REGC ? Halo3<B>::N : 1];            // B <= 16: block values with x/y halo ring
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
   const float h = (float)(1 << lev);
@@ -986,7 +998,8 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
   __shared__ float h3[COL ? Halo3<B>::N : 1];
   __shared__ float faces[COL ? 6 * Cfg<B>::NF : 1];
   __shared__ double red[16];
-  const int bid = blocks[blockIdx.x];
+  const int li = xcdListIndex();
+  const int bid = blocks[li];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
   const float h = (float)(1 << lev);
@@ -1013,7 +1026,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
   }
   if (partials) {
     const double s = blockReduceSum(lev == 0 ? acc : 0.0, red);
-    if (threadIdx.x == 0) partials[partialBase + blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[partialBase + li] = s;   // list order: the fixed summation order
   }
 }
 
