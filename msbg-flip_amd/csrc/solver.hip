@@ -128,6 +128,18 @@ enum FaceKind { FK_SAME = 0, FK_DOMAIN = 1, FK_COARSER = 2, FK_FINER = 3 };
 /* Stage the block (h*x, zero on all-non-fluid segments) and its six halo
  * faces into LDS (src/msbg3.cpp:564-1044).  Returns per-thread skip mask over
  * z and the face kinds/neighbour offsets used for the +face coefficients. */
+/* List index of this workgroup, XCD-aware: workgroups are dispatched to the
+ * eight XCDs round robin (workgroup i on XCD i mod 8), so each XCD is given a
+ * contiguous run of the block list instead of every eighth entry -- lists are
+ * in block order, so spatial neighbours, whose halo faces a block reads, share
+ * that XCD's L2.  A bijection on [0, gridDim.x). */
+__device__ __forceinline__ int xcdListIndex()
+{
+  constexpr int NX = 8;
+  const int i = blockIdx.x, n = gridDim.x, q = n / NX, r = n % NX, x = i % NX;
+  return x * q + (x < r ? x : r) + i / NX;
+}
+
 /* True when every lane of the caller's SIMD segment (S consecutive lanes)
  * reports a SOLID/VOID cell: such segments are skipped by the reference's
  * block kernel (src/msbg3.cpp:589-617, 1201-1209). */
@@ -291,7 +303,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
   if (*done) return;
   constexpr int S = Cfg<B>::S, T = Cfg<B>::T;
   __shared__ float tile[Cfg<B>::NIN + 6 * Cfg<B>::NF];
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
   const float h = (float)(1 << lev);
@@ -378,18 +390,6 @@ __device__ __forceinline__ bool sameLevelFaces(const LevelArgs &a, int bid, int 
 __device__ __forceinline__ float sameLevelFaceValue(float x, int kind, float h)
 {
   return kind == FK_SAME ? x * h : 0.0f;
-}
-
-/* List index of this workgroup, XCD-aware: workgroups are dispatched to the
- * eight XCDs round robin (workgroup i on XCD i mod 8), so each XCD is given a
- * contiguous run of the block list instead of every eighth entry -- lists are
- * in block order, so spatial neighbours, whose halo faces a block reads, share
- * that XCD's L2.  A bijection on [0, gridDim.x). */
-__device__ __forceinline__ int xcdListIndex()
-{
-  constexpr int NX = 8;
-  const int i = blockIdx.x, n = gridDim.x, q = n / NX, r = n % NX, x = i % NX;
-  return x * q + (x < r ? x : r) + i / NX;
 }
 
 /* Workgroup barrier that orders LDS only: outstanding global loads stay in
@@ -1040,7 +1040,7 @@ __global__ void k_restrict(const int *__restrict__ blocks, const int *__restrict
                            const float *__restrict__ src, float *__restrict__ dst, int copyMode, const int *done)
 {
   if (*done) return;
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int lev = levC[bid];
   const int b = bsx0 >> lev, n = b * b * b;
   const int64_t oc = offC[bid], of = offF[bid];
@@ -1069,7 +1069,7 @@ __global__ void k_prolong(const int *__restrict__ blocks, const int *__restrict_
                           const float *__restrict__ xc, const int *done)
 {
   if (*done) return;
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int lev = levF[bid];
   const int b = bsx0 >> lev, n = b * b * b;
   const int64_t of = offF[bid], oc = offC[bid];
@@ -1091,7 +1091,7 @@ __global__ void k_zero_blocks(const int *__restrict__ blocks, const int *__restr
                               int bsx0, float *__restrict__ x, const int *done)
 {
   if (*done) return;
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int b = bsx0 >> lev[bid], n = b * b * b;
   const int64_t o = off[bid];
   for (int v = threadIdx.x; v < n; v += blockDim.x) x[o + v] = 0.0f;
@@ -1101,7 +1101,7 @@ __global__ void k_copy_blocks(const int *__restrict__ blocks, const int *__restr
                               int bsx0, const float *__restrict__ s, float *__restrict__ d, const int *done)
 {
   if (*done) return;
-  const int bid = blocks[blockIdx.x];
+  const int bid = blocks[xcdListIndex()];
   const int b = bsx0 >> lev[bid], n = b * b * b;
   const int64_t o = off[bid];
   for (int v = threadIdx.x; v < n; v += blockDim.x) d[o + v] = s[o + v];
