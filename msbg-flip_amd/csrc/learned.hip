@@ -718,13 +718,20 @@ __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, in
 }
 
 /* Scale the correction by the Galerkin step gamma over the whole MG0 pool:
- * zc *= gamma and rs = r - gamma A zc = (1 - gamma) r + gamma rs. */
+ * zc *= gamma and rs = r - gamma A zc = (1 - gamma) r + gamma rs.  With the
+ * network off (plain iterations, fallback) rs = r exactly, so the one V-cycle
+ * of the iteration, which reads rs, gives the plain MG-PCG's z. */
 __global__ void k_nn_gamma(float *__restrict__ zc, float *__restrict__ rs, const float *__restrict__ r, int64_t n,
                            const PcgState *st)
 {
-  if (st->skipNet) return;
+  if (st->done) return;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
+  if (st->skipNet) {
+    for (int64_t i = i0; i < n; i += di) rs[i] = r[i];
+    return;
+  }
   const float gm = st->gamma;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = i0; i < n; i += di) {
     zc[i] = gm * zc[i];
     rs[i] = fmaf(gm, rs[i] - r[i], r[i]);
   }
@@ -1292,8 +1299,8 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   hipLaunchKernelGGL(k_nn_gamma, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, c.rs, m0.d_ch[C_R],
                      (int64_t)g.mg[0].ncells, d.state);
   std::swap(m0.d_ch[C_R], c.rs);   // the V-cycle below reads its right-hand side from rs
-  zeroActive(g, 0, m0.d_ch[C_Z], skip, s);
-  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, skip, s);
+  zeroActive(g, 0, m0.d_ch[C_Z], &d.state->done, s);
+  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->done, s);
   std::swap(m0.d_ch[C_R], c.rs);
   hipLaunchKernelGGL(k_nn_add, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], c.zc, c.n0, skip);
 }
@@ -1304,7 +1311,6 @@ static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbg
   MgLevel &m0 = g.mg[0];
   const int *done = &d.state->done;
   const int *skip = &d.state->skipNet;
-  const int *skipPlain = &d.state->skipPlain;
   const int64_t n0 = c.n0;
   const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
   finalize(d, nPart, FIN_ALPHA, nullptr);
@@ -1312,9 +1318,7 @@ static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbg
                      m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
   finalize(d, kVecBlocks, FIN_CONV, d.hist);
   hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zprev, n0, skip, 0);
-  coarseLearnedPath(g, n, c, p, d.stream);
-  zeroActive(g, 0, m0.d_ch[C_Z], skipPlain, d.stream);
-  vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, skipPlain, d.stream);
+  coarseLearnedPath(g, n, c, p, d.stream);   // its V-cycle is the plain one while the network is off
   hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zprev, n0, d.partials,
                      skip);
   finalize(d, kVecBlocks, FIN_RZOLD, nullptr);
@@ -1402,9 +1406,7 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
   const int *done = &d.state->done;
   const int *skip = &d.state->skipNet;
   if (coarse) {
-    coarseLearnedPath(g, n, c, p, d.stream);
-    zeroActive(g, 0, m0.d_ch[C_Z], &d.state->skipPlain, d.stream);
-    vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->skipPlain, d.stream);
+    coarseLearnedPath(g, n, c, p, d.stream);   // its V-cycle is the plain one while the network is off
   } else {
     zeroActive(g, 0, m0.d_ch[C_Z], done, d.stream);
     vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, done, d.stream);

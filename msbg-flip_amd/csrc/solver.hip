@@ -103,7 +103,7 @@ struct PcgState {
    * the plain V-cycle output (usePlain); r.z_old for the flexible beta and
    * r.V(r) for the fallback; |r|^2 of the last kStallWindow iterations */
   int learned, restart, bad, skipNet, fellBack, usePlain;
-  int coarseMode, skipPlain;   // coarse-correction form; kernels of the plain-only path skip while set
+  int coarseMode, pad5;        // coarse-correction form
   int stallWin, pad3;          // stall window (0: no stall test), at most kMaxStallWindow
   double stallFactor;          // |r|^2 must fall below stallFactor * |r|^2 of stallWin iterations ago
   double rrBest;               // smallest |r|^2 before the stall window
@@ -1297,7 +1297,6 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
     case FIN_FINAL: if (out) out[0] = s; break;
   }
   st->skipNet = (st->done || !st->learned) ? 1 : 0;
-  st->skipPlain = (st->done || st->learned) ? 1 : 0;
 }
 
 /* Divergence right-hand side of level-0 blocks from the scene's MAC velocity
