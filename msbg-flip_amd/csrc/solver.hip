@@ -992,16 +992,21 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
                                                          float *__restrict__ Y, int residual, double *partials,
                                                          int partialBase, const int *done)
 {
-  if (*done) return;
   constexpr bool COL = B <= 16;
   __shared__ float tile[COL ? 1 : Cfg<B>::NIN + 6 * Cfg<B>::NF];
   __shared__ float h3[COL ? Halo3<B>::N : 1];
   __shared__ float faces[COL ? 6 * Cfg<B>::NF : 1];
   __shared__ double red[16];
   const int li = xcdListIndex();
+  /* the block's descriptor is read before the done flag is tested (the empty
+   * asm keeps the test after the loads), so that the flag's load joins the
+   * descriptor's instead of preceding it */
   const int bid = blocks[li];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
+  const int isDone = *done;
+  asm volatile("" ::"s"(lev), "s"(isDone));
+  if (isDone) return;
   const float h = (float)(1 << lev);
   int fkind[6];
   int64_t foff[6];
