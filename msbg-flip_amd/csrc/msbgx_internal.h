@@ -39,6 +39,16 @@ struct DevList {
   int n = 0;
 };
 
+/* Face neighbours of a block at one MG level (0..5 = -x, +x, -y, +y, -z, +z):
+ * their slot offsets and levels, level -1 outside the domain.  Built on the
+ * host from the raster or the batched neighbour table, so a kernel reads a
+ * block's halo topology with one load. */
+struct NbDesc {
+  int64_t off[6];
+  int lev[6];
+  int pad[4];
+};
+
 struct MgLevel {
   /* host topology */
   std::vector<int> level;       // block level at this MG level = max(map, L)
@@ -54,6 +64,7 @@ struct MgLevel {
   /* device */
   int *d_level = nullptr;
   int64_t *d_off = nullptr;
+  NbDesc *d_nbd = nullptr;      // per block: face neighbours' offsets and levels
   uint16_t *d_flags = nullptr;
   float *d_W[3] = {nullptr, nullptr, nullptr};
   float *d_D = nullptr;

@@ -59,7 +59,7 @@ struct LevelArgs {
   const uint16_t *H[4];
   int prec;      // MSBGX_PREC_FP32 / FP16 / BF16
   /* batched block sets (several independent problems in one block set) */
-  const int *nbr;        // 6 face neighbours per block, -1 = domain boundary; null: raster arithmetic
+  const NbDesc *nbd;     // per block: face neighbours' offsets and levels at this MG level
   const int *probOf;     // problem of each block
   const float *omegaOf;  // smoother weight of each problem
   /* MG0 with 32^3 level-0 blocks: per-cell coefficient records of the column
@@ -151,20 +151,6 @@ __device__ __forceinline__ bool segmentAllNonFluid(bool nonFluid)
   return segBits == ((1ull << S) - 1);
 }
 
-/* Face neighbour of block bid across face (0..5 = -x, +x, -y, +y, -z, +z),
- * -1 outside the domain. */
-__device__ __forceinline__ int faceNeighbour(const LevelArgs &a, int bid, int face)
-{
-  if (a.nbr) return a.nbr[6 * bid + face];   // batched block sets: explicit face-neighbour table
-  const int dir = face >> 1, side = face & 1;
-  int nbxx = bid % a.nbx, nbyy = (bid / a.nbx) % a.nby, nbzz = bid / (a.nbx * a.nby);
-  if (dir == 0) nbxx += side ? 1 : -1;
-  else if (dir == 1) nbyy += side ? 1 : -1;
-  else nbzz += side ? 1 : -1;
-  return (nbxx < 0 || nbyy < 0 || nbzz < 0 || nbxx >= a.nbx || nbyy >= a.nby || nbzz >= a.nbz)
-             ? -1 : nbxx + a.nbx * (nbyy + a.nby * nbzz);
-}
-
 /* Halo faces of a block into faces[6][B*B] (h*x of same-level neighbours,
  * mixed-resolution aggregates, 0 at the domain boundary) plus per-face kinds
  * and neighbour slot offsets (src/msbg3.cpp:665-1044, 46-161). */
@@ -181,15 +167,14 @@ __device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int
 #pragma unroll
   for (int face = 0; face < 6; face++) {
     const int dir = face >> 1, side = face & 1;
-    const int nb = faceNeighbour(a, bid, face);
+    const int lev2 = a.nbd[bid].lev[face];
     float val = 0.0f;
     int kind;
     int64_t off2 = 0;
-    if (nb < 0) {
+    if (lev2 < 0) {
       kind = FK_DOMAIN;
     } else {
-      const int lev2 = a.level[nb];
-      off2 = a.off[nb];
+      off2 = a.nbd[bid].off[face];
       if (lev2 == lev) {
         kind = FK_SAME;
         const int e = side ? 0 : B - 1;
@@ -349,7 +334,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
  * neighbours all have its own level or lie outside the domain (every block
  * of a single-level grid, the interior of a level): the six neighbour levels
  * and offsets are read in one group and then the six face values in one
- * group, instead of one dependent round trip per face.  Thread (u, w) =
+ * group (the block's NbDesc), instead of one dependent round trip per face.  Thread (u, w) =
  * (tx, ty) gets x of face cell u + B*w of each face; the face value stageFaces
  * writes is that times h for a same-level face and 0 at the domain boundary
  * (sameLevelFaceValue), applied only where the value is used so that the
@@ -360,26 +345,24 @@ __device__ __forceinline__ bool sameLevelFaces(const LevelArgs &a, int bid, int 
                                                const float *__restrict__ X, float *fv, int *fkind, int64_t *foff)
 {
   const int u = threadIdx.x % B, w = threadIdx.x / B;
-  int nb[6], lev2[6];
+  const NbDesc &nd = a.nbd[bid];
+  int lev2[6];
   int64_t off2[6];
 #pragma unroll
-  for (int face = 0; face < 6; face++) nb[face] = faceNeighbour(a, bid, face);
-#pragma unroll
   for (int face = 0; face < 6; face++) {
-    const int n = nb[face] < 0 ? bid : nb[face];
-    lev2[face] = a.level[n];
-    off2[face] = a.off[n];
+    lev2[face] = nd.lev[face];
+    off2[face] = nd.off[face];
   }
   bool same = true;
 #pragma unroll
-  for (int face = 0; face < 6; face++) same = same && (nb[face] < 0 || lev2[face] == lev);
+  for (int face = 0; face < 6; face++) same = same && (lev2[face] < 0 || lev2[face] == lev);
   if (!same) return false;
 #pragma unroll
   for (int face = 0; face < 6; face++) {
     const int dir = face >> 1, side = face & 1;
     const int e = side ? 0 : B - 1;
     const int src = dir == 0 ? e + B * u + B * B * w : (dir == 1 ? u + B * e + B * B * w : u + B * w + B * B * e);
-    const bool in = nb[face] >= 0;
+    const bool in = lev2[face] >= 0;
     fv[face] = X[(in ? off2[face] : off) + src];   // own block stands in outside the domain
     fkind[face] = in ? FK_SAME : FK_DOMAIN;
     foff[face] = in ? off2[face] : 0;
@@ -1366,7 +1349,7 @@ struct Dev {
   int64_t refCap = 0;
   bool b0Valid = false;
   /* batched block sets */
-  int *nbr = nullptr, *probOf = nullptr;
+  int *probOf = nullptr;
   float *omegaOf = nullptr;
   BatchDev *batch = nullptr;
   LearnedCtx *learned = nullptr;   // buffers and graph of the learned-correction solve
@@ -1405,11 +1388,12 @@ template <typename T> static int dalloc(T **p, size_t n)
 static void freeLevel(MgLevel &m)
 {
   auto f = [](void *p) { if (p) (void)hipFree(p); };
-  f(m.d_level); f(m.d_off); f(m.d_flags); f(m.d_D); f(m.d_lists);
+  f(m.d_level); f(m.d_off); f(m.d_nbd); f(m.d_flags); f(m.d_D); f(m.d_lists);
   for (int d = 0; d < 3; d++) { f(m.d_W[d]); m.d_W[d] = nullptr; }
   for (int k = 0; k < 4; k++) { f(m.d_H[k]); m.d_H[k] = nullptr; }
   for (int c = 0; c < N_VEC; c++) { f(m.d_ch[c]); m.d_ch[c] = nullptr; }
-  m.d_level = nullptr; m.d_off = nullptr; m.d_flags = nullptr; m.d_D = nullptr; m.d_lists = nullptr;
+  m.d_level = nullptr; m.d_off = nullptr; m.d_nbd = nullptr; m.d_flags = nullptr; m.d_D = nullptr;
+  m.d_lists = nullptr;
 }
 
 void deviceFree(Grid &g)
@@ -1426,7 +1410,7 @@ void deviceFree(Grid &g)
     if (d.b0) (void)hipFree(d.b0);
     if (d.dActive0) (void)hipFree(d.dActive0);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
-    f(d.nbr); f(d.probOf); f(d.omegaOf);
+    f(d.probOf); f(d.omegaOf);
     freeLearned(d.learned);
     freeFlip(d);
     if (d.pack32) (void)hipFree(d.pack32);
@@ -1455,7 +1439,7 @@ static LevelArgs levelArgs(const Grid &g, int L)
   for (int k = 0; k < 4; k++) a.H[k] = m.d_H[k];
   a.prec = (g.prec != MSBGX_PREC_FP32 && m.d_H[0]) ? g.prec : MSBGX_PREC_FP32;
   const Dev *dd = (const Dev *)g.dev;
-  a.nbr = dd ? dd->nbr : nullptr;
+  a.nbd = m.d_nbd;
   a.probOf = dd ? dd->probOf : nullptr;
   a.omegaOf = dd ? dd->omegaOf : nullptr;
   a.pack = (L == 0 && g.bsx0 == 32 && dd) ? dd->pack32 : nullptr;
@@ -1526,6 +1510,20 @@ static int buildPack32(Grid &g)
   return MSBGX_OK;
 }
 
+/* Face neighbour of block bid across face (0..5 = -x, +x, -y, +y, -z, +z), -1
+ * outside the domain: the batched neighbour table where the block set has
+ * one, else the raster. */
+static int hostFaceNeighbour(const Grid &g, int bid, int face)
+{
+  if (!g.nbrTable.empty()) return g.nbrTable[(size_t)bid * 6 + face];
+  const int dir = face >> 1, side = face & 1;
+  int x = bid % g.nbx, y = (bid / g.nbx) % g.nby, z = bid / (g.nbx * g.nby);
+  if (dir == 0) x += side ? 1 : -1;
+  else if (dir == 1) y += side ? 1 : -1;
+  else z += side ? 1 : -1;
+  return (x < 0 || y < 0 || z < 0 || x >= g.nbx || y >= g.nby || z >= g.nbz) ? -1 : x + g.nbx * (y + g.nby * z);
+}
+
 int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scene)
 {
   if (!g.dev) {
@@ -1560,6 +1558,17 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
     if (int rc = dalloc(&m.d_D, nc)) return rc;
     HIPCHK(hipMemcpy(m.d_level, m.level.data(), nb * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.d_off, m.off.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
+    {
+      std::vector<NbDesc> nd(nb);
+      for (int bid = 0; bid < g.nBlocks; bid++)
+        for (int face = 0; face < 6; face++) {
+          const int n = hostFaceNeighbour(g, bid, face);
+          nd[bid].lev[face] = n < 0 ? -1 : m.level[n];
+          nd[bid].off[face] = n < 0 ? 0 : m.off[n];
+        }
+      if (int rc = dalloc(&m.d_nbd, nb)) return rc;
+      HIPCHK(hipMemcpy(m.d_nbd, nd.data(), nb * sizeof(NbDesc), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(m.d_flags, m.flags.data(), nc * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.d_D, m.D.data(), nc * sizeof(float), hipMemcpyHostToDevice));
     for (int k = 0; k < 3; k++) {
@@ -2276,13 +2285,10 @@ int deviceUploadBatch(Grid &m, const std::vector<const Grid *> &probs)
   if (int rc = deviceUpload(m, noRhs, nullptr)) return rc;
   Dev &d = dev(m);
   const int nb = m.nBlocks;
-  if (int rc = dalloc(&d.nbr, (size_t)nb * 6)) return rc;
   if (int rc = dalloc(&d.probOf, nb)) return rc;
   if (int rc = dalloc(&d.omegaOf, m.nProb)) return rc;
-  HIPCHK(hipMemcpy(d.nbr, m.nbrTable.data(), (size_t)nb * 6 * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.probOf, m.probOf.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d.omegaOf, m.probOmega.data(), m.nProb * sizeof(float), hipMemcpyHostToDevice));
-  if (int rc = buildPack32(m)) return rc;   // again, with the explicit neighbour table
   /* right-hand sides: scene problems on the device, others from their host copy */
   MgLevel &m0 = m.mg[0];
   std::vector<float> hostB;
