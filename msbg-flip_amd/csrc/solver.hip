@@ -712,16 +712,21 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax_col(LevelArgs a, const int
                                                           float *__restrict__ X, const float *__restrict__ Bv,
                                                           float omega, int reverse, const int *done)
 {
-  if (*done) return;
   constexpr bool REGC = B <= 16;
   constexpr int S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
   constexpr int P = Halo3<B>::P, P2 = Halo3<B>::P2;
   __shared__ float faces[6 * NF];
   __shared__ float pub[REGC ? 1 : 2][3][REGC ? 1 : NT];   // B = 32: double-buffered step values
   __shared__ float h3[REGC ? Halo3<B>::N : 1];            // B <= 16: block values with x/y halo ring
+  /* the block's descriptor is read before the done flag is tested (the empty
+   * asm keeps the test after the loads), so that the flag's load joins the
+   * descriptor's instead of preceding it */
   const int bid = blocks[xcdListIndex()];
   const int lev = a.level[bid];
   const int64_t off = a.off[bid];
+  const int isDone = *done;
+  asm volatile("" ::"s"(lev), "s"(isDone));
+  if (isDone) return;
   const float h = (float)(1 << lev);
   int fkind[6];
   int64_t foff[6];
