@@ -276,49 +276,50 @@ __global__ void __launch_bounds__(256) k_p2g_gather(P2gArgs a, const int *__rest
     /* one pass over the 27 cells around the cell in (z, y, x) order: each
      * particle adds to every face whose stencil it reaches, so each face's
      * sums run over its own 18 source cells in (z, y, x) order, then by
-     * particle, as three separate 18-cell passes would */
+     * particle, as three separate 18-cell passes would.  Bins are x-fastest,
+     * so the three cells of one (z, y) row hold one contiguous particle range,
+     * visited as one loop in the same order. */
     float sv[3] = {0.0f, 0.0f, 0.0f}, sw[3] = {0.0f, 0.0f, 0.0f}, sr[3] = {0.0f, 0.0f, 0.0f};
     for (int cz = f[2] - 1; cz <= f[2] + 1; cz++)
-      for (int cy = f[1] - 1; cy <= f[1] + 1; cy++)
-        for (int cx = f[0] - 1; cx <= f[0] + 1; cx++) {
-          const int64_t c = p2gBin(cx, cy, cz, a.nx, a.ny);   // f - 1 >= -1, f + 1 <= n: inside the bin raster
-          const uint32_t s = a.start[c], e = a.start[c + 1];
-          for (uint32_t k = s; k < e; k++) {
-            const float pos[3] = {a.sp[0][k], a.sp[1][k], a.sp[2][k]};
-            /* per axis: the node weights on the grid whose nodes sit on the
-             * axis (the face's own axis) and on the half-shifted grids */
-            float frOn[3], frOff[3];
-            int ddOn[3], ddOff[3];
+      for (int cy = f[1] - 1; cy <= f[1] + 1; cy++) {
+        const int64_t c = p2gBin(f[0] - 1, cy, cz, a.nx, a.ny);   // f - 1 >= -1, f + 1 <= n: inside the bin raster
+        const uint32_t s = a.start[c], e = a.start[c + 3];
+        for (uint32_t k = s; k < e; k++) {
+          const float pos[3] = {a.sp[0][k], a.sp[1][k], a.sp[2][k]};
+          /* per axis: the node weights on the grid whose nodes sit on the
+           * axis (the face's own axis) and on the half-shifted grids */
+          float frOn[3], frOff[3];
+          int ddOn[3], ddOff[3];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const int i0 = (int)floorf(pos[d]);
+            frOn[d] = pos[d] - (float)i0;
+            ddOn[d] = f[d] - i0;
+            const float g1 = pos[d] - 0.5f;
+            const int i1 = (int)floorf(g1);
+            frOff[d] = g1 - (float)i1;
+            ddOff[d] = f[d] - i1;
+          }
+          const float mp = a.sph[k] == 0 ? a.rhoL : a.rhoG;
+#pragma unroll
+          for (int dir = 0; dir < 3; dir++) {
+            float fr[3];
+            int dd[3], ok = 1;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-              const int i0 = (int)floorf(pos[d]);
-              frOn[d] = pos[d] - (float)i0;
-              ddOn[d] = f[d] - i0;
-              const float g1 = pos[d] - 0.5f;
-              const int i1 = (int)floorf(g1);
-              frOff[d] = g1 - (float)i1;
-              ddOff[d] = f[d] - i1;
+              fr[d] = d == dir ? frOn[d] : frOff[d];
+              dd[d] = d == dir ? ddOn[d] : ddOff[d];
+              if (dd[d] < 0 || dd[d] > 1) ok = 0;
             }
-            const float mp = a.sph[k] == 0 ? a.rhoL : a.rhoG;
-#pragma unroll
-            for (int dir = 0; dir < 3; dir++) {
-              float fr[3];
-              int dd[3], ok = 1;
-#pragma unroll
-              for (int d = 0; d < 3; d++) {
-                fr[d] = d == dir ? frOn[d] : frOff[d];
-                dd[d] = d == dir ? ddOn[d] : ddOff[d];
-                if (dd[d] < 0 || dd[d] > 1) ok = 0;
-              }
-              if (!ok) continue;
-              const float w = ((dd[0] ? fr[0] : 1.0f - fr[0]) * (dd[1] ? fr[1] : 1.0f - fr[1])) *
-                              (dd[2] ? fr[2] : 1.0f - fr[2]);
-              sv[dir] = sv[dir] + a.sp[3 + dir][k] * w;
-              sw[dir] = sw[dir] + w;
-              sr[dir] = sr[dir] + mp * w;
-            }
+            if (!ok) continue;
+            const float w = ((dd[0] ? fr[0] : 1.0f - fr[0]) * (dd[1] ? fr[1] : 1.0f - fr[1])) *
+                            (dd[2] ? fr[2] : 1.0f - fr[2]);
+            sv[dir] = sv[dir] + a.sp[3 + dir][k] * w;
+            sw[dir] = sw[dir] + w;
+            sr[dir] = sr[dir] + mp * w;
           }
         }
+      }
 #pragma unroll
     for (int dir = 0; dir < 3; dir++) {
       vel[dir][o + v] = sw[dir] > a.massEps ? sv[dir] / sw[dir] : 0.0f;
