@@ -44,18 +44,27 @@ __global__ void __launch_bounds__(256) k_pde_mc(PdeArgs a, const int *__restrict
 {
   constexpr int H = B + 2, H2 = H * H, N = B * B * B;
   __shared__ float halo[H * H * H];
+  __shared__ int nslot[27];   // slots of the 3x3x3 block neighbourhood, -1: none or outside
   const int bid = blocks[blockIdx.x];
   const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
   const int64_t own = (int64_t)a.slot[bid] * N;
   const int g0[3] = {bx * B, by * B, bz * B};
   const float inv = 1.f / 65535;   // 1.f / numeric_limits<uint16_t>::max()
+  /* the 27 neighbour slots once, so that each halo voxel costs one load */
+  if (threadIdx.x < 27) {
+    const int nx = bx + (int)threadIdx.x % 3 - 1, ny = by + ((int)threadIdx.x / 3) % 3 - 1,
+              nz = bz + (int)threadIdx.x / 9 - 1;
+    const bool in = nx >= 0 && ny >= 0 && nz >= 0 && nx < a.nbx && ny < a.nby && nz < a.nbz;
+    nslot[threadIdx.x] = in ? a.slot[nx + a.nbx * (ny + a.nby * nz)] : -1;
+  }
+  __syncthreads();
   for (int h = threadIdx.x; h < H * H * H; h += blockDim.x) {
     const int hx = h % H, hy = (h / H) % H, hz = h / H2;
     const int g[3] = {g0[0] + hx - 1, g0[1] + hy - 1, g0[2] + hz - 1};
     int64_t src = -1;
     if (g[0] >= 0 && g[1] >= 0 && g[2] >= 0 && g[0] < a.n[0] && g[1] < a.n[1] && g[2] < a.n[2]) {
-      const int nb = (g[0] >> a.lb) + a.nbx * ((g[1] >> a.lb) + a.nby * (g[2] >> a.lb));
-      const int s = a.slot[nb];
+      /* hx - 1 in [-1, B]: neighbour index 0, 1 or 2 along each axis */
+      const int s = nslot[(hx + B - 1) / B + 3 * ((hy + B - 1) / B) + 9 * ((hz + B - 1) / B)];
       if (s >= 0) src = (int64_t)s * N + (g[0] & (B - 1)) + B * ((g[1] & (B - 1)) + B * (g[2] & (B - 1)));
     }
     if (src < 0) {   // Neumann: the block's own nearest voxel
