@@ -185,6 +185,7 @@ int msbgx_set_options(msbgx_grid *g, unsigned options)
 {
   if (int rc = checkGrid(g)) return rc;
   g->options = options;
+  deviceInvalidateGraphs(*g);   // captured graphs bake in the V-cycle form
   return MSBGX_OK;
 }
 

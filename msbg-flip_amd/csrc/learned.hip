@@ -804,6 +804,7 @@ struct NetBuffers {
 };
 
 struct Net {
+  uint64_t serial = 0;                           // unique per created net (graph keys)
   int mode = MSBGX_NET_FINE;                     // MSBGX_NET_FINE or MSBGX_NET_COARSE
   int stallWin = 5;                              // fallback when |r| drops by less than
   float stallDrop = 0.1f;                        // stallDrop over stallWin iterations
@@ -872,7 +873,9 @@ static int netPack(Net &n)
 
 int netCreate(Net **out, uint64_t seed)
 {
+  static std::atomic<uint64_t> nextSerial{1};
   Net *n = new Net();
+  n->serial = nextSerial++;
   *out = n;
   HIPCHK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
   if (int rc = dalloc(&n->w, nn::NPARAM)) return rc;
@@ -1191,9 +1194,8 @@ struct LearnedCtx {
   float *cin = nullptr, *cout = nullptr, *zc = nullptr, *rs = nullptr;
   int ntx = 0, nty = 0, ntz = 0, ncoarse = 0;
   int64_t n0 = 0;
-  const Net *graphNet = nullptr;   // net whose buffers the captured graph uses
   hipGraphExec_t graph = nullptr;
-  int graphChunk = 0, graphMode = -1;
+  GraphKey graphKey;               // includes the serial of the net whose buffers the graph uses
 };
 
 static void freeLearned(LearnedCtx *c)
@@ -1376,17 +1378,15 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
     if (int rc = dalloc(&c.zprev, std::max<int64_t>(1, c.n0))) return rc;
   if (!c.zplain)
     if (int rc = dalloc(&c.zplain, std::max<int64_t>(1, c.n0))) return rc;
-  if (d.histCap < p.max_iter + 1) {
-    if (d.hist) (void)hipFree(d.hist);
-    d.hist = nullptr;
-    if (int rc = dalloc(&d.hist, p.max_iter + 1)) return rc;
-    d.histCap = p.max_iter + 1;
-  }
+  if (int rc = ensureHist(g, p.max_iter + 1)) return rc;
   const bool coarse = n.mode == MSBGX_NET_COARSE;
   if (coarse)
     if (int rc = coarseSetup(g, c, d.stream)) return rc;
-  const int chunk = std::max(1, g.graphChunk);
-  if (c.graph && (c.graphNet != &n || c.graphChunk != chunk || c.graphMode != n.mode)) {
+  GraphKey key = graphKey(g, p);
+  key.net = n.serial;
+  key.netMode = n.mode;
+  const int chunk = key.chunk;
+  if (c.graph && c.graphKey != key) {
     (void)hipGraphExecDestroy(c.graph);
     c.graph = nullptr;
   }
@@ -1441,9 +1441,7 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
         HIPCHK(hipStreamEndCapture(d.stream, &graph));
         HIPCHK(hipGraphInstantiate(&c.graph, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
-        c.graphNet = &n;
-        c.graphChunk = chunk;
-        c.graphMode = n.mode;
+        c.graphKey = key;
       }
       HIPCHK(hipGraphLaunch(c.graph, d.stream));
     } else {

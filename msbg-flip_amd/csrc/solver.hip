@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1313,6 +1314,34 @@ __global__ void k_scene_rhs(msbgx_scene s, const int *__restrict__ blocks, const
 /* Device state of a batched block set: the chunk table of the PCG vector
  * kernels (chunk c covers cells [cBeg[c], cEnd[c]) of problem cProb[c];
  * problem p owns chunks [cb[p], ce[p])) and one PCG state per problem. */
+/* What a captured PCG graph bakes in besides its buffers: the iterations per
+ * replay, the V-cycle's smoothing counts and the inert-coarse decision
+ * (MSBGX_OPT_LITERAL_VCYCLE), and for the learned solve the network and its
+ * form.  A solve whose key differs from the graph's recaptures. */
+struct GraphKey {
+  int chunk = 0, nPre = -1, nPost = -1, nCoarse = -1;
+  unsigned opts = 0;
+  uint64_t net = 0;
+  int netMode = -1;
+  bool operator==(const GraphKey &o) const
+  {
+    return chunk == o.chunk && nPre == o.nPre && nPost == o.nPost && nCoarse == o.nCoarse && opts == o.opts &&
+           net == o.net && netMode == o.netMode;
+  }
+  bool operator!=(const GraphKey &o) const { return !(*this == o); }
+};
+
+static GraphKey graphKey(const Grid &g, const msbgx_solve_params &p)
+{
+  GraphKey k;
+  k.chunk = std::max(1, g.graphChunk);
+  k.nPre = p.n_pre;
+  k.nPost = p.n_post;
+  k.nCoarse = p.n_coarse;
+  k.opts = g.options & MSBGX_OPT_LITERAL_VCYCLE;
+  return k;
+}
+
 struct BatchDev {
   int nChunks = 0;
   int *cProb = nullptr, *cb = nullptr, *ce = nullptr;
@@ -1330,7 +1359,7 @@ struct BatchDev {
   double *hist = nullptr;
   int histStride = 0;
   hipGraphExec_t graph = nullptr;
-  int graphChunk = 0;
+  GraphKey graphKey;
 };
 
 struct LearnedCtx;
@@ -1346,7 +1375,7 @@ struct Dev {
   int *zeroFlag = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graphChunk = 0;
+  GraphKey graphKey;
   /* iterative refinement (precision form) */
   double *x64 = nullptr;
   float *b0 = nullptr;
@@ -1401,12 +1430,34 @@ static void freeLevel(MgLevel &m)
   m.d_lists = nullptr;
 }
 
+/* The PCG residual history (one double per iteration plus the final |r|^2).
+ * Captured graphs hold its address, so growing it drops them first. */
+static int ensureHist(Grid &g, int n)
+{
+  Dev &d = dev(g);
+  if (d.histCap >= n) return MSBGX_OK;
+  HIPCHK(hipStreamSynchronize(d.stream));
+  deviceInvalidateGraphs(g);
+  if (d.hist) (void)hipFree(d.hist);
+  d.hist = nullptr;
+  d.histCap = 0;
+  if (int rc = dalloc(&d.hist, n)) return rc;
+  d.histCap = n;
+  return MSBGX_OK;
+}
+
+/* Teardown order: drain the stream, destroy every captured graph (they
+ * reference the level, learned and batch buffers), then free the buffers. */
 void deviceFree(Grid &g)
 {
+  if (g.dev) {
+    Dev &d = dev(g);
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    deviceInvalidateGraphs(g);
+  }
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
   if (g.dev) {
     Dev &d = dev(g);
-    if (d.graph) (void)hipGraphExecDestroy(d.graph);
     if (d.state) (void)hipFree(d.state);
     if (d.partials) (void)hipFree(d.partials);
     if (d.hist) (void)hipFree(d.hist);
@@ -1420,7 +1471,6 @@ void deviceFree(Grid &g)
     freeFlip(d);
     if (d.pack32) (void)hipFree(d.pack32);
     if (BatchDev *b = d.batch) {
-      if (b->graph) (void)hipGraphExecDestroy(b->graph);
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
       f(b->cProbL); f(b->cbL); f(b->ceL); f(b->cBegL); f(b->cEndL);
       delete b;
@@ -1542,7 +1592,8 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
     HIPCHK(hipMemset(d->zeroFlag, 0, sizeof(int)));
   }
   Dev &d = dev(g);
-  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  HIPCHK(hipStreamSynchronize(d.stream));
+  deviceInvalidateGraphs(g);
   freeLearned(d.learned);
   d.learned = nullptr;
   freeFlip(d);
@@ -1788,12 +1839,7 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
-  if (d.histCap < p.max_iter + 1) {
-    if (d.hist) (void)hipFree(d.hist);
-    d.hist = nullptr;
-    if (int rc = dalloc(&d.hist, p.max_iter + 1)) return rc;
-    d.histCap = p.max_iter + 1;
-  }
+  if (int rc = ensureHist(g, p.max_iter + 1)) return rc;
   PcgState init{};
   init.tolSq = (double)p.tol * (double)p.tol;
   init.maxIter = p.max_iter;
@@ -1818,8 +1864,9 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
   HIPCHK(hipGetLastError());
 
   const bool useGraph = !(g.options & MSBGX_OPT_NO_GRAPH);
-  const int chunk = std::max(1, g.graphChunk);
-  if (d.graph && d.graphChunk != chunk) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
+  const GraphKey key = graphKey(g, p);
+  const int chunk = key.chunk;
+  if (d.graph && d.graphKey != key) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
   PcgState hs{};
   for (int done_it = 0; done_it < p.max_iter;) {
     const int n = std::min(chunk, p.max_iter - done_it);
@@ -1831,7 +1878,7 @@ int deviceSolve(Grid &g, const msbgx_solve_params &p, int *iters, float *rel)
         HIPCHK(hipStreamEndCapture(d.stream, &graph));
         HIPCHK(hipGraphInstantiate(&d.graph, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
-        d.graphChunk = chunk;
+        d.graphKey = key;
       }
       HIPCHK(hipGraphLaunch(d.graph, d.stream));
     } else {
@@ -2177,12 +2224,7 @@ int deviceSolveRefined(Grid &g, const msbgx_solve_params &p, double tol, int max
   }
   HIPCHK(hipMemsetAsync(d.x64, 0, n0 * sizeof(double), d.stream));
   hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, d.b0, d.b0, n0, d.partials, d.zeroFlag);
-  if (d.histCap < 2) {
-    if (d.hist) (void)hipFree(d.hist);
-    d.hist = nullptr;
-    if (int rc = dalloc(&d.hist, 2)) return rc;
-    d.histCap = 2;
-  }
+  if (int rc = ensureHist(g, 2)) return rc;
   finalize(d, kVecBlocks, FIN_FINAL, d.hist);
   double bb = 0.0;
   HIPCHK(hipMemcpyAsync(&bb, d.hist, sizeof(double), hipMemcpyDeviceToHost, d.stream));
@@ -2508,9 +2550,10 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
   finB(m, FIN_RHO0, nullptr, 0);
   hipLaunchKernelGGL(k_all_done, dim3(1), dim3(256), 0, d.stream, b.states, P, b.allDone);
   HIPCHK(hipGetLastError());
-  const int chunk = std::max(1, m.graphChunk);
+  const GraphKey key = graphKey(m, p);
+  const int chunk = key.chunk;
   const bool useGraph = !(m.options & MSBGX_OPT_NO_GRAPH);
-  if (b.graph && b.graphChunk != chunk) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
+  if (b.graph && b.graphKey != key) { (void)hipGraphExecDestroy(b.graph); b.graph = nullptr; }
   std::vector<PcgState> hs(P);
   for (int it = 0; it < p.max_iter;) {
     const int n = std::min(chunk, p.max_iter - it);
@@ -2522,7 +2565,7 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
         HIPCHK(hipStreamEndCapture(d.stream, &graph));
         HIPCHK(hipGraphInstantiate(&b.graph, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
-        b.graphChunk = chunk;
+        b.graphKey = key;
       }
       HIPCHK(hipGraphLaunch(b.graph, d.stream));
     } else {
@@ -2656,6 +2699,8 @@ void deviceInvalidateGraphs(Grid &g)
 {
   if (!g.dev) return;
   Dev &d = dev(g);
+  const bool any = d.graph || (d.batch && d.batch->graph) || (d.learned && d.learned->graph);
+  if (any && d.stream) (void)hipStreamSynchronize(d.stream);   // no replay may still be running
   if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
   if (d.batch && d.batch->graph) { (void)hipGraphExecDestroy(d.batch->graph); d.batch->graph = nullptr; }
   if (d.learned && d.learned->graph) { (void)hipGraphExecDestroy(d.learned->graph); d.learned->graph = nullptr; }
