@@ -397,6 +397,22 @@ int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap);
 int msbgx_pde_mean_curvature(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active, int num_iter,
                              float time_step, double *ms);
 
+/* Allocation-free variant of msbgx_pde_mean_curvature.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments (no timing output), same results bit for bit;
+ * it calls no hipMalloc/hipFree and creates no stream or event: its device
+ * work space is one static array of MSBGX_PDE_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  A call needs
+ *   2 sx0*sy0*sz0 + 2 (level-0 blocks) bsx0^3 + 12 (blocks) + 8 n_active
+ * bytes (each of the seven parts rounded up to 256) of it; larger inputs are
+ * rejected with MSBGX_EINVAL.  It runs on the calling thread's per-thread
+ * default stream and returns when `density` holds the result.  Calls that
+ * overlap on one GPU share the array and race. */
+#define MSBGX_PDE_NOALLOC_SCRATCH_BYTES (80u << 20)
+int msbgx_pde_mean_curvature_noalloc(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active,
+                                     int num_iter, float time_step);
+
 /* Caller-side block level policy (src/flip_sim.cpp:480-548) over a scene:
  * 26-neighbourhood block distance d from seed blocks (mode 0: blocks holding
  * liquid; mode 1: blocks holding a liquid cell with an air face neighbour);

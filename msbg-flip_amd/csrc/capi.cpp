@@ -748,6 +748,17 @@ int msbgx_pde_mean_curvature(msbgx_grid *g, uint16_t *density, const int *active
   return devicePdeMeanCurvature(*g, density, active_blocks, n_active, num_iter, time_step, ms);
 }
 
+int msbgx_pde_mean_curvature_noalloc(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active,
+                                     int num_iter, float time_step)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!density || n_active < 0 || (n_active > 0 && !active_blocks) || num_iter < 0 || !std::isfinite(time_step)) {
+    setError("bad argument");
+    return MSBGX_EINVAL;
+  }
+  return devicePdeMeanCurvatureNoAlloc(*g, density, active_blocks, n_active, num_iter, time_step);
+}
+
 int msbgx_level_map_from_scene(const msbgx_scene *s, int bsx0, int n_levels, int mode, int d0, int *out)
 {
   const int rc = levelMapFromScene(s, bsx0, n_levels, mode, d0, out);

@@ -171,6 +171,8 @@ int deviceCompactApply(Grid &g, float *preconOut, float *zOut, int cap, int *nRo
 /* msbg_demo mean-curvature smoothing (pde.hip) */
 int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter, float dt,
                            double *ms);
+int devicePdeMeanCurvatureNoAlloc(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter,
+                                  float dt);
 int deviceMultiply(Grid &g, int L, bool residual, int chX, int chB, int chDst);
 int deviceDownsample(Grid &g, int Lc, int chSrc, int chDst);
 int deviceProlongate(Grid &g, int Lf, int ch);

@@ -30,6 +30,9 @@
  * Algorithmic bytes per voxel and iteration: 2 read + 2 written.
  */
 
+#include <mutex>
+#include <string>
+
 namespace msbgx {
 
 struct PdeArgs {
@@ -97,6 +100,22 @@ __global__ void __launch_bounds__(256) k_pde_mc(PdeArgs a, const int *__restrict
     phi = fminf(fmaxf(phi, 0.0f), 1.0f);
     rho[own + v] = (uint16_t)(int)rintf(phi * 65535);
   }
+}
+
+/* nIter iterations of the 8 colour passes: per colour one launch over that
+ * colour's listed level-0 blocks (lists[start[c] .. start[c+1])). */
+static void pdeColourPasses(int B, const PdeArgs &a, const int *lists, const int start[9], int nIter, uint16_t *rho,
+                            hipStream_t s)
+{
+  for (int it = 0; it < nIter; it++)
+    for (int c = 0; c < 8; c++) {
+      const int cnt = start[c + 1] - start[c];
+      if (cnt == 0) continue;
+      if (B == 16) hipLaunchKernelGGL(k_pde_mc<16>, dim3(cnt), dim3(256), 0, s, a, lists + start[c], rho);
+      else if (B == 8) hipLaunchKernelGGL(k_pde_mc<8>, dim3(cnt), dim3(256), 0, s, a, lists + start[c], rho);
+      else if (B == 4) hipLaunchKernelGGL(k_pde_mc<4>, dim3(cnt), dim3(256), 0, s, a, lists + start[c], rho);
+      else hipLaunchKernelGGL(k_pde_mc<2>, dim3(cnt), dim3(256), 0, s, a, lists + start[c], rho);
+    }
 }
 
 /* Host: upload the dense channel, run the colour passes, download. */
@@ -183,15 +202,7 @@ int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, in
   a.slot = dSlot; a.nbx = g.nbx; a.nby = g.nby; a.nbz = g.nbz; a.lb = lb;
   a.n[0] = g.sx0; a.n[1] = g.sy0; a.n[2] = g.sz0; a.dt = dt;
   HIPCHK(hipEventRecord(res.e0, s));
-  for (int it = 0; it < nIter; it++)
-    for (int c = 0; c < 8; c++) {
-      const int cnt = start[c + 1] - start[c];
-      if (cnt == 0) continue;
-      if (B == 16) hipLaunchKernelGGL(k_pde_mc<16>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
-      else if (B == 8) hipLaunchKernelGGL(k_pde_mc<8>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
-      else if (B == 4) hipLaunchKernelGGL(k_pde_mc<4>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
-      else hipLaunchKernelGGL(k_pde_mc<2>, dim3(cnt), dim3(256), 0, s, a, dLists + start[c], dRho);
-    }
+  pdeColourPasses(B, a, dLists, start, nIter, dRho, s);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(res.e1, s));
   HIPCHK(hipEventSynchronize(res.e1));
@@ -213,4 +224,192 @@ int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, in
   return MSBGX_OK;
 }
 
+/* ------------------------------------------------- allocation-free variant */
+
+/* Work space of devicePdeMeanCurvatureNoAlloc, one copy per GPU, held until
+ * the process ends.  In host code `scratch` names a host-side stand-in, not
+ * device memory: host code only ever uses the per-device address that
+ * hipGetSymbolAddress returns (scratchBase). */
+static __device__ __align__(256) unsigned char scratch[MSBGX_PDE_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratchMu;
+static unsigned char *g_scratchBase[kMaxScratchDevices] = {};
+
+/* The current device's copy of `scratch`, looked up once per device with
+ * that device current. */
+static int scratchBase(unsigned char **out)
+{
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    setError("device index beyond the scratch table");
+    return MSBGX_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchBase[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      setError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return MSBGX_EHIP;
+    }
+    g_scratchBase[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratchBase[dev];
+  return MSBGX_OK;
+}
+
+struct PdeStarts {
+  int start[9];   // colour c's blocks are lists[start[c] .. start[c+1])
+};
+
+/* Exclusive prefix sum over the 1024 threads of a workgroup (tmp: 1024 ints). */
+__device__ int blockExclusiveSum1024(int v, int *tmp)
+{
+  const int t = threadIdx.x;
+  tmp[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int x = t >= o ? tmp[t - o] : 0;
+    __syncthreads();
+    tmp[t] += x;
+    __syncthreads();
+  }
+  const int incl = tmp[t];
+  __syncthreads();
+  return incl - v;
+}
+
+/* One workgroup of 1024 threads builds the tables the host path builds on
+ * the host: slot[b] = rank of b among the level-0 blocks of the map (-1
+ * otherwise), level0[rank] = b, and the listed level-0 blocks per colour in
+ * list order.  Thread t takes a contiguous run of blocks (and of list
+ * entries), so ranks follow block order and the colour lists follow list
+ * order.  Every entry of slot[0, nBlocks), level0[0, #level-0) and
+ * lists[0, start[8]) is written, so nothing left from an earlier call is read. */
+__global__ void __launch_bounds__(1024) k_pde_na_tables(const int *__restrict__ map, int nBlocks,
+                                                        const int *__restrict__ active, int nActive, int nbx, int nby,
+                                                        PdeStarts st, int *__restrict__ slot, int *__restrict__ level0,
+                                                        int *__restrict__ lists)
+{
+  __shared__ int tmp[1024];
+  const int t = threadIdx.x;
+  const int kb = (nBlocks + 1023) / 1024;
+  const int b0 = min(t * kb, nBlocks), b1 = min(b0 + kb, nBlocks);
+  int cnt = 0;
+  for (int b = b0; b < b1; b++) cnt += map[b] == 0;
+  int r = blockExclusiveSum1024(cnt, tmp);
+  for (int b = b0; b < b1; b++) {
+    if (map[b] == 0) {
+      slot[b] = r;
+      level0[r] = b;
+      r++;
+    } else {
+      slot[b] = -1;
+    }
+  }
+  auto colourOf = [&](int b) {
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    return (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
+  };
+  const int ka = (nActive + 1023) / 1024;
+  const int a0 = min(t * ka, nActive), a1 = min(a0 + ka, nActive);
+  int cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = a0; i < a1; i++)
+    if (map[active[i]] == 0) cc[colourOf(active[i])]++;
+  int pos[8];
+  for (int c = 0; c < 8; c++) pos[c] = st.start[c] + blockExclusiveSum1024(cc[c], tmp);
+  for (int i = a0; i < a1; i++) {
+    const int b = active[i];
+    if (map[b] == 0) lists[pos[colourOf(b)]++] = b;
+  }
+}
+
+/* The level-0 blocks between the dense channel (x fastest over the domain)
+ * and the slot layout the smoothing kernels use: TO_DENSE false gathers,
+ * true scatters back.  One workgroup per level-0 block. */
+template <bool TO_DENSE>
+__global__ void __launch_bounds__(256) k_pde_na_copy(const int *__restrict__ level0, int nbx, int nby, int lb, int sx,
+                                                     int sy, uint16_t *__restrict__ dense, uint16_t *__restrict__ nat)
+{
+  const int B = 1 << lb, N = B * B * B;
+  const int b = level0[blockIdx.x];
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  uint16_t *blk = nat + (int64_t)blockIdx.x * N;
+  for (int v = threadIdx.x; v < N; v += blockDim.x) {
+    const int vx = v & (B - 1), vy = (v >> lb) & (B - 1), vz = v >> (2 * lb);
+    const int64_t di = ((int64_t)(bz * B + vz) * sy + (by * B + vy)) * sx + (int64_t)bx * B + vx;
+    if (TO_DENSE) dense[di] = blk[v];
+    else blk[v] = dense[di];
+  }
+}
+
+/* devicePdeMeanCurvature without any device allocation: the dense channel,
+ * the slot-layout copy and the tables live in `scratch`; the host keeps only
+ * counts on the stack.  Same kernels, same results. */
+int devicePdeMeanCurvatureNoAlloc(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter, float dt)
+{
+  const int B = g.bsx0, lb = __builtin_ctz(B);
+  if (B > 16) {
+    setError("mean-curvature smoothing: block edge above 16 is not supported (the halo block is staged in LDS)");
+    return MSBGX_EINVAL;
+  }
+  int nLevel0 = 0;
+  for (int b = 0; b < g.nBlocks; b++) nLevel0 += g.map[b] == 0;
+  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < nActive; i++) {
+    const int b = active[i];
+    if (b < 0 || b >= g.nBlocks) {
+      setError("mean-curvature smoothing: active block id out of range");
+      return MSBGX_EINVAL;
+    }
+    if (g.map[b] != 0) continue;   // only level-0 blocks are smoothed (src/msbg4.cpp:447)
+    int bx, by, bz;
+    g.blockCoords(b, bx, by, bz);
+    cnt[(bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2)]++;
+  }
+  PdeStarts st{};
+  for (int c = 0; c < 8; c++) st.start[c + 1] = st.start[c] + cnt[c];
+  /* scratch layout, every part 256-byte aligned */
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t nDense = (size_t)g.sx0 * g.sy0 * g.sz0, nv = (size_t)nLevel0 * B * B * B;
+  const size_t nb = (size_t)g.nBlocks, na = (size_t)nActive;
+  const size_t oNat = al(2 * nDense), oMap = oNat + al(2 * nv), oSlot = oMap + al(4 * nb), oLev0 = oSlot + al(4 * nb),
+               oAct = oLev0 + al(4 * nb), oLists = oAct + al(4 * na), total = oLists + al(4 * na);
+  if (total > (size_t)MSBGX_PDE_NOALLOC_SCRATCH_BYTES) {
+    setError("mean-curvature smoothing without allocation: needs " + std::to_string(total) +
+             " bytes of scratch, the maximum is " + std::to_string((size_t)MSBGX_PDE_NOALLOC_SCRATCH_BYTES));
+    return MSBGX_EINVAL;
+  }
+  const int nIter = (numIter % 2 == 1) ? numIter + 1 : numIter;
+  unsigned char *base = nullptr;
+  if (int rc = scratchBase(&base)) return rc;
+  uint16_t *dDense = reinterpret_cast<uint16_t *>(base), *dNat = reinterpret_cast<uint16_t *>(base + oNat);
+  int *dMap = reinterpret_cast<int *>(base + oMap), *dSlot = reinterpret_cast<int *>(base + oSlot);
+  int *dLev0 = reinterpret_cast<int *>(base + oLev0), *dAct = reinterpret_cast<int *>(base + oAct);
+  int *dLists = reinterpret_cast<int *>(base + oLists);
+  hipStream_t s = hipStreamPerThread;
+  HIPCHK(hipMemcpyAsync(dDense, dense, 2 * nDense, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dMap, g.map.data(), 4 * nb, hipMemcpyHostToDevice, s));
+  if (nActive > 0) HIPCHK(hipMemcpyAsync(dAct, active, 4 * na, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pde_na_tables, dim3(1), dim3(1024), 0, s, dMap, g.nBlocks, dAct, nActive, g.nbx, g.nby, st, dSlot,
+                     dLev0, dLists);
+  if (nLevel0 > 0)
+    hipLaunchKernelGGL(k_pde_na_copy<false>, dim3(nLevel0), dim3(256), 0, s, dLev0, g.nbx, g.nby, lb, g.sx0, g.sy0,
+                       dDense, dNat);
+  PdeArgs a;
+  a.slot = dSlot; a.nbx = g.nbx; a.nby = g.nby; a.nbz = g.nbz; a.lb = lb;
+  a.n[0] = g.sx0; a.n[1] = g.sy0; a.n[2] = g.sz0; a.dt = dt;
+  pdeColourPasses(B, a, dLists, st.start, nIter, dNat, s);
+  if (nLevel0 > 0)
+    hipLaunchKernelGGL(k_pde_na_copy<true>, dim3(nLevel0), dim3(256), 0, s, dLev0, g.nbx, g.nby, lb, g.sx0, g.sy0,
+                       dDense, dNat);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dense, dDense, 2 * nDense, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSBGX_OK;
+}
+
 }  // namespace msbgx
+

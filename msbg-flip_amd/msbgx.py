@@ -226,6 +226,7 @@ def _load():
         "msbgx_solve_pressure_compact": (C.c_int, [P, C.c_int, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
         "msbgx_compact_mic0": (C.c_int, [P, P, P, C.c_int]),
         "msbgx_pde_mean_curvature": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_double)]),
+        "msbgx_pde_mean_curvature_noalloc": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_float]),
         "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
         "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
         "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
@@ -260,6 +261,7 @@ EXPORTED = [
     "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
     "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
     "msbgx_solve_pressure_compact", "msbgx_compact_mic0", "msbgx_pde_mean_curvature",
+    "msbgx_pde_mean_curvature_noalloc",
 ]
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
@@ -387,6 +389,16 @@ class MultiresSparseGrid:
         ms = C.c_double()
         _ck(lib.msbgx_pde_mean_curvature(self._h, _ptr(d), _ptr(a), len(a), num_iter, time_step, C.byref(ms)), "pde")
         return d, ms.value
+
+    def pde_mean_curvature_noalloc(self, density, active_blocks, num_iter=5, time_step=0.05):
+        """The same smoothing through the allocation-free entry point
+        (static per-GPU work space, no device allocation); returns the
+        smoothed copy."""
+        d = np.ascontiguousarray(density, np.uint16).copy()
+        a = np.ascontiguousarray(active_blocks, np.int32)
+        _ck(lib.msbgx_pde_mean_curvature_noalloc(self._h, _ptr(d), _ptr(a), len(a), num_iter, time_step),
+            "pde_noalloc")
+        return d
 
     def solve_pressure_compact(self, max_iter=1000, tol=1e-6):
         """Compact MIC(0)-PCG comparator (src/flip_sim.cpp:858-1151); the
