@@ -214,14 +214,15 @@ def _fma(a, b, c):
     return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
-def brute_relax_block(x, bvec, D, W, omega, reverse, halo, wplus, S):
-    """Pure-Python restatement of one block's 5 local sweeps (level 0, h = 1):
-    SIMD segments of S lanes along x are Jacobi inside, Gauss-Seidel across;
+def brute_relax_block(x, bvec, D, W, omega, reverse, halo, wplus, S, h=1.0):
+    """Pure-Python restatement of one block's 5 local sweeps at grid scale h
+    (values h*x inside the block, halo values already scaled by h): SIMD
+    segments of S lanes along x are Jacobi inside, Gauss-Seidel across;
     sweep k reversed iff reverse == (k & 1); the last sweep reads only the
-    previous sweep's values and is the one stored."""
+    previous sweep's values and is the one stored, divided by h."""
     b = x.shape[0]
     X = np.zeros((b + 2, b + 2, b + 2), np.float32)
-    X[1:-1, 1:-1, 1:-1] = x
+    X[1:-1, 1:-1, 1:-1] = x * np.float32(h)
     X[0, 1:-1, 1:-1], X[-1, 1:-1, 1:-1] = halo["z-"], halo["z+"]
     X[1:-1, 0, 1:-1], X[1:-1, -1, 1:-1] = halo["y-"], halo["y+"]
     X[1:-1, 1:-1, 0], X[1:-1, 1:-1, -1] = halo["x-"], halo["x+"]
@@ -255,7 +256,7 @@ def brute_relax_block(x, bvec, D, W, omega, reverse, halo, wplus, S):
                                   if d != 0 else f0)
                     for l in range(S):
                         if last:
-                            out[zz, yy, sg * S + l] = ys[l]
+                            out[zz, yy, sg * S + l] = np.float32(ys[l] / np.float32(h))
                         else:
                             X[zz + 1, yy + 1, sg * S + l + 1] = ys[l]
     return out
