@@ -157,8 +157,10 @@ int msbgx_batch_residual_history(const msbgx_grid *batch, int problem, double *o
 /* Autotuner (no reference counterpart): measures, on the handle's own
  * prepared problem (single or batch), the smoother kernel per block size
  * (bit k of *smoother_mask set = the LDS-tile kernel for blocks of
- * bsx0 >> k; FP32 operands, blocks up to 16^3) and the PCG iterations per
- * graph replay (*graph_chunk), and keeps the fastest.  The choices change
+ * bsx0 >> k, FP32 operands, blocks up to 16^3; for 32^3 blocks with 16-bit
+ * operands, bit 0 set = the record-streaming kernel instead of the
+ * register-resident one) and the PCG iterations per graph replay
+ * (*graph_chunk), and keeps the fastest.  The choices change
  * launch shapes only: results stay bit-identical.  Runs trial solves with
  * `p`; the pressure channel is zero on return. */
 int msbgx_autotune(msbgx_grid *g, const msbgx_solve_params *p, int *graph_chunk, int *smoother_mask);

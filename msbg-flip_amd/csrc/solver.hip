@@ -67,6 +67,13 @@ struct LevelArgs {
    * smoother, two float4 per cell {D, W-x, W+x, W-y}, {W+y, W-z, W+z, 0}, at
    * record off[bid] + slot * 1024 + (x + 32 y), slot = (z + x/8 + y) mod 32 */
   const float4 *pack;
+  /* the same for 16-bit operands (precision form): one uint4 per cell with the
+   * 16-bit encodings {D | W-x << 16, W-y | W+y << 16, W-z | W+x << 16,
+   * flags | W+z << 16} at record off[bid] + slot * 1024 + (x + 32 y) */
+  const uint4 *pack16;
+  /* {W-y | W+y << 16} of each cell again, alone, in the same slot-plane order
+   * (read once per local sweep by the 32^3 16-bit smoother) */
+  const uint32_t *pack16y;
 };
 
 __device__ __forceinline__ float fromBits16(uint16_t v, int prec)
@@ -157,10 +164,11 @@ __device__ __forceinline__ bool segmentAllNonFluid(bool nonFluid)
  * and neighbour slot offsets (src/msbg3.cpp:665-1044, 46-161). */
 template <int B>
 __device__ __forceinline__ void stageFaces(float *faces, const LevelArgs &a, int bid, int lev, float h, int64_t off,
-                                           const float *__restrict__ X, int *fkind, int64_t *foff)
+                                           const float *__restrict__ X, int *fkind, int64_t *foff,
+                                           int tid = (int)threadIdx.x)
 {
   constexpr int NF = Cfg<B>::NF;
-  const int tx = threadIdx.x % B, ty = threadIdx.x / B;
+  const int tx = tid % B, ty = tid / B;
   const float a13 = (float)(1.0 * (1. / 3.));      // DPMR_GRAD_COEFF_CF (src/msbg.h:85-87)
   const float a23 = (float)(1.0 * 2 * (1. / 3.));  // 2 * DPMR_GRAD_COEFF_CF
   const float *Wd[3] = {a.W0, a.W1, a.W2};
@@ -343,9 +351,10 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
  * when a neighbour has another level. */
 template <int B>
 __device__ __forceinline__ bool sameLevelFaces(const LevelArgs &a, int bid, int lev, float h, int64_t off,
-                                               const float *__restrict__ X, float *fv, int *fkind, int64_t *foff)
+                                               const float *__restrict__ X, float *fv, int *fkind, int64_t *foff,
+                                               int tid = (int)threadIdx.x)
 {
-  const int u = threadIdx.x % B, w = threadIdx.x / B;
+  const int u = tid % B, w = tid / B;
   const NbDesc &nd = a.nbd[bid];
   int lev2[6];
   int64_t off2[6];
@@ -857,6 +866,456 @@ __global__ void __launch_bounds__(1024) k_pack_col32(LevelArgs a, const int *__r
     r[0] = make_float4(D, w1, w2, w3);
     r[1] = make_float4(w4, w5, w6, 0.0f);
   }
+}
+
+/* 16-bit encoding of a smoother operand that loadCoeffs returned in the
+ * precision form: a value read from the 16-bit copies or one of the exact
+ * constants 0 and 1, so the conversion back is exact. */
+__device__ __forceinline__ uint32_t enc16(float v, int prec)
+{
+  if (prec == MSBGX_PREC_FP16) return __builtin_bit_cast(uint16_t, (_Float16)v);
+  return __float_as_uint(v) >> 16;
+}
+
+/* 16-bit coefficient records of the register-resident 32^3 smoother
+ * (LevelArgs::pack16): the values loadCoeffs returns for each cell, in the
+ * level's 16-bit encoding, with the cell flags, at the record of the
+ * register slot (z + x/8 + y) mod 32. */
+__global__ void __launch_bounds__(1024) k_pack_col32h(LevelArgs a, const int *__restrict__ blocks,
+                                                      const float *__restrict__ X, uint4 *__restrict__ pack,
+                                                      uint32_t *__restrict__ packy)
+{
+  constexpr int B = 32, S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
+  __shared__ float faces[6 * NF];
+  const int bid = blocks[blockIdx.x];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  int fkind[6];
+  int64_t foff[6];
+  stageFaces<B>(faces, a, bid, lev, 1.0f, off, X, fkind, foff);
+  ColCtx c;
+  c.a = &a; c.Bv = X; c.Xg = nullptr; c.faces = faces; c.off = off; c.pack = nullptr;
+  c.tx = threadIdx.x % B; c.ty = threadIdx.x / B; c.o = c.tx / S + c.ty;
+  c.fk1 = fkind[1]; c.fk3 = fkind[3]; c.fk5 = fkind[5];
+  c.fo1 = foff[1]; c.fo3 = foff[3]; c.fo5 = foff[5];
+  c.mixed0 = fkind[0] >= FK_COARSER; c.mixed2 = fkind[2] >= FK_COARSER; c.mixed4 = fkind[4] >= FK_COARSER;
+  c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  c.om = 0.f; c.omm = 0.f; c.HR = 1.f;
+  const int p = a.prec;
+  for (int z = 0; z < B; z++) {
+    float D, Bb, w1, w2, w3, w4, w5, w6;
+    loadCoeffs<B>(c, z, D, Bb, w1, w2, w3, w4, w5, w6);
+    const int64_t q = off + c.tx + B * c.ty + B * B * z;
+    const int sl = (z + c.o) & (B - 1);
+    uint4 r;
+    r.x = enc16(D, p) | (enc16(w1, p) << 16);
+    r.y = enc16(w3, p) | (enc16(w4, p) << 16);
+    r.z = enc16(w5, p) | (enc16(w2, p) << 16);
+    r.w = (uint32_t)a.flags[q] | (enc16(w6, p) << 16);
+    pack[off + (int64_t)sl * NT + threadIdx.x] = r;
+    packy[off + (int64_t)sl * NT + threadIdx.x] = r.y;
+  }
+}
+
+/* Register-resident smoother for 32^3 blocks with 16-bit operands (the
+ * precision form; the same schedule and arithmetic as k_relax_col, so the
+ * same bits).  The FP32 record kernel re-reads its 32-byte coefficient
+ * record at every one of the five local sweeps.  Here a workgroup of 512
+ * threads takes the block, each thread the two z-columns (x, y) and
+ * (x, y + 16): both sit at register slot hp mod 32 at hyperplane step hp.
+ * A thread keeps the coefficients of its 64 cells as packed 16-bit pairs in
+ * registers, loaded once from LevelArgs::pack16 -- {D, W-x} per cell, W-z
+ * two cells per register -- and b of the 64 cells; {W-y, W+y} of the step's
+ * cells streams from LevelArgs::pack16y, one contiguous plane per step,
+ * loaded K steps ahead (a ring of K registers).  W+x of a cell is W-x of its
+ * x+1 neighbour, which the next lane holds (a lane shuffle), except on the
+ * block's +x face, whose 32^2 values sit in LDS.  The block's h*x values
+ * with the x/y halo faces as a one-cell ring live in LDS (34^2 * 32 floats =
+ * 145 KB of gfx950's 160 KB).
+ *
+ * Registers are indexed by compile-time slot, so a sweep's 32-step period is
+ * unrolled.  One unrolled body serves all four sweeps: a reverse sweep visits
+ * slot (1 - s) mod 32 at its step s, so before it the register arrays are
+ * mirrored (entry j <-> slot (1 - j) mod 32, an involution) and the same
+ * code runs with the direction a run-time value.  That keeps the hot code
+ * (about 35 KB) within the instruction cache; four unrolled copies did not
+ * fit and every sweep streamed its code from L2. */
+struct C32 {
+  static constexpr int B = 32, S = 8, NT = 512, NC = 2, T = Cfg<32>::T, P = B + 2, P2 = P * P, N = P2 * B;
+  static constexpr int K = 4;
+};
+
+struct C32Col {
+  uint32_t c0[C32::B];       // {D, W-x} per register index (slot, or mirrored slot)
+  uint32_t c5[C32::B / 2];   // {W-z of index 2k, W-z of index 2k+1}
+  float bb[C32::B];          // b per register index
+  uint32_t wq[C32::K];       // {W-y, W+y} of the next K steps
+};
+
+/* Buffer resource over one block's cells (gfx9 descriptor word 3): the block
+ * base stays in scalar registers, each access needs one 32-bit offset. */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t blockRsrc(const void *base, int bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+
+struct C32Ctx {
+  __amdgpu_buffer_rsrc_t rb, rx;   // b and x of the block
+  __amdgpu_buffer_rsrc_t rw;       // {W-y, W+y} records of the block (LevelArgs::pack16y)
+  const uint16_t *wxp;             // LDS: W+x of the block's +x face cells, [z][y]
+  int tx, ty0;                     // x and the first column's y (the second: ty0 + 16)
+  int o0;                          // first column's slot shift x/8 + y (the second: o0 + 16)
+  int h30;                         // first column's halo-array index at z = 0
+  uint32_t skip[C32::NC];          // skipped (all non-fluid) segments by slot
+  float fz0[C32::NC], fz1[C32::NC], w6top[C32::NC];
+  float om, omm, HR;
+};
+
+template <int PREC> __device__ __forceinline__ float lo16(uint32_t v)
+{
+  if constexpr (PREC == MSBGX_PREC_FP16) return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+  else return __uint_as_float(v << 16);
+}
+template <int PREC> __device__ __forceinline__ float hi16(uint32_t v)
+{
+  if constexpr (PREC == MSBGX_PREC_FP16) return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+  else return __uint_as_float(v & 0xffff0000u);
+}
+
+/* Byte offset of cell (x, y, z) in a block's cell-ordered arrays. */
+__device__ __forceinline__ uint32_t c32Cell(int x, int y, int z) { return (uint32_t)(x + 32 * y + 1024 * z) * 4u; }
+
+/* {W-y, W+y} of column CI's cell at slot sl (uniform, taken mod 32): one
+ * dword of the block's contiguous record plane sl. */
+template <int CI> __device__ __forceinline__ uint32_t c32LoadWy(const C32Ctx &c, int sl)
+{
+  return __builtin_amdgcn_raw_buffer_load_b32(c.rw, c32Cell(c.tx, c.ty0 + 16 * CI, 0), ((uint32_t)sl & 31u) * 4096u,
+                                              0);
+}
+
+/* W-z held at register index J (a half of c5[J / 2]). */
+template <int PREC, int J> __device__ __forceinline__ float c32Wz(uint32_t k5)
+{
+  return (J & 1) ? hi16<PREC>(k5) : lo16<PREC>(k5);
+}
+
+/* Mirror a column's register arrays: entry j <-> entry (1 - j) mod 32. */
+__device__ __forceinline__ void c32Mirror(C32Col &r)
+{
+  constexpr int B = C32::B;
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    const int m = (B + 1 - j) % B;   // (1 - j) mod 32: pairs (0,1), (2,31), ..., (16,17)
+    if (j < m) {
+      uint32_t t = r.c0[j]; r.c0[j] = r.c0[m]; r.c0[m] = t;
+      float u = r.bb[j]; r.bb[j] = r.bb[m]; r.bb[m] = u;
+    }
+  }
+  /* W-z pairs: entry pair k holds mirrored slots of pair (16 - k) mod 16, halves swapped */
+  uint32_t n5[B / 2];
+#pragma unroll
+  for (int k = 0; k < B / 2; k++) n5[k] = __builtin_amdgcn_alignbit(r.c5[(16 - k) & 15], r.c5[(16 - k) & 15], 16);
+#pragma unroll
+  for (int k = 0; k < B / 2; k++) r.c5[k] = n5[k];
+}
+
+/* Column CI's cell at hyperplane hp in a wavefront sweep of direction dir
+ * (run time), at register index I of the (mirrored, for dir < 0) arrays.
+ * The slot of the +z neighbour -- and of the next lane's cell when it starts
+ * the next SIMD segment -- is at index I + 1 forward and I - 1 mirrored.
+ * Call in uniform control flow (the shuffle reads other lanes). */
+template <int PREC, int I, int CI>
+__device__ __forceinline__ void c32Cell(C32Col &r, float *h3, const C32Ctx &c, int hp, bool fwd, int iring)
+{
+  constexpr int B = C32::B, P = C32::P, P2 = C32::P2, K = C32::K;
+  constexpr int IP = (I + 1) % B, IM = (I + B - 1) % B;
+  const int o = c.o0 + 16 * CI;
+  const int z = hp - o;
+  const int sl = hp & (B - 1);
+  uint32_t sk = c.skip[CI];   // opaque copy: the slot test is made here, not hoisted as lane masks
+  asm volatile("" : "+v"(sk));
+  const bool active = z >= 0 && z < B && !((sk >> sl) & 1u);
+  /* opaque copies: the conversions stay here instead of being hoisted out of
+   * the sweep loop as loop-invariant floats */
+  uint32_t k0 = r.c0[I], k0p = r.c0[IP], k0m = r.c0[IM], k5 = r.c5[I / 2], k5p = r.c5[IP / 2], k5m = r.c5[IM / 2];
+  asm volatile("" : "+v"(k0), "+v"(k0p), "+v"(k0m), "+v"(k5), "+v"(k5p), "+v"(k5m));
+  /* both neighbours read first, then selected: a select of two array
+   * elements would become a load from a selected address (scratch memory) */
+  const uint32_t k0n = fwd ? k0p : k0m, k5n = fwd ? k5p : k5m;
+  const uint32_t send = (c.tx & 7) == 0 ? k0n : k0;
+  const uint32_t got = (uint32_t)__shfl_down((int)send, 1, 64);
+  const float Bb = r.bb[I];
+  const uint32_t wy = r.wq[iring];
+  if (active) {
+    const float D = lo16<PREC>(k0), w1 = hi16<PREC>(k0);
+    const float w3 = lo16<PREC>(wy), w4 = hi16<PREC>(wy);
+    const float w5 = c32Wz<PREC, I>(k5);
+    const float w6n = fwd ? c32Wz<PREC, IP>(k5n) : c32Wz<PREC, IM>(k5n);
+    const float w6 = z < B - 1 ? w6n : c.w6top[CI];
+    const float w2 = c.tx == B - 1 ? fromBits16(c.wxp[z * B + c.ty0 + 16 * CI], PREC) : hi16<PREC>(got);
+    float *p = h3 + (c.h30 + 16 * CI * P) + z * P2;
+    const float F0 = p[0], F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F5 = z > 0 ? p[-P2] : c.fz0[CI];
+    const float F6 = z < B - 1 ? p[P2] : c.fz1[CI];
+    float Sm = F1 * w1;
+    Sm = fmaf(F2, w2, Sm);
+    Sm = fmaf(F3, w3, Sm);
+    Sm = fmaf(F4, w4, Sm);
+    Sm = fmaf(F5, w5, Sm);
+    Sm = fmaf(F6, w6, Sm);
+    *p = D != 0.0f ? fmaf(c.om * (Bb + Sm), D, c.omm * F0) : F0;
+  }
+  /* refill the ring entry after its value's last use, so the load can target
+   * the same register (a copy would wait for the load) */
+  r.wq[iring] = c32LoadWy<CI>(c, hp + (fwd ? K : -K));
+}
+
+/* Steps base + I .. base + END - 1 of a sweep, unrolled, with no condition
+ * between them (a guarded step would make the loop's back edge merge states
+ * in which the ring was just refilled, and its loads would be waited for at
+ * once).  hp is laundered through an empty asm so that the step's addresses
+ * are formed at the step and not hoisted out of the unrolled period. */
+template <int PREC, int I, int END> struct C32Period {
+  __device__ __forceinline__ static void run(C32Col (&r)[C32::NC], float *h3, const C32Ctx &c, int base, bool fwd)
+  {
+    const int step = base + I;
+    int hp = fwd ? step : C32::T - 1 - step;
+    asm volatile("" : "+s"(hp));
+    c32Cell<PREC, I, 0>(r[0], h3, c, hp, fwd, I % C32::K);
+    c32Cell<PREC, I, 1>(r[1], h3, c, hp, fwd, I % C32::K);
+    ldsBarrier();
+    C32Period<PREC, I + 1, END>::run(r, h3, c, base, fwd);
+  }
+};
+template <int PREC, int END> struct C32Period<PREC, END, END> {
+  __device__ __forceinline__ static void run(C32Col (&)[C32::NC], float *, const C32Ctx &, int, bool) {}
+};
+
+/* One local sweep (direction fwd, uniform; the arrays are mirrored for a
+ * reverse sweep): {W-y, W+y} of the first K steps is loaded before the first
+ * step, then two unrolled 32-step periods and the T - 64 = 2 remaining steps. */
+template <int PREC>
+__device__ __forceinline__ void c32Sweep(C32Col (&r)[C32::NC], float *h3, const C32Ctx &c, bool fwd)
+{
+  constexpr int T = C32::T, K = C32::K, B = C32::B, FULL = T / B * B;
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    const int hp = fwd ? s : T - 1 - s;
+    r[0].wq[s] = c32LoadWy<0>(c, hp);
+    r[1].wq[s] = c32LoadWy<1>(c, hp);
+  }
+#pragma unroll 1
+  for (int base = 0; base < FULL; base += B) C32Period<PREC, 0, B>::run(r, h3, c, base, fwd);
+  C32Period<PREC, 0, T - FULL>::run(r, h3, c, FULL, fwd);
+}
+
+/* The fifth local sweep reads only the fourth sweep's values (the halo array
+ * after its last barrier): each thread computes its cells independently and
+ * stores the result / h.  Register slot SL (forward order) holds cell
+ * z = (SL - o) mod 32; its ring entry was loaded K slots earlier. */
+template <int PREC, int SL, int CI>
+__device__ __forceinline__ void c32JacobiCell(C32Col &r, const float *h3, const C32Ctx &c, int slv)
+{
+  constexpr int B = C32::B, P = C32::P, P2 = C32::P2, K = C32::K, SLP = (SL + 1) % B;
+  const int o = c.o0 + 16 * CI;
+  const int z = (slv - o + 4 * B) & (B - 1);
+  uint32_t k0 = r.c0[SL], k0n = r.c0[SLP], k5 = r.c5[SL / 2], k5n = r.c5[SLP / 2];
+  asm volatile("" : "+v"(k0), "+v"(k0n), "+v"(k5), "+v"(k5n));
+  const uint32_t send = (c.tx & 7) == 0 ? k0n : k0;
+  const uint32_t got = (uint32_t)__shfl_down((int)send, 1, 64);
+  uint32_t sk = c.skip[CI];
+  asm volatile("" : "+v"(sk));
+  const uint32_t wy = r.wq[SL % K];
+  if (!((sk >> SL) & 1u)) {
+    const float D = lo16<PREC>(k0), w1 = hi16<PREC>(k0);
+    const float w3 = lo16<PREC>(wy), w4 = hi16<PREC>(wy);
+    const float w5 = c32Wz<PREC, SL>(k5);
+    const float w6 = z < B - 1 ? c32Wz<PREC, SLP>(k5n) : c.w6top[CI];
+    const float w2 = c.tx == B - 1 ? fromBits16(c.wxp[z * B + c.ty0 + 16 * CI], PREC) : hi16<PREC>(got);
+    const float *p = h3 + (c.h30 + 16 * CI * P) + z * P2;
+    const float F0 = p[0], F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F5 = z > 0 ? p[-P2] : c.fz0[CI];
+    const float F6 = z < B - 1 ? p[P2] : c.fz1[CI];
+    float Sm = F1 * w1;
+    Sm = fmaf(F2, w2, Sm);
+    Sm = fmaf(F3, w3, Sm);
+    Sm = fmaf(F4, w4, Sm);
+    Sm = fmaf(F5, w5, Sm);
+    Sm = fmaf(F6, w6, Sm);
+    const float Y = D != 0.0f ? fmaf(c.om * (r.bb[SL] + Sm), D, c.omm * F0) : F0;
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(Y * c.HR), c.rx, c32Cell(c.tx, c.ty0 + 16 * CI, z), 0, 0);
+  }
+  r.wq[SL % K] = c32LoadWy<CI>(c, slv + K);
+}
+template <int PREC, int SL> struct C32JacobiAll {
+  __device__ __forceinline__ static void run(C32Col (&r)[C32::NC], const float *h3, const C32Ctx &c)
+  {
+    int slv = SL;   // laundered: z and the addresses are formed here
+    asm volatile("" : "+s"(slv));
+    c32JacobiCell<PREC, SL, 0>(r[0], h3, c, slv);
+    c32JacobiCell<PREC, SL, 1>(r[1], h3, c, slv);
+    C32JacobiAll<PREC, SL + 1>::run(r, h3, c);
+  }
+};
+template <int PREC> struct C32JacobiAll<PREC, C32::B> {
+  __device__ __forceinline__ static void run(C32Col (&)[C32::NC], const float *, const C32Ctx &) {}
+};
+
+/* Records of register slots SL .. SL+7 of column CI into registers: one
+ * group of loads in flight at a time keeps the prologue within the register
+ * budget. */
+template <int SL, int CI>
+__device__ __forceinline__ void c32LoadRecords(C32Col &r, const __amdgpu_buffer_rsrc_t &rr, C32Ctx &c, uint16_t *wxp,
+                                               uint32_t &w6bits)
+{
+  constexpr int B = C32::B, S = C32::S, NF = B * B;
+  const int ty = c.ty0 + 16 * CI, o = c.o0 + 16 * CI;
+  uint4 v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                         rr, (uint32_t)(c.tx + B * ty) * 16u, (uint32_t)(SL + k) * (NF * 16u), 0));
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int sl = SL + k;
+    /* copies through an empty asm: the kept words get registers of their
+     * own instead of holding the whole 4-register load tuple live */
+    uint32_t vx = v[k].x, vz = v[k].z, vw = v[k].w;
+    asm volatile("" : "+v"(vx), "+v"(vz), "+v"(vw));
+    r.c0[sl] = vx;
+    if (sl & 1) r.c5[sl / 2] |= vz << 16;
+    else r.c5[sl / 2] = vz & 0xffffu;
+    int slv = sl;   // laundered: z is formed here, not precomputed for every slot
+    asm volatile("" : "+s"(slv));
+    const int z = (slv - o + 4 * B) & (B - 1);
+    if (c.tx == B - 1) wxp[z * B + ty] = (uint16_t)(vz >> 16);
+    if (z == B - 1) w6bits = vw;
+    const bool sk = segmentAllNonFluid<S>(((uint16_t)vw & (CELL_SOLID | CELL_VOID)) != 0);
+    c.skip[CI] |= (sk ? 1u : 0u) << sl;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+/* x of column CI into the halo array (h*x, 0 on skipped segments) and b into
+ * its registers. */
+template <int CI>
+__device__ __forceinline__ void c32StageX(C32Col &r, float *h3, const C32Ctx &c, float h)
+{
+  constexpr int B = C32::B, P = C32::P, P2 = C32::P2;
+  const int o = c.o0 + 16 * CI;
+#pragma unroll
+  for (int sl = 0; sl < B; sl++) {
+    int slv = sl;
+    asm volatile("" : "+s"(slv));
+    const int z = (slv - o + 4 * B) & (B - 1);
+    const uint32_t q = c32Cell(c.tx, c.ty0 + 16 * CI, z);
+    const float xv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rx, q, 0, 0));
+    r.bb[sl] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rb, q, 0, 0));
+    h3[c.h30 + 16 * CI * P + z * P2] = ((c.skip[CI] >> sl) & 1u) ? 0.0f : xv * h;
+  }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(512) k_relax_c32h(LevelArgs a, const int *__restrict__ blocks,
+                                                    float *__restrict__ X, const float *__restrict__ Bv,
+                                                    float omega, int reverse, const int *done)
+{
+  constexpr int B = C32::B, S = C32::S, NF = B * B, P = C32::P, P2 = C32::P2, NT = C32::NT;
+  __shared__ float h3[C32::N];
+  __shared__ uint16_t wxp[NF];
+  const int bid = blocks[xcdListIndex()];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const int isDone = *done;
+  asm volatile("" ::"s"(lev), "s"(isDone));
+  if (isDone) return;
+  const float h = (float)(1 << lev);
+  const int tx = threadIdx.x % B, ty = threadIdx.x / B;   // ty in [0, 16)
+  /* halo face values of this thread's face cells (u, w) = (tx, ty) and (tx, ty + 16) */
+  int fkind[6];
+  int64_t foff[6];
+  float fv[2][6];
+  const bool same = sameLevelFaces<B>(a, bid, lev, h, off, X, fv[0], fkind, foff, threadIdx.x) &&
+                    sameLevelFaces<B>(a, bid, lev, h, off, X, fv[1], fkind, foff, threadIdx.x + NT);
+  if (same) {
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int f = 0; f < 6; f++) fv[k][f] = sameLevelFaceValue(fv[k][f], fkind[f], h);
+  } else {
+    float *scratch = h3 + (C32::N - 6 * NF);   // overwritten by x values below
+    stageFaces<B>(scratch, a, bid, lev, h, off, X, fkind, foff, threadIdx.x);
+    stageFaces<B>(scratch, a, bid, lev, h, off, X, fkind, foff, threadIdx.x + NT);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int f = 0; f < 6; f++) fv[k][f] = scratch[f * NF + tx + B * (ty + 16 * k)];
+    __syncthreads();
+  }
+  C32Ctx c;
+  c.tx = tx;
+  c.ty0 = ty;
+  c.o0 = tx / S + ty;
+  c.h30 = (ty + 1) * P + tx + 1;
+  c.rb = blockRsrc(Bv + off, B * B * B * 4);
+  c.rx = blockRsrc(X + off, B * B * B * 4);
+  c.rw = blockRsrc(a.pack16y + off, B * B * B * 4);
+  c.wxp = wxp;
+  if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
+  c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    c.fz0[k] = fv[k][4];
+    c.fz1[k] = fv[k][5];
+    c.skip[k] = 0;
+  }
+  /* coefficient records, then x and b */
+  C32Col r[C32::NC];
+  uint32_t w6b0 = 0, w6b1 = 0;
+  const __amdgpu_buffer_rsrc_t rr = blockRsrc(a.pack16 + off, B * B * B * 16);
+  c32LoadRecords<0, 0>(r[0], rr, c, wxp, w6b0);
+  c32LoadRecords<8, 0>(r[0], rr, c, wxp, w6b0);
+  c32LoadRecords<16, 0>(r[0], rr, c, wxp, w6b0);
+  c32LoadRecords<24, 0>(r[0], rr, c, wxp, w6b0);
+  c32LoadRecords<0, 1>(r[1], rr, c, wxp, w6b1);
+  c32LoadRecords<8, 1>(r[1], rr, c, wxp, w6b1);
+  c32LoadRecords<16, 1>(r[1], rr, c, wxp, w6b1);
+  c32LoadRecords<24, 1>(r[1], rr, c, wxp, w6b1);
+  c.w6top[0] = hi16<PREC>(w6b0);
+  c.w6top[1] = hi16<PREC>(w6b1);
+  c32StageX<0>(r[0], h3, c, h);
+  c32StageX<1>(r[1], h3, c, h);
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int w = ty + 16 * k;
+    h3[w * P2 + (tx + 1) * P] = fv[k][0];
+    h3[w * P2 + (tx + 1) * P + B + 1] = fv[k][1];
+    h3[w * P2 + tx + 1] = fv[k][2];
+    h3[w * P2 + (B + 1) * P + tx + 1] = fv[k][3];
+  }
+  __syncthreads();
+  /* sweep k is reversed iff reverse == (k & 1) (src/msbg3.cpp:1157); the
+   * register arrays are mirrored around each reversed sweep */
+#pragma unroll 1
+  for (int k = 0; k < 4; k++) {
+    const bool fwd = (reverse != 0) != ((k & 1) != 0);
+    if (!fwd) {
+      c32Mirror(r[0]);
+      c32Mirror(r[1]);
+    }
+    c32Sweep<PREC>(r, h3, c, fwd);
+    if (!fwd) {
+      c32Mirror(r[0]);
+      c32Mirror(r[1]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C32::K; k++) {
+    r[0].wq[k] = c32LoadWy<0>(c, k);
+    r[1].wq[k] = c32LoadWy<1>(c, k);
+  }
+  C32JacobiAll<PREC, 0>::run(r, h3, c);
 }
 
 __device__ __forceinline__ double blockReduceSum(double v, double *scratch)
@@ -1387,8 +1846,12 @@ struct Dev {
   float *omegaOf = nullptr;
   BatchDev *batch = nullptr;
   LearnedCtx *learned = nullptr;   // buffers and graph of the learned-correction solve
-  /* 32^3 column smoother: coefficient records over the MG0 pool (LevelArgs::pack) */
+  /* 32^3 column smoother: coefficient records over the MG0 pool (LevelArgs::pack,
+   * LevelArgs::pack16 for 16-bit operands, built for precision pack16Prec) */
   float4 *pack32 = nullptr;
+  uint4 *pack16 = nullptr;
+  uint32_t *pack16y = nullptr;
+  int pack16Prec = MSBGX_PREC_FP32;
   /* FLIP caller steps: velocity channels (MG0 pool layout) and the level-0 block list */
   float *vel[3] = {nullptr, nullptr, nullptr};
   int *level0 = nullptr;
@@ -1470,6 +1933,8 @@ void deviceFree(Grid &g)
     freeLearned(d.learned);
     freeFlip(d);
     if (d.pack32) (void)hipFree(d.pack32);
+    if (d.pack16) (void)hipFree(d.pack16);
+    if (d.pack16y) (void)hipFree(d.pack16y);
     if (BatchDev *b = d.batch) {
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
       f(b->cProbL); f(b->cbL); f(b->ceL); f(b->cBegL); f(b->cEndL);
@@ -1498,6 +1963,9 @@ static LevelArgs levelArgs(const Grid &g, int L)
   a.probOf = dd ? dd->probOf : nullptr;
   a.omegaOf = dd ? dd->omegaOf : nullptr;
   a.pack = (L == 0 && g.bsx0 == 32 && dd) ? dd->pack32 : nullptr;
+  a.pack16 = (L == 0 && g.bsx0 == 32 && dd && a.prec != MSBGX_PREC_FP32 && dd->pack16Prec == a.prec) ? dd->pack16
+                                                                                                      : nullptr;
+  a.pack16y = a.pack16 ? dd->pack16y : nullptr;
   return a;
 }
 
@@ -1557,6 +2025,22 @@ static int buildPack32(Grid &g)
   HIPCHK(hipMemcpy(dl, l0.data(), l0.size() * sizeof(int), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pack_col32, dim3((unsigned)l0.size()), dim3(1024), 0, d.stream, levelArgs(g, 0), dl,
                      m.d_ch[C_X], d.pack32);
+  /* 16-bit records for the register-resident smoother when the operands are 16-bit */
+  const LevelArgs a0 = levelArgs(g, 0);
+  d.pack16Prec = MSBGX_PREC_FP32;
+  if (a0.prec != MSBGX_PREC_FP32) {
+    const size_t nc = std::max<size_t>((size_t)m.ncells, 1);
+    if (!d.pack16 && hipMalloc((void **)&d.pack16, nc * sizeof(uint4)) != hipSuccess) d.pack16 = nullptr;
+    if (!d.pack16y && hipMalloc((void **)&d.pack16y, nc * sizeof(uint32_t)) != hipSuccess) d.pack16y = nullptr;
+    if (!d.pack16 || !d.pack16y) {
+      (void)hipFree(dl);
+      setError("out of device memory for the 16-bit smoother records");
+      return MSBGX_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_pack_col32h, dim3((unsigned)l0.size()), dim3(1024), 0, d.stream, a0, dl, m.d_ch[C_X],
+                       d.pack16, d.pack16y);
+    d.pack16Prec = a0.prec;
+  }
   const hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(d.stream);
   (void)hipFree(dl);
@@ -1660,6 +2144,9 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   }
   HIPCHK(hipStreamSynchronize(d.stream));
   if (d.pack32) { (void)hipFree(d.pack32); d.pack32 = nullptr; }   // the pool layout may have changed
+  if (d.pack16) { (void)hipFree(d.pack16); d.pack16 = nullptr; }
+  if (d.pack16y) { (void)hipFree(d.pack16y); d.pack16y = nullptr; }
+  d.pack16Prec = MSBGX_PREC_FP32;
   return buildPack32(g);
 }
 
@@ -1672,6 +2159,17 @@ static void launchRelaxB(const Grid &g, int L, const DevList &l, float *X, const
   if (l.n == 0) return;
   const LevelArgs a = levelArgs(g, L);
   if constexpr (B == 32) {
+    /* 16-bit operands: the register-resident kernel, unless smoother variant 1
+     * (the record-streaming kernel) was chosen for this block size */
+    if (a.pack16 && g.smootherVariant[0] != 1) {
+      if (a.prec == MSBGX_PREC_FP16)
+        hipLaunchKernelGGL(k_relax_c32h<MSBGX_PREC_FP16>, dim3(l.n), dim3(C32::NT), 0, s, a, l.ptr, X, Bv, g.omega,
+                           reverse, done);
+      else
+        hipLaunchKernelGGL(k_relax_c32h<MSBGX_PREC_BF16>, dim3(l.n), dim3(C32::NT), 0, s, a, l.ptr, X, Bv, g.omega,
+                           reverse, done);
+      return;
+    }
     if (!a.pack) return;   // records are built at upload for every 32^3 MG0; never reached without them
   }
   if constexpr (B <= 16) {
@@ -2621,8 +3119,9 @@ int deviceSolveBatch(Grid &m, const msbgx_solve_params &p, int *iters, float *re
 
 /* Picks, by measurement on this handle's own problem:
  *  - the smoother kernel per block size (column-resident or LDS tile; the
- *    tile form exists for FP32 operands and blocks up to 16^3), timing the
- *    two colour passes over every MG level's relaxation lists;
+ *    tile form exists for FP32 operands and blocks up to 16^3; for 32^3
+ *    blocks with 16-bit operands, register-resident or record-streaming),
+ *    timing the two colour passes over every MG level's relaxation lists;
  *  - the number of PCG iterations per hipGraph replay (1, 2, 4, 8, 16),
  *    timing a whole solve (its second run, so graph capture is excluded).
  * Both choices reorder nothing inside a kernel, so the solver's results are
@@ -2635,7 +3134,9 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
   for (int lev = 0; lev < g.nLevels; lev++) {
     g.smootherVariant[lev] = 0;
     const int B = g.bsx0 >> lev;
-    if (B > 16 || g.prec != MSBGX_PREC_FP32) continue;
+    const bool tile = B <= 16 && g.prec == MSBGX_PREC_FP32;                   // column vs LDS tile
+    const bool c32 = B == 32 && lev == 0 && levelArgs(g, 0).pack16 != nullptr;  // registers vs records
+    if (!tile && !c32) continue;
     int nBlocks = 0;
     for (int L = 0; L <= lev; L++) nBlocks += g.mg[L].dRelax[0][lev].n + g.mg[L].dRelax[1][lev].n;
     if (nBlocks == 0) continue;
@@ -2657,8 +3158,7 @@ int deviceAutotune(Grid &g, const msbgx_solve_params &p, int *chunkOut, int *var
     g.smootherVariant[lev] = t[1] < 0.95f * t[0] ? 1 : 0;
     if (g.smootherVariant[lev]) mask |= 1 << lev;
   }
-  if (d.graph) { (void)hipGraphExecDestroy(d.graph); d.graph = nullptr; }
-  if (d.batch && d.batch->graph) { (void)hipGraphExecDestroy(d.batch->graph); d.batch->graph = nullptr; }
+  deviceInvalidateGraphs(g);   // the graphs hold the previous kernel choices
 
   const int P = g.batched ? g.nProb : 1;
   std::vector<int> it(P);

@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <deque>
 #include <memory>
 #include <queue>
@@ -117,6 +118,50 @@ static float denseRhs(void *u, int x, int y, int z) { auto *d = (DenseInput *)u;
 
 static inline int simdWidth(int bsx) { return bsx >= 8 ? 8 : bsx; }   // src/msbg3.cpp:1700-1743
 
+/* Precision form (a build addition, no reference counterpart): the
+ * smoother's face coefficients and inverse diagonal rounded to 16 bits with
+ * round-to-nearest-even and read back exactly.  prec 1: IEEE binary16,
+ * prec 2: bfloat16 (NaN stays NaN). */
+static float roundHalf(float f)
+{
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+  uint32_t h;
+  if (ax > 0x7f800000u) h = sign | 0x7e00u;                      // NaN
+  else if (ax >= 0x477ff000u) h = sign | 0x7c00u;                // >= 65520: infinity
+  else if (ax < 0x38800000u) {                                    // below 2^-14: subnormal or zero
+    float a;
+    std::memcpy(&a, &ax, 4);
+    h = sign | (uint32_t)std::nearbyint((double)a * 16777216.0);  // units of 2^-24, ties to even
+  } else {
+    const uint32_t e = (ax >> 23) - 127 + 15, m = ax & 0x7fffffu;
+    h = (e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    h |= sign;
+  }
+  const uint32_t hs = h & 0x8000u, he = (h >> 10) & 0x1fu, hm = h & 0x3ffu;
+  float v;
+  if (he == 0) v = std::ldexp((float)hm, -24);
+  else if (he == 31) v = hm ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+  else v = std::ldexp((float)(hm | 0x400u), (int)he - 25);
+  return hs ? -v : v;
+}
+static float roundBf16(float f)
+{
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  uint32_t b;
+  if ((x & 0x7fffffffu) > 0x7f800000u) b = (x >> 16) | 0x40u;
+  else b = (x + 0x7fffu + ((x >> 16) & 1u)) >> 16;
+  b <<= 16;
+  float v;
+  std::memcpy(&v, &b, 4);
+  return v;
+}
+static inline float round16(float v, int prec) { return prec == 1 ? roundHalf(v) : (prec == 2 ? roundBf16(v) : v); }
+
 struct Oracle {
   int sx0, sy0, sz0, bsx0, nLevels, nbx, nby, nbz, nBlocks;
   int domainBc[3][2];
@@ -128,6 +173,7 @@ struct Oracle {
   std::vector<float> zeroBlk;
   std::vector<CellFlags> zeroBlkF;
   float omegaDefault = 6.f / 7.f, omega = 6.f / 7.f;
+  int smPrec = 0;   // precision form: 16-bit smoother operands (0 FP32, 1 FP16, 2 BF16)
   int nBlockIter = 5;
   int nThreads = 1;
 
@@ -678,6 +724,11 @@ struct Oracle {
         }
       }
 
+    /* precision form: the smoother reads its face coefficients in 16 bits */
+    if ((op & OP_RELAX) && smPrec && blockFC)
+      for (int k = 0; k < 6; k++)
+        for (int v = 0; v < nv; v++) Wt[k][v] = round16(Wt[k][v], smPrec);
+
     /* sweeps (src/msbg3.cpp:1109-1506) */
     const float HR = 1.f / h;
     const float om = omega, oneMinusOm = 1.0f - om;
@@ -700,7 +751,7 @@ struct Oracle {
               const float F1 = X[H(x - 1, vy, vz)], F2 = X[H(x + 1, vy, vz)];
               const float F3 = X[H(x, vy - 1, vz)], F4 = X[H(x, vy + 1, vz)];
               const float F5 = X[H(x, vy, vz - 1)], F6 = X[H(x, vy, vz + 1)];
-              const float D = dataD[v];
+              const float D = (op & OP_RELAX) ? round16(dataD[v], smPrec) : dataD[v];
               float Sm;
               if (weighted) {
                 Sm = F1 * Wt[0][v];
@@ -1159,6 +1210,13 @@ void orc_prepare_dense(void *h, int nx, int ny, int nz, const uint16_t *flags, c
   ((Oracle *)h)->prepare(cb, true);
 }
 float orc_omega(void *h) { return ((Oracle *)h)->omega; }
+/* Precision form: relax with 16-bit smoother operands (0 FP32, 1 FP16, 2 BF16). */
+void orc_set_smoother_precision(void *h, int prec) { ((Oracle *)h)->smPrec = prec; }
+/* The rounding the precision form applies, element-wise (test hook). */
+void orc_round16(int prec, const float *in, float *out, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++) out[i] = round16(in[i], prec);
+}
 
 void orc_block_info(void *h, int L, int *level, uint32_t *flags)
 {

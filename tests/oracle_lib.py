@@ -24,6 +24,8 @@ for name, res, args in [
     ("orc_prepare_scene", None, [P, C.POINTER(Scene)]),
     ("orc_prepare_dense", None, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
     ("orc_omega", C.c_float, [P]),
+    ("orc_set_smoother_precision", None, [P, C.c_int]),
+    ("orc_round16", None, [C.c_int, P, P, C.c_int64]),
     ("orc_block_info", None, [P, C.c_int, P, P]),
     ("orc_relax_list", C.c_int, [P, C.c_int, P, C.c_int]),
     ("orc_get_block", C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
@@ -53,6 +55,14 @@ for name, res, args in [
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def round16(values, prec):
+    """The oracle's round-to-nearest-even to FP16 (prec 1) / BF16 (prec 2) and back."""
+    a = np.ascontiguousarray(values, np.float32)
+    out = np.empty_like(a)
+    _lib.orc_round16(prec, _p(a), _p(out), a.size)
+    return out
 
 
 def level_map_from_scene(scene, bsx, levels, mode, d0):
@@ -101,6 +111,11 @@ class Oracle:
 
     def omega(self):
         return _lib.orc_omega(self._h)
+
+    def set_smoother_precision(self, prec):
+        """Precision form: the relax reads its coefficients rounded to 16 bits
+        (0 FP32, 1 FP16, 2 BF16)."""
+        _lib.orc_set_smoother_precision(self._h, prec)
 
     def block_info(self, mg=0):
         lv = np.zeros(self.n_blocks, np.int32)
