@@ -214,6 +214,26 @@ int msbgx_net_collect(msbgx_net *n, msbgx_grid *g, const msbgx_solve_params *p, 
                       int blocks_per_pair, uint64_t seed, int *pairs);
 int msbgx_net_set_mode(msbgx_net *n, int mode);
 int msbgx_net_get_mode(const msbgx_net *n);
+/* Coarse form: how the network's block values extend over a PCG block's
+ * unknowns -- constant (the block's own value, the default), or trilinear
+ * between block centres (cell-centred), the weights renormalised over the
+ * PCG blocks among the eight nearest centres.  Training is the same for
+ * both (targets are block means). */
+#define MSBGX_NET_PROLONG_CONSTANT 0
+#define MSBGX_NET_PROLONG_LINEAR 1
+int msbgx_net_set_prolongation(msbgx_net *n, int kind);
+int msbgx_net_get_prolongation(const msbgx_net *n);
+/* Coarse form: the number of scales (1 to MSBGX_NET_MAX_SCALES, default 1)
+ * the network is applied at -- the dense block grid coarsened 1x, 2x, 4x
+ * (means over the PCG blocks of each cell), one direction per scale.  With
+ * one scale the correction is scaled by its Galerkin step; with several it
+ * is the combination of the directions that minimises the A-norm error
+ * over their span (coefficients from their Gram matrix, FP64).  Training
+ * samples (msbgx_net_collect) are drawn at every scale; changing the number
+ * drops the collected samples. */
+#define MSBGX_NET_MAX_SCALES 3
+int msbgx_net_set_coarse_scales(msbgx_net *n, int scales);
+int msbgx_net_get_coarse_scales(const msbgx_net *n);
 /* Stall test of msbgx_solve_pressure_learned: fall back when |r| drops by
  * less than `drop` (relative) over `window` iterations (0 <= window <= 16;
  * 0 disables the test; defaults per form, see above). */
@@ -224,8 +244,12 @@ int msbgx_net_num_samples(const msbgx_net *n);
 int msbgx_net_train(msbgx_net *n, int steps, int batch, float lr, uint64_t seed, double *loss_first,
                     double *loss_last);
 double msbgx_net_last_train_ms(const msbgx_net *n);
-/* Per-kernel entry point: ch_out += N(ch_in) on the level-0 PCG blocks
- * (MG level 0 channels; the halo is read from level-0 neighbours). */
+/* Per-kernel entry point on the level-0 PCG blocks (MG level 0 channels).
+ * Fine form: ch_out += N(ch_in), the halo read from level-0 neighbours
+ * (16^3 blocks only).  Coarse form: ch_out += P N(R ch_in), R the mean
+ * over each PCG block's cells on the dense block grid (0 elsewhere), N the
+ * network over 16^3 tiles of that grid, P the net's prolongation onto the
+ * block's unknowns. */
 int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out);
 /* Measurement: mean duration of one forward 16 -> 16 MFMA layer launch over
  * `blocks` patches (HIP events, `reps` launches) and its algorithmic FLOP

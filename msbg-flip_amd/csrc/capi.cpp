@@ -575,13 +575,44 @@ int msbgx_net_train(msbgx_net *n, int steps, int batch, float lr, uint64_t seed,
   return netTrain(*netOf(n), steps, batch, lr, seed, loss_first, loss_last);
 }
 
+int msbgx_net_set_prolongation(msbgx_net *n, int kind)
+{
+  if (!n || (kind != MSBGX_NET_PROLONG_CONSTANT && kind != MSBGX_NET_PROLONG_LINEAR)) {
+    setError("bad argument");
+    return MSBGX_EINVAL;
+  }
+  return netSetProlongation(*reinterpret_cast<Net *>(n), kind);
+}
+
+int msbgx_net_get_prolongation(const msbgx_net *n)
+{
+  return n ? netProlongation(*reinterpret_cast<const Net *>(n)) : MSBGX_EINVAL;
+}
+
+int msbgx_net_set_coarse_scales(msbgx_net *n, int scales)
+{
+  if (!n || scales < 1 || scales > MSBGX_NET_MAX_SCALES) {
+    setError("bad argument");
+    return MSBGX_EINVAL;
+  }
+  return netSetScales(*reinterpret_cast<Net *>(n), scales);
+}
+
+int msbgx_net_get_coarse_scales(const msbgx_net *n)
+{
+  return n ? netScales(*reinterpret_cast<const Net *>(n)) : MSBGX_EINVAL;
+}
+
 double msbgx_net_last_train_ms(const msbgx_net *n) { return n ? netLastTrainMs(*reinterpret_cast<const Net *>(n)) : 0.0; }
 
 int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out)
 {
   if (!n) { setError("null argument"); return MSBGX_EINVAL; }
   if (int rc = checkLearnedGrid(g, n)) return rc;
-  if (g->bsx0 != 16) { setError("msbgx_net_apply works on 16^3 blocks"); return MSBGX_EINVAL; }
+  if (g->bsx0 != 16 && netMode(*netOf(n)) != MSBGX_NET_COARSE) {
+    setError("msbgx_net_apply: the fine form works on 16^3 blocks");
+    return MSBGX_EINVAL;
+  }
   if (ch_in < 0 || ch_in > MSBGX_CH_Q || ch_out < 0 || ch_out > MSBGX_CH_Q || ch_in == ch_out) {
     setError("bad channel");
     return MSBGX_EINVAL;

@@ -686,21 +686,49 @@ __global__ void __launch_bounds__(256) k_nn_out_dense(const uint16_t *__restrict
   }
 }
 
-/* zc = coarse[bid] on the unknowns of each PCG block (0 elsewhere in the
- * block); a non-finite value is dropped and flagged. */
-__global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restrict__ coarse,
-                                                           const int *__restrict__ blocks,
+/* Coarse grid at scale f: cell C covers the f^3 blocks f C .. f C + f - 1
+ * of the block grid.  wgt[C] counts the PCG blocks (level-0 blocks with
+ * unknowns) among them; cells with none take no part. */
+
+/* coarse[C] = mean of fine[] over the PCG blocks of C's 2^3 children on the
+ * next finer coarse grid (weights wf), 0 without any (skipped while *skip) */
+__global__ void k_nn_coarsen(const float *__restrict__ fine, const float *__restrict__ wf, int fx, int fy, int fz,
+                             float *__restrict__ coarse, int cx, int cy, int cz, const int *skip)
+{
+  if (skip && *skip) return;
+  const int n = cx * cy * cz;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int x = i % cx, y = (i / cx) % cy, z = i / (cx * cy);
+    float sw = 0.0f, sv = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int X = 2 * x + (k & 1), Y = 2 * y + ((k >> 1) & 1), Z = 2 * z + (k >> 2);
+      if (X >= fx || Y >= fy || Z >= fz) continue;
+      const int j = X + fx * (Y + fy * Z);
+      sw += wf[j];
+      sv = fmaf(wf[j], fine[j], sv);
+    }
+    coarse[i] = sw > 0.0f ? sv / sw : 0.0f;
+  }
+}
+
+/* z = coarse[C] on the unknowns of each PCG block, C the scale-f cell that
+ * holds the block (0 elsewhere in the block); a non-finite value is dropped
+ * and flagged (st may be null: no skip flag, nothing flagged). */
+__global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restrict__ coarse, int f, int nbx,
+                                                           int nby, int sx, int sy, const int *__restrict__ blocks,
                                                            const int64_t *__restrict__ off,
                                                            const uint16_t *__restrict__ flags,
                                                            const float *__restrict__ D, int ncell,
                                                            float *__restrict__ zc, PcgState *st)
 {
-  if (st->skipNet) return;
+  if (st && st->skipNet) return;
   const int bid = blocks[blockIdx.x];
   const int64_t o = off[bid];
-  float v = coarse[bid];
+  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  float v = coarse[bx / f + sx * (by / f + sy * (bz / f))];
   if (!isfinite(v)) {
-    if (threadIdx.x == 0) st->bad = 1;
+    if (threadIdx.x == 0 && st) st->bad = 1;
     v = 0.0f;
   }
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
@@ -709,10 +737,215 @@ __global__ void __launch_bounds__(256) k_nn_prolong_blocks(const float *__restri
   }
 }
 
+/* z on the unknowns of each PCG block by trilinear interpolation between
+ * the centres of the scale-f cells (cell-centred: a cell at offset v in a
+ * block at offset j = 0..f-1 inside its scale cell sits
+ * (j B + v + 0.5) / (f B) - 0.5 scale cells from that cell's centre), the
+ * weights renormalised over the cells with PCG blocks (wgt > 0) among the
+ * eight nearest centres (air, coarser levels and cells outside the raster
+ * take no part).  A non-finite value among them is dropped and flagged. */
+__global__ void __launch_bounds__(256) k_nn_prolong_lin(const float *__restrict__ coarse,
+                                                        const float *__restrict__ wgt, int f, int nbx, int nby,
+                                                        int sx, int sy, int sz, const int *__restrict__ blocks,
+                                                        const int64_t *__restrict__ off,
+                                                        const uint16_t *__restrict__ flags,
+                                                        const float *__restrict__ D, int bsx,
+                                                        float *__restrict__ zc, PcgState *st)
+{
+  if (st && st->skipNet) return;
+  __shared__ float cv[27], cm[27];
+  const int bid = blocks[blockIdx.x];
+  const int b3[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
+  const int C[3] = {b3[0] / f, b3[1] / f, b3[2] / f};
+  if (threadIdx.x < 27) {
+    const int nx = C[0] + (int)threadIdx.x % 3 - 1, ny = C[1] + ((int)threadIdx.x / 3) % 3 - 1,
+              nz = C[2] + (int)threadIdx.x / 9 - 1;
+    float v = 0.0f, m = 0.0f;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < sx && ny < sy && nz < sz) {
+      const int nc = nx + sx * (ny + sy * nz);
+      if (wgt[nc] > 0.0f) {
+        v = coarse[nc];
+        m = 1.0f;
+        if (!isfinite(v)) {
+          if (st) st->bad = 1;
+          v = 0.0f;
+        }
+      }
+    }
+    cv[threadIdx.x] = v;
+    cm[threadIdx.x] = m;
+  }
+  __syncthreads();
+  const int64_t o = off[bid];
+  const int ncell = bsx * bsx * bsx;
+  const float inv = 1.0f / (float)(f * bsx);
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int v3[3] = {c % bsx, (c / bsx) % bsx, c / (bsx * bsx)};
+    int lo[3];
+    float t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int pos = (b3[a] - f * C[a]) * bsx + v3[a];      // offset inside the scale cell
+      const float u = ((float)pos + 0.5f) * inv - 0.5f;      // in (-0.5, 0.5)
+      lo[a] = u < 0.0f ? 0 : 1;                              // 27-neighbourhood index of the lower centre
+      t[a] = u < 0.0f ? u + 1.0f : u;
+    }
+    float sw = 0.0f, sv = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int ix = lo[0] + (k & 1), iy = lo[1] + ((k >> 1) & 1), iz = lo[2] + (k >> 2);
+      const float w = ((k & 1) ? t[0] : 1.0f - t[0]) * ((k & 2) ? t[1] : 1.0f - t[1]) *
+                      ((k & 4) ? t[2] : 1.0f - t[2]);
+      const int q = ix + 3 * iy + 9 * iz;
+      sw = fmaf(w, cm[q], sw);
+      sv = fmaf(w * cm[q], cv[q], sv);
+    }
+    const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
+    zc[o + c] = unk ? (sw > 0.0f ? sv / sw : cv[13]) : 0.0f;
+  }
+}
+
+/* Directions of the coarse form at several scales: z[a] and q[a] = r - A z[a]. */
+constexpr int kMaxScales = 3;
+struct DirSet {
+  const float *z[kMaxScales];
+  const float *q[kMaxScales];
+};
+constexpr int kGramBlocks = 512;
+
+/* Partial Galerkin sums over [0, n) in FP64, unless *skip: for a <= b,
+ * <z_a, A z_b> = <z_a, r - q_b>, then <z_a, r>; value j of workgroup w at
+ * partials[j * gridDim.x + w]. */
+template <int K>
+__global__ void __launch_bounds__(256) k_nn_gram(DirSet ds, const float *__restrict__ r, int64_t n,
+                                                 double *partials, const int *skip)
+{
+  if (*skip) return;
+  constexpr int NG = K * (K + 1) / 2;
+  __shared__ double red[4];
+  double acc[NG + K];
+#pragma unroll
+  for (int j = 0; j < NG + K; j++) acc[j] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = r[i];
+    double zv[K], aq[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+      zv[a] = ds.z[a][i];
+      aq[a] = ri - (double)ds.q[a][i];
+    }
+    int j = 0;
+#pragma unroll
+    for (int a = 0; a < K; a++)
+#pragma unroll
+      for (int b = a; b < K; b++) acc[j++] += zv[a] * aq[b];
+#pragma unroll
+    for (int a = 0; a < K; a++) acc[NG + a] += zv[a] * ri;
+  }
+#pragma unroll
+  for (int j = 0; j < NG + K; j++) {
+    const double t = blockReduceSum(acc[j], red);
+    if (threadIdx.x == 0) partials[j * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+/* Galerkin coefficients c = G^-1 g of the K directions (G_ab = <z_a, A z_b>,
+ * g_a = <z_a, r>) into st->coef by Cholesky in FP64; a direction whose pivot
+ * is not positive beyond rounding (zero or dependent on the ones before it)
+ * gets coefficient 0. */
+template <int K>
+__global__ void __launch_bounds__(256) k_nn_gram_solve(const double *partials, int nPart, PcgState *st)
+{
+  if (st->done || st->skipNet) return;
+  constexpr int NG = K * (K + 1) / 2;
+  __shared__ double v[NG + K];
+  for (int j = 0; j < NG + K; j++) {
+    const double t = sumPartials(partials + (size_t)j * nPart, nPart);
+    if (threadIdx.x == 0) v[j] = t;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double G[K][K], L[K][K], y[K], c[K];
+  bool use[K];
+  int j = 0;
+  for (int a = 0; a < K; a++)
+    for (int b = a; b < K; b++) G[a][b] = G[b][a] = v[j++];
+  for (int a = 0; a < K; a++) {
+    use[a] = G[a][a] > 1e-300 && isfinite(G[a][a]) && isfinite(v[NG + a]);
+    for (int b = 0; b < K; b++) L[a][b] = 0.0;
+  }
+  for (int a = 0; a < K; a++) {
+    if (!use[a]) continue;
+    double d = G[a][a];
+    for (int k = 0; k < a; k++)
+      if (use[k]) d -= L[a][k] * L[a][k];
+    if (!(d > 1e-10 * G[a][a]) || !isfinite(d)) {
+      use[a] = false;
+      continue;
+    }
+    L[a][a] = sqrt(d);
+    for (int b = a + 1; b < K; b++) {
+      if (!use[b]) continue;
+      double e = G[b][a];
+      for (int k = 0; k < a; k++)
+        if (use[k]) e -= L[b][k] * L[a][k];
+      L[b][a] = e / L[a][a];
+    }
+  }
+  for (int a = 0; a < K; a++) {
+    y[a] = 0.0;
+    if (!use[a]) continue;
+    double e = v[NG + a];
+    for (int k = 0; k < a; k++)
+      if (use[k]) e -= L[a][k] * y[k];
+    y[a] = e / L[a][a];
+  }
+  for (int a = K - 1; a >= 0; a--) {
+    c[a] = 0.0;
+    if (!use[a]) continue;
+    double e = y[a];
+    for (int k = a + 1; k < K; k++)
+      if (use[k]) e -= L[k][a] * c[k];
+    c[a] = e / L[a][a];
+  }
+  for (int a = 0; a < K; a++) st->coef[a] = isfinite(c[a]) ? (float)c[a] : 0.0f;
+}
+
+/* zc = sum_a c_a z_a and rs = r - A zc = (1 - sum_a c_a) r + sum_a c_a q_a
+ * over the MG0 pool (z[0] may be zc and q[0] rs); with the network off,
+ * rs = r exactly (see k_nn_gamma). */
+template <int K>
+__global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *zc, float *rs, const float *__restrict__ r,
+                                                    int64_t n, const PcgState *st)
+{
+  if (st->done) return;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
+  if (st->skipNet) {
+    for (int64_t i = i0; i < n; i += di) rs[i] = r[i];
+    return;
+  }
+  float cf[K], cs = 0.0f;
+#pragma unroll
+  for (int a = 0; a < K; a++) {
+    cf[a] = st->coef[a];
+    cs += cf[a];
+  }
+  for (int64_t i = i0; i < n; i += di) {
+    float zv = 0.0f, rv = (1.0f - cs) * r[i];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+      zv = fmaf(cf[a], ds.z[a][i], zv);
+      rv = fmaf(cf[a], ds.q[a][i], rv);
+    }
+    zc[i] = zv;
+    rs[i] = rv;
+  }
+}
+
 /* z += zc over [0, n) unless *skip */
 __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, int64_t n, const int *skip)
 {
-  if (*skip) return;
+  if (skip && *skip) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     z[i] += zc[i];
 }
@@ -752,10 +985,11 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_dot_diff(const float *__rest
 }
 
 /* Coarse-form training sample of tile t: input patch of the dense residual
- * means, target = dense error means on PCG blocks (mask 1 there), both
- * divided by the target's RMS over the tile. */
+ * means, target = dense error means on the cells with PCG blocks
+ * (cmask > 0; mask 1 there), both divided by the target's RMS over the
+ * tile. */
 __global__ void __launch_bounds__(256) k_nn_tile_target(const float *__restrict__ cerr,
-                                                        const uint8_t *__restrict__ cmask, int nbx, int nby, int nbz,
+                                                        const float *__restrict__ cmask, int nbx, int nby, int nbz,
                                                         int ntx, int nty, int ox, int oy, int oz,
                                                         float *__restrict__ scale, float *__restrict__ tgt,
                                                         uint8_t *__restrict__ tmask)
@@ -767,7 +1001,7 @@ __global__ void __launch_bounds__(256) k_nn_tile_target(const float *__restrict_
   auto cellOf = [&](int c, int &idx) {
     const int cx = 16 * tx + (c & 15) - ox, cy = 16 * ty + ((c >> 4) & 15) - oy, cz = 16 * tz + (c >> 8) - oz;
     idx = cx + nbx * (cy + nby * cz);
-    return cx >= 0 && cy >= 0 && cz >= 0 && cx < nbx && cy < nby && cz < nbz && cmask[idx];
+    return cx >= 0 && cy >= 0 && cz >= 0 && cx < nbx && cy < nby && cz < nbz && cmask[idx] > 0.0f;
   };
   double acc = 0.0, cnt = 0.0;
   for (int c = threadIdx.x; c < nn::NCELL; c += blockDim.x) {
@@ -806,6 +1040,8 @@ struct NetBuffers {
 struct Net {
   uint64_t serial = 0;                           // unique per created net (graph keys)
   int mode = MSBGX_NET_FINE;                     // MSBGX_NET_FINE or MSBGX_NET_COARSE
+  int prolong = MSBGX_NET_PROLONG_CONSTANT;      // coarse form: extension of the block values
+  int scales = 1;                                // coarse form: block grid coarsened 1x, 2x, 4x
   int stallWin = 5;                              // fallback when |r| drops by less than
   float stallDrop = 0.1f;                        // stallDrop over stallWin iterations
   hipStream_t stream = nullptr;
@@ -1145,6 +1381,22 @@ int netSetMode(Net &n, int mode)
   return MSBGX_OK;
 }
 int netMode(const Net &n) { return n.mode; }
+int netSetProlongation(Net &n, int kind)
+{
+  n.prolong = kind;
+  return MSBGX_OK;
+}
+int netProlongation(const Net &n) { return n.prolong; }
+int netSetScales(Net &n, int k)
+{
+  if (k != n.scales) {
+    n.scales = k;
+    n.nS = 0;               // samples are drawn at every scale the net works on
+    n.maskCount.clear();
+  }
+  return MSBGX_OK;
+}
+int netScales(const Net &n) { return n.scales; }
 int netSetStall(Net &n, int window, float drop)
 {
   n.stallWin = window;
@@ -1189,10 +1441,17 @@ int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLa
 struct LearnedCtx {
   NetBuffers ib;                // inference buffers for the level-0 PCG blocks (or the coarse tiles)
   float *zprev = nullptr, *zplain = nullptr;
-  /* coarse form: dense block-resolution grids, tile counts, the prolonged
-   * correction and the residual it leaves (MG0 pool size) */
-  float *cin = nullptr, *cout = nullptr, *zc = nullptr, *rs = nullptr;
-  int ntx = 0, nty = 0, ntz = 0, ncoarse = 0;
+  /* coarse form: per scale f = 1, 2, 4 the dense grid of f^3-block cells
+   * (dims, 16^3 tiles covering it, network input and output, PCG blocks
+   * per cell) and its direction z with q = r - A z (MG0 pool size); scale
+   * 0's direction and residual are the correction zc and rs themselves */
+  struct Scale {
+    int f = 1, sx = 0, sy = 0, sz = 0, ntx = 0, nty = 0, ntz = 0, ncoarse = 0;
+    float *cin = nullptr, *cout = nullptr, *w = nullptr, *z = nullptr, *q = nullptr;
+  } sc[kMaxScales];
+  int nScales = 0;                 // scales allocated
+  float *zc = nullptr, *rs = nullptr;
+  double *gram = nullptr;          // Galerkin partials
   int64_t n0 = 0;
   hipGraphExec_t graph = nullptr;
   GraphKey graphKey;               // includes the serial of the net whose buffers the graph uses
@@ -1204,8 +1463,16 @@ static void freeLearned(LearnedCtx *c)
   freeBuffers(c->ib);
   if (c->zprev) (void)hipFree(c->zprev);
   if (c->zplain) (void)hipFree(c->zplain);
-  for (float *p : {c->cin, c->cout, c->zc, c->rs})
+  for (int k = 0; k < c->nScales; k++) {
+    for (float *p : {c->sc[k].cin, c->sc[k].cout, c->sc[k].w})
+      if (p) (void)hipFree(p);
+    if (k > 0)
+      for (float *p : {c->sc[k].z, c->sc[k].q})
+        if (p) (void)hipFree(p);
+  }
+  for (float *p : {c->zc, c->rs})
     if (p) (void)hipFree(p);
+  if (c->gram) (void)hipFree(c->gram);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   delete c;
 }
@@ -1226,7 +1493,13 @@ static void netCorrect(Grid &g, Net &n, LearnedCtx &c, hipStream_t s)
                      m0.d_ch[C_Z], d.state, (const uint8_t *)nullptr, (const int *)nullptr, (float *)nullptr);
 }
 
-/* ch_out += N(ch_in) on the level-0 PCG blocks (per-kernel entry point). */
+static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s);
+static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s);
+
+/* Per-kernel entry point on the level-0 PCG blocks: fine form
+ * ch_out += N(ch_in); coarse form ch_out += sum over the net's scales of
+ * P_f N(R_f ch_in) (R_f the mean over the PCG blocks of each f^3-block
+ * cell, P_f the net's prolongation from those cells). */
 int netApply(Net &n, Grid &g, int chIn, int chOut)
 {
   Dev &d = dev(g);
@@ -1235,6 +1508,17 @@ int netApply(Net &n, Grid &g, int chIn, int chOut)
   LearnedCtx &c = *d.learned;
   const int nb = g.nActive0;
   if (nb == 0) return MSBGX_OK;
+  if (n.mode == MSBGX_NET_COARSE) {
+    if (int rc = coarseSetup(g, c, n.scales, d.stream)) return rc;
+    HIPCHK(hipStreamSynchronize(n.stream));
+    coarseNetwork(g, n, c, m0.d_ch[chIn], nullptr, d.stream);
+    for (int k = 0; k < n.scales; k++)
+      hipLaunchKernelGGL(k_nn_add, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[chOut], c.sc[k].z,
+                         (int64_t)m0.ncells, (const int *)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(d.stream));
+    return MSBGX_OK;
+  }
   if (int rc = allocBuffers(c.ib, nb, false, d.stream)) return rc;
   HIPCHK(hipStreamSynchronize(n.stream));
   hipLaunchKernelGGL(k_nn_gather, dim3(nb), dim3(256), 0, d.stream, m0.d_ch[chIn], d.dActive0, m0.d_level, m0.d_off,
@@ -1248,58 +1532,151 @@ int netApply(Net &n, Grid &g, int chIn, int chOut)
   return MSBGX_OK;
 }
 
-/* Buffers of the coarse form on grid g: dense block-resolution grids, the
- * tiles of 16^3 coarse cells covering the raster, zc and rs. */
-static int coarseSetup(Grid &g, LearnedCtx &c, hipStream_t s)
+/* Buffers of the coarse form on grid g for the first `scales` scales: the
+ * dense grids, their tiles and PCG block counts, the directions and the
+ * residuals they leave, zc and rs. */
+static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
 {
-  if (c.cin) return MSBGX_OK;
-  c.ntx = (g.nbx + 15) / 16;
-  c.nty = (g.nby + 15) / 16;
-  c.ntz = (g.nbz + 15) / 16;
-  c.ncoarse = g.nbx * g.nby * g.nbz;
   const size_t nc = (size_t)g.mg[0].ncells;
-  if (int rc = dalloc(&c.cin, c.ncoarse)) return rc;
-  if (int rc = dalloc(&c.cout, c.ncoarse)) return rc;
-  if (int rc = dalloc(&c.zc, nc)) return rc;
-  if (int rc = dalloc(&c.rs, nc)) return rc;
-  HIPCHK(hipMemsetAsync(c.zc, 0, nc * sizeof(float), s));
-  HIPCHK(hipMemsetAsync(c.rs, 0, nc * sizeof(float), s));
-  if (int rc = allocBuffers(c.ib, c.ntx * c.nty * c.ntz, false, s)) return rc;
+  if (c.nScales >= scales) return MSBGX_OK;
+  HIPCHK(hipStreamSynchronize(s));
+  if (!c.zc) {
+    if (int rc = dalloc(&c.zc, nc)) return rc;
+    if (int rc = dalloc(&c.rs, nc)) return rc;
+    if (int rc = dalloc(&c.gram, (size_t)(kMaxScales * (kMaxScales + 3) / 2) * kGramBlocks)) return rc;
+    HIPCHK(hipMemset(c.zc, 0, nc * sizeof(float)));
+    HIPCHK(hipMemset(c.rs, 0, nc * sizeof(float)));
+  }
+  /* PCG blocks per cell, scale by scale */
+  std::vector<float> w((size_t)g.nbx * g.nby * g.nbz, 0.0f);
+  for (int b : g.active0) w[b] = 1.0f;
+  int px = g.nbx, py = g.nby, pz = g.nbz;
+  for (int k = 0; k < scales; k++) {
+    LearnedCtx::Scale &q = c.sc[k];
+    if (k > 0) {
+      const int cx = (px + 1) / 2, cy = (py + 1) / 2, cz = (pz + 1) / 2;
+      std::vector<float> wc((size_t)cx * cy * cz, 0.0f);
+      for (int z = 0; z < pz; z++)
+        for (int y = 0; y < py; y++)
+          for (int x = 0; x < px; x++) wc[x / 2 + cx * (y / 2 + cy * (z / 2))] += w[x + px * (y + py * z)];
+      w.swap(wc);
+      px = cx;
+      py = cy;
+      pz = cz;
+    }
+    if (k < c.nScales) continue;
+    q.f = 1 << k;
+    q.sx = px;
+    q.sy = py;
+    q.sz = pz;
+    q.ntx = (px + 15) / 16;
+    q.nty = (py + 15) / 16;
+    q.ntz = (pz + 15) / 16;
+    q.ncoarse = px * py * pz;
+    if (int rc = dalloc(&q.cin, q.ncoarse)) return rc;
+    if (int rc = dalloc(&q.cout, q.ncoarse)) return rc;
+    if (int rc = dalloc(&q.w, q.ncoarse)) return rc;
+    HIPCHK(hipMemcpy(q.w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (k == 0) {
+      q.z = c.zc;
+      q.q = c.rs;
+    } else {
+      if (int rc = dalloc(&q.z, nc)) return rc;
+      if (int rc = dalloc(&q.q, nc)) return rc;
+      HIPCHK(hipMemset(q.z, 0, nc * sizeof(float)));
+      HIPCHK(hipMemset(q.q, 0, nc * sizeof(float)));
+    }
+    c.nScales = k + 1;
+  }
+  /* the network buffers are shared by the scales, one after another */
+  if (int rc = allocBuffers(c.ib, c.sc[0].ntx * c.sc[0].nty * c.sc[0].ntz, false, s)) return rc;
   HIPCHK(hipStreamSynchronize(s));
   return MSBGX_OK;
 }
 
-/* Coarse form, learned path (skipped while skipNet): zc = P N(R r) with R the
- * per-block mean and P the constant extension over a block's unknowns; then
+/* Directions of the coarse form, z_f = P_f N(R_f src) for the net's scales
+ * f = 1, 2, 4: R_1 the per-block mean over all the block's cells on the
+ * dense block grid (0 for blocks without PCG unknowns), R_2f the mean of
+ * R_f over the PCG blocks of each 2^3 cells, N the network over 16^3 tiles
+ * of each grid, P_f the net's prolongation onto the PCG blocks' unknowns
+ * (skipped while st->skipNet; st may be null). */
+static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  const int *skip = st ? &st->skipNet : nullptr;
+  const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
+  if (g.nActive0 == 0) return;
+  LearnedCtx::Scale &s0 = c.sc[0];
+  hipLaunchKernelGGL(k_nn_fill, dim3((s0.ncoarse + 255) / 256), dim3(256), 0, s, s0.cin, s0.ncoarse, 0.0f, skip);
+  hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, src, d.dActive0, m0.d_off, m0.d_flags,
+                     m0.d_D, ncell, 0, s0.cin, skip);
+  for (int k = 0; k < n.scales; k++) {
+    LearnedCtx::Scale &q = c.sc[k];
+    if (k > 0) {
+      const LearnedCtx::Scale &p = c.sc[k - 1];
+      hipLaunchKernelGGL(k_nn_coarsen, dim3((q.ncoarse + 255) / 256), dim3(256), 0, s, p.cin, p.w, p.sx, p.sy, p.sz,
+                         q.cin, q.sx, q.sy, q.sz, skip);
+    }
+    const int nt = q.ntx * q.nty * q.ntz;
+    hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, q.cin, q.sx, q.sy, q.sz, q.ntx, q.nty, 0, 0, 0,
+                       (const float *)nullptr, c.ib.X0, skip);
+    netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
+    hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, q.sx, q.sy, q.sz, q.ntx, q.nty,
+                       q.cout, skip);
+    if (n.prolong == MSBGX_NET_PROLONG_LINEAR)
+      hipLaunchKernelGGL(k_nn_prolong_lin, dim3(g.nActive0), dim3(256), 0, s, q.cout, q.w, q.f, g.nbx, g.nby, q.sx,
+                         q.sy, q.sz, d.dActive0, m0.d_off, m0.d_flags, m0.d_D, g.bsx0, q.z, st);
+    else
+      hipLaunchKernelGGL(k_nn_prolong_blocks, dim3(g.nActive0), dim3(256), 0, s, q.cout, q.f, g.nbx, g.nby, q.sx,
+                         q.sy, d.dActive0, m0.d_off, m0.d_flags, m0.d_D, ncell, q.z, st);
+  }
+}
+
+/* Galerkin combination of K > 1 directions: zc and rs = r - A zc with the
+ * coefficients that minimise the A-norm error over their span. */
+template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s)
+{
+  Dev &d = dev(g);
+  MgLevel &m0 = g.mg[0];
+  DirSet ds{};
+  for (int a = 0; a < K; a++) {
+    ds.z[a] = c.sc[a].z;
+    ds.q[a] = c.sc[a].q;
+  }
+  hipLaunchKernelGGL(k_nn_gram<K>, dim3(kGramBlocks), dim3(256), 0, s, ds, m0.d_ch[C_R], c.n0, c.gram,
+                     &d.state->skipNet);
+  hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, kGramBlocks, d.state);
+  hipLaunchKernelGGL(k_nn_combine<K>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.zc, c.rs, m0.d_ch[C_R],
+                     (int64_t)m0.ncells, d.state);
+}
+
+/* Coarse form, learned path (skipped while skipNet): directions
+ * z_f = P_f N(R_f r) at the net's scales, zc their Galerkin combination
+ * (one scale: the step gamma = <zc, r> / <zc, A zc>); then
  * z = zc + V(r - A zc). */
 static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p, hipStream_t s)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *skip = &d.state->skipNet;
-  const int nt = c.ntx * c.nty * c.ntz;
-  const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
-  if (g.nActive0 > 0) {
-    hipLaunchKernelGGL(k_nn_fill, dim3((c.ncoarse + 255) / 256), dim3(256), 0, s, c.cin, c.ncoarse, 0.0f, skip);
-    hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, m0.d_ch[C_R], d.dActive0, m0.d_off,
-                       m0.d_flags, m0.d_D, ncell, 0, c.cin, skip);
-    hipLaunchKernelGGL(k_nn_gather_dense, dim3(nt), dim3(256), 0, s, c.cin, g.nbx, g.nby, g.nbz, c.ntx, c.nty, 0, 0,
-                       0, (const float *)nullptr, c.ib.X0, skip);
-    netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
-    hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, g.nbx, g.nby, g.nbz, c.ntx,
-                       c.nty, c.cout, skip);
-    hipLaunchKernelGGL(k_nn_prolong_blocks, dim3(g.nActive0), dim3(256), 0, s, c.cout, d.dActive0, m0.d_off,
-                       m0.d_flags, m0.d_D, ncell, c.zc, d.state);
+  coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s);
+  for (int k = 0; k < n.scales; k++)
+    multiplyLaunch(g, 0, true, c.sc[k].z, m0.d_ch[C_R], c.sc[k].q, nullptr, skip, s);
+  if (n.scales == 1) {
+    /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
+    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.n0, d.partials, skip);
+    finalize(d, kVecBlocks, FIN_ZCR, nullptr);
+    hipLaunchKernelGGL(k_nn_dot_diff, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.rs, c.n0,
+                       d.partials, skip);
+    finalize(d, kVecBlocks, FIN_GAMMA, nullptr);
+    hipLaunchKernelGGL(k_nn_gamma, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, c.rs, m0.d_ch[C_R],
+                       (int64_t)g.mg[0].ncells, d.state);
+  } else if (n.scales == 2) {
+    coarseCombine<2>(g, c, s);
+  } else {
+    coarseCombine<3>(g, c, s);
   }
-  multiplyLaunch(g, 0, true, c.zc, m0.d_ch[C_R], c.rs, nullptr, skip, s);
-  /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
-  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.n0, d.partials, skip);
-  finalize(d, kVecBlocks, FIN_ZCR, nullptr);
-  hipLaunchKernelGGL(k_nn_dot_diff, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.rs, c.n0,
-                     d.partials, skip);
-  finalize(d, kVecBlocks, FIN_GAMMA, nullptr);
-  hipLaunchKernelGGL(k_nn_gamma, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, c.rs, m0.d_ch[C_R],
-                     (int64_t)g.mg[0].ncells, d.state);
   std::swap(m0.d_ch[C_R], c.rs);   // the V-cycle below reads its right-hand side from rs
   zeroActive(g, 0, m0.d_ch[C_Z], &d.state->done, s);
   vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->done, s);
@@ -1381,10 +1758,10 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
   if (int rc = ensureHist(g, p.max_iter + 1)) return rc;
   const bool coarse = n.mode == MSBGX_NET_COARSE;
   if (coarse)
-    if (int rc = coarseSetup(g, c, d.stream)) return rc;
+    if (int rc = coarseSetup(g, c, n.scales, d.stream)) return rc;
   GraphKey key = graphKey(g, p);
   key.net = n.serial;
-  key.netMode = n.mode;
+  key.netMode = n.mode + 16 * n.prolong + 64 * n.scales;
   const int chunk = key.chunk;
   if (c.graph && c.graphKey != key) {
     (void)hipGraphExecDestroy(c.graph);
@@ -1541,24 +1918,21 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
   std::vector<int> pick(perPair);
   PcgState hs{};
   int pairs = 0;
-  /* coarse form: dense grids of the residual and error block means */
+  /* coarse form: dense grids of the residual and error means, scale by scale */
   const bool coarse = n.mode == MSBGX_NET_COARSE;
   LearnedCtx *lc = nullptr;
-  float *cerr = nullptr, *tsc = nullptr;
-  uint8_t *cmask = nullptr;
+  float *cerr[kMaxScales] = {nullptr, nullptr, nullptr}, *tsc = nullptr;
   if (coarse) {
     if (!d.learned) d.learned = new LearnedCtx();
     lc = d.learned;
     lc->n0 = n0;
-    if (int rc = coarseSetup(g, *lc, d.stream)) return rc;
+    if (int rc = coarseSetup(g, *lc, n.scales, d.stream)) return rc;
     /* tiles per iterate with any origin offset: at most one more per axis */
-    const int maxTiles = (lc->ntx + 1) * (lc->nty + 1) * (lc->ntz + 1);
-    if (int rc = scratch.alloc(&cerr, lc->ncoarse)) return rc;
+    const LearnedCtx::Scale &s0 = lc->sc[0];
+    const int maxTiles = (s0.ntx + 1) * (s0.nty + 1) * (s0.ntz + 1);
     if (int rc = scratch.alloc(&tsc, maxTiles)) return rc;
-    if (int rc = scratch.alloc(&cmask, lc->ncoarse)) return rc;
-    std::vector<uint8_t> hm(lc->ncoarse, 0);
-    for (int b : g.active0) hm[b] = 1;
-    HIPCHK(hipMemcpy(cmask, hm.data(), hm.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < n.scales; k++)
+      if (int rc = scratch.alloc(&cerr[k], lc->sc[k].ncoarse)) return rc;
   }
   for (int k = 1; k <= p.max_iter && pairs < maxPairs; k++) {
     pcgIteration(g, p);
@@ -1566,36 +1940,49 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
     HIPCHK(hipStreamSynchronize(d.stream));
     if (hs.brk || hs.done) break;   // a converged iteration leaves Z unset
     if (k % every == 0 && coarse) {
-      /* input: block means of r_k; target: block means of e_k = x* - x_k over the unknowns */
+      /* input: means of r_k; target: means of e_k = x* - x_k over the unknowns */
       hipLaunchKernelGGL(k_nn_sub, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, xs, m0.d_ch[C_X], err, n0);
-      hipLaunchKernelGGL(k_nn_fill, dim3((lc->ncoarse + 255) / 256), dim3(256), 0, d.stream, lc->cin, lc->ncoarse,
-                         0.0f, (const int *)nullptr);
-      hipLaunchKernelGGL(k_nn_fill, dim3((lc->ncoarse + 255) / 256), dim3(256), 0, d.stream, cerr, lc->ncoarse, 0.0f,
+      LearnedCtx::Scale &s0 = lc->sc[0];
+      hipLaunchKernelGGL(k_nn_fill, dim3((s0.ncoarse + 255) / 256), dim3(256), 0, d.stream, s0.cin, s0.ncoarse, 0.0f,
+                         (const int *)nullptr);
+      hipLaunchKernelGGL(k_nn_fill, dim3((s0.ncoarse + 255) / 256), dim3(256), 0, d.stream, cerr[0], s0.ncoarse, 0.0f,
                          (const int *)nullptr);
       const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
       hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, m0.d_ch[C_R], d.dActive0,
-                         m0.d_off, m0.d_flags, m0.d_D, ncell, 0, lc->cin, (const int *)nullptr);
+                         m0.d_off, m0.d_flags, m0.d_D, ncell, 0, s0.cin, (const int *)nullptr);
       hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, d.stream, err, d.dActive0, m0.d_off,
-                         m0.d_flags, m0.d_D, ncell, 1, cerr, (const int *)nullptr);
-      /* random tile origin: the domain boundary falls anywhere inside a tile */
-      const int ox = (int)(next() % 16u), oy = (int)(next() % 16u), oz = (int)(next() % 16u);
-      const int tx = (g.nbx + ox + 15) / 16, ty = (g.nby + oy + 15) / 16, tz = (g.nbz + oz + 15) / 16;
-      const int ntp = tx * ty * tz;
-      if (int rc = growSet(n, n.nS + ntp)) return rc;
-      hipLaunchKernelGGL(k_nn_tile_target, dim3(ntp), dim3(256), 0, d.stream, cerr, cmask, g.nbx, g.nby, g.nbz, tx,
-                         ty, ox, oy, oz, tsc, n.setT + (size_t)n.nS * nn::NCELL, n.setM + (size_t)n.nS * nn::NCELL);
-      hipLaunchKernelGGL(k_nn_gather_dense, dim3(ntp), dim3(256), 0, d.stream, lc->cin, g.nbx, g.nby, g.nbz, tx, ty,
-                         ox, oy, oz, tsc, n.setX + (size_t)n.nS * nn::PS, (const int *)nullptr);
-      HIPCHK(hipGetLastError());
-      std::vector<uint8_t> mk((size_t)ntp * nn::NCELL);
-      HIPCHK(hipMemcpyAsync(mk.data(), n.setM + (size_t)n.nS * nn::NCELL, mk.size(), hipMemcpyDeviceToHost, d.stream));
-      HIPCHK(hipStreamSynchronize(d.stream));
-      for (int j = 0; j < ntp; j++) {
-        int cnt = 0;
-        for (int cc = 0; cc < nn::NCELL; cc++) cnt += mk[(size_t)j * nn::NCELL + cc];
-        n.maskCount.push_back(cnt);
+                         m0.d_flags, m0.d_D, ncell, 1, cerr[0], (const int *)nullptr);
+      for (int sk = 0; sk < n.scales; sk++) {
+        LearnedCtx::Scale &q = lc->sc[sk];
+        if (sk > 0) {
+          const LearnedCtx::Scale &pq = lc->sc[sk - 1];
+          hipLaunchKernelGGL(k_nn_coarsen, dim3((q.ncoarse + 255) / 256), dim3(256), 0, d.stream, pq.cin, pq.w, pq.sx,
+                             pq.sy, pq.sz, q.cin, q.sx, q.sy, q.sz, (const int *)nullptr);
+          hipLaunchKernelGGL(k_nn_coarsen, dim3((q.ncoarse + 255) / 256), dim3(256), 0, d.stream, cerr[sk - 1], pq.w,
+                             pq.sx, pq.sy, pq.sz, cerr[sk], q.sx, q.sy, q.sz, (const int *)nullptr);
+        }
+        /* random tile origin: the domain boundary falls anywhere inside a tile */
+        const int ox = (int)(next() % 16u), oy = (int)(next() % 16u), oz = (int)(next() % 16u);
+        const int tx = (q.sx + ox + 15) / 16, ty = (q.sy + oy + 15) / 16, tz = (q.sz + oz + 15) / 16;
+        const int ntp = tx * ty * tz;
+        if (int rc = growSet(n, n.nS + ntp)) return rc;
+        hipLaunchKernelGGL(k_nn_tile_target, dim3(ntp), dim3(256), 0, d.stream, cerr[sk], q.w, q.sx, q.sy, q.sz, tx,
+                           ty, ox, oy, oz, tsc, n.setT + (size_t)n.nS * nn::NCELL,
+                           n.setM + (size_t)n.nS * nn::NCELL);
+        hipLaunchKernelGGL(k_nn_gather_dense, dim3(ntp), dim3(256), 0, d.stream, q.cin, q.sx, q.sy, q.sz, tx, ty, ox,
+                           oy, oz, tsc, n.setX + (size_t)n.nS * nn::PS, (const int *)nullptr);
+        HIPCHK(hipGetLastError());
+        std::vector<uint8_t> mk((size_t)ntp * nn::NCELL);
+        HIPCHK(hipMemcpyAsync(mk.data(), n.setM + (size_t)n.nS * nn::NCELL, mk.size(), hipMemcpyDeviceToHost,
+                              d.stream));
+        HIPCHK(hipStreamSynchronize(d.stream));
+        for (int j = 0; j < ntp; j++) {
+          int cnt = 0;
+          for (int cc = 0; cc < nn::NCELL; cc++) cnt += mk[(size_t)j * nn::NCELL + cc];
+          n.maskCount.push_back(cnt);
+        }
+        n.nS += ntp;
       }
-      n.nS += ntp;
       pairs++;
     } else if (k % every == 0) {
       /* z = V(r_k) is in Z */

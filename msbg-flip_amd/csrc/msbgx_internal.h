@@ -196,6 +196,10 @@ int netCollect(Net &n, Grid &g, const msbgx_solve_params &p, int every, int maxP
 int netNumSamples(const Net &n);
 int netSetMode(Net &n, int mode);
 int netMode(const Net &n);
+int netSetProlongation(Net &n, int kind);
+int netProlongation(const Net &n);
+int netSetScales(Net &n, int k);
+int netScales(const Net &n);
 int netSetStall(Net &n, int window, float drop);
 double netLastTrainMs(const Net &n);
 int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters, float *rel, int *fellBack);

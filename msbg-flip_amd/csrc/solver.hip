@@ -119,6 +119,7 @@ struct PcgState {
   float gamma, pad4;           // coarse form: Galerkin step <zc, r> / <zc, A zc> of the correction
   double rzOld, rzPlain;
   double rrWin[16];
+  float coef[4];               // coarse form, several scales: Galerkin coefficients of the directions
 };
 constexpr int kMaxStallWindow = 16;
 

@@ -206,6 +206,10 @@ def _load():
         "msbgx_net_num_samples": (C.c_int, [P]),
         "msbgx_net_set_mode": (C.c_int, [P, C.c_int]),
         "msbgx_net_get_mode": (C.c_int, [P]),
+        "msbgx_net_set_prolongation": (C.c_int, [P, C.c_int]),
+        "msbgx_net_get_prolongation": (C.c_int, [P]),
+        "msbgx_net_set_coarse_scales": (C.c_int, [P, C.c_int]),
+        "msbgx_net_get_coarse_scales": (C.c_int, [P]),
         "msbgx_net_set_stall": (C.c_int, [P, C.c_int, C.c_float]),
         "msbgx_net_train": (C.c_int, [P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
@@ -256,7 +260,8 @@ EXPORTED = [
     "msbgx_net_set_weights", "msbgx_net_set_samples", "msbgx_net_loss_grad", "msbgx_net_collect",
     "msbgx_net_num_samples", "msbgx_net_train", "msbgx_net_last_train_ms", "msbgx_net_apply",
     "msbgx_solve_pressure_learned", "msbgx_net_time_conv", "msbgx_net_set_mode", "msbgx_net_get_mode",
-    "msbgx_net_set_stall", "msbgx_flip_set_velocity", "msbgx_flip_set_velocity_scene", "msbgx_flip_get_velocity",
+    "msbgx_net_set_stall", "msbgx_net_set_prolongation", "msbgx_net_get_prolongation",
+    "msbgx_net_set_coarse_scales", "msbgx_net_get_coarse_scales", "msbgx_flip_set_velocity", "msbgx_flip_set_velocity_scene", "msbgx_flip_get_velocity",
     "msbgx_flip_rhs_from_velocity", "msbgx_flip_update_velocity", "msbgx_default_p2g_params", "msbgx_p2g_create",
     "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
     "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
@@ -268,6 +273,7 @@ OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
 FLIP_SIGN_REFERENCE, FLIP_SIGN_CORRECTED = 0, 1
 NET_PARAMS = 14688
 NET_FINE, NET_COARSE = 0, 1
+PROLONG_CONSTANT, PROLONG_LINEAR = 0, 1
 
 
 class MsbgxError(RuntimeError):
@@ -525,6 +531,24 @@ class LearnedNet:
     @mode.setter
     def mode(self, m):
         _ck(lib.msbgx_net_set_mode(self._h, m), "net_set_mode")
+
+    @property
+    def prolongation(self):
+        """Coarse form: PROLONG_CONSTANT or PROLONG_LINEAR extension of block values."""
+        return lib.msbgx_net_get_prolongation(self._h)
+
+    @prolongation.setter
+    def prolongation(self, k):
+        _ck(lib.msbgx_net_set_prolongation(self._h, k), "net_set_prolongation")
+
+    @property
+    def scales(self):
+        """Coarse form: number of scales (block grid coarsened 1x, 2x, 4x)."""
+        return lib.msbgx_net_get_coarse_scales(self._h)
+
+    @scales.setter
+    def scales(self, k):
+        _ck(lib.msbgx_net_set_coarse_scales(self._h, k), "net_set_coarse_scales")
 
     def destroy(self):
         if self._h:
