@@ -1484,6 +1484,231 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
   }
 }
 
+/* Halo value of face `face` (kind, neighbour offset off2) at face cell
+ * (u, w): the value stageFaces writes at faces[face][u + B w], by the same
+ * float operations. */
+template <int B>
+__device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int64_t off, const float *__restrict__ X,
+                                               int face, int kind, int64_t off2, int u, int w)
+{
+  const float a13 = (float)(1.0 * (1. / 3.));
+  const float a23 = (float)(1.0 * 2 * (1. / 3.));
+  const float *Wd[3] = {a.W0, a.W1, a.W2};
+  const int dir = face >> 1, side = face & 1;
+  if (kind == FK_DOMAIN) return 0.0f;
+  if (kind == FK_SAME) {
+    const int e = side ? 0 : B - 1;
+    const int src = dir == 0 ? e + B * u + B * B * w : (dir == 1 ? u + B * e + B * B * w : u + B * w + B * B * e);
+    return X[off2 + src] * h;
+  }
+  if (kind == FK_COARSER) {
+    constexpr int Bc = B / 2 > 0 ? B / 2 : 1;
+    const int e = side ? 0 : Bc - 1;
+    const int cu = u / 2, cw = w / 2;
+    const int src = dir == 0 ? e + Bc * cu + Bc * Bc * cw
+                             : (dir == 1 ? cu + Bc * e + Bc * Bc * cw : cu + Bc * cw + Bc * Bc * e);
+    float aa = a23;
+    if (!side) {
+      const int own = dir == 0 ? B * u + B * B * w : (dir == 1 ? u + B * B * w : u + B * w);
+      aa *= Wd[dir][off + own];
+    }
+    const double sv = (double)(aa * X[off2 + src]);
+    return (float)(sv * (double)h);
+  }
+  constexpr int Bf = 2 * B;
+  const int e = side ? 0 : Bf - 1;
+  double sv = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int d1 = c & 1, d2 = c >> 1;
+    const int fu = 2 * u + d1, fw = 2 * w + d2;
+    const int src = dir == 0 ? e + Bf * fu + Bf * Bf * fw
+                             : (dir == 1 ? fu + Bf * e + Bf * Bf * fw : fu + Bf * fw + Bf * Bf * e);
+    float aa = a13;
+    if (side) aa *= Wd[dir][off2 + src];
+    sv += (double)(aa * X[off2 + src]);
+  }
+  return (float)(sv * (double)h);
+}
+
+/* y = A x (or r = b - A x) for B = 32: one workgroup of B*B threads per
+ * block marches through the block's z-planes.  Thread (x, y) owns its
+ * z-column; each step publishes the plane's h*x (0 on all-non-fluid
+ * segments) with its x/y halo ring into a double-buffered LDS plane (one
+ * LDS-only barrier per step), takes the z neighbours from registers, and
+ * issues the next step's loads (x and flags two planes ahead, coefficients
+ * and b one plane ahead, the ring's face values one plane ahead) before it
+ * computes, so that they are in flight across the barrier.  Cells, face
+ * values, operation order and the FP64 partial are those of k_multiply. */
+template <int B>
+__global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__restrict__ blocks,
+                                                      const float *__restrict__ X, const float *__restrict__ Bv,
+                                                      float *__restrict__ Y, int residual, double *partials,
+                                                      int partialBase, const int *done)
+{
+  constexpr int S = Cfg<B>::S, P = B + 2, P2 = P * P, BB = B * B;
+  __shared__ float pl[2 * P2];
+  __shared__ double red[16];
+  const int li = xcdListIndex();
+  const int bid = blocks[li];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const int isDone = *done;
+  asm volatile("" ::"s"(lev), "s"(isDone));
+  if (isDone) return;
+  const float h = (float)(1 << lev);
+  const int x = threadIdx.x % B, y = threadIdx.x / B;
+  const NbDesc &nd = a.nbd[bid];
+  int fkind[6];
+  int64_t foff[6];
+#pragma unroll
+  for (int face = 0; face < 6; face++) {
+    const int l2 = nd.lev[face];
+    fkind[face] = l2 < 0 ? FK_DOMAIN : (l2 == lev ? FK_SAME : (l2 > lev ? FK_COARSER : FK_FINER));
+    foff[face] = l2 < 0 ? 0 : nd.off[face];
+  }
+  const float *bsrc = residual ? Bv : X;   // X stands in when b is not read
+  const int64_t q0 = off + x + B * y;
+  /* ring role: threads 0 .. 4B-1 load the x/y halo of each plane, face
+   * rf = t / B at face cell (ri, z) */
+  const int t = threadIdx.x;
+  const bool ring = t < 4 * B;
+  const int rf = ring ? t / B : 0, ri = t % B;
+  const int rdir = rf >> 1, rside = rf & 1;
+  const int rl2 = nd.lev[rf];
+  const int rkind = !ring ? FK_DOMAIN : (rl2 < 0 ? FK_DOMAIN : (rl2 == lev ? FK_SAME : (rl2 > lev ? FK_COARSER : FK_FINER)));
+  const int64_t roff = rl2 < 0 ? 0 : nd.off[rf];
+  const int rpos = rf == 0 ? (ri + 1) * P : (rf == 1 ? (ri + 1) * P + B + 1 : (rf == 2 ? ri + 1 : (B + 1) * P + ri + 1));
+  const float *rW = rdir == 0 ? a.W0 : a.W1;
+  /* raw loads of the ring value at plane z: same level x; coarser x and own
+   * -face coefficient */
+  auto ringLoad = [&](int z, float &r1, float &r2) {
+    r1 = 0.0f;
+    r2 = 1.0f;
+    if (rkind == FK_SAME) {
+      const int e = rside ? 0 : B - 1;
+      const int src = rdir == 0 ? e + B * ri + BB * z : ri + B * e + BB * z;
+      r1 = X[roff + src];
+    } else if (rkind == FK_COARSER) {
+      constexpr int Bc = B / 2;
+      const int e = rside ? 0 : Bc - 1;
+      const int cu = ri / 2, cw = z / 2;
+      const int src = rdir == 0 ? e + Bc * cu + Bc * Bc * cw : cu + Bc * e + Bc * Bc * cw;
+      r1 = X[roff + src];
+      if (!rside) r2 = rW[off + (rdir == 0 ? B * ri + BB * z : ri + BB * z)];
+    }
+  };
+  auto ringValue = [&](int z, float r1, float r2) -> float {
+    if (rkind == FK_SAME) return r1 * h;
+    if (rkind == FK_COARSER) {
+      float aa = (float)(1.0 * 2 * (1. / 3.));
+      if (!rside) aa *= r2;
+      const double sv = (double)(aa * r1);
+      return (float)(sv * (double)h);
+    }
+    if (rkind == FK_FINER) return faceCellValue<B>(a, h, off, X, rf, rkind, roff, ri, z);
+    return 0.0f;
+  };
+  const float fz0 = faceCellValue<B>(a, h, off, X, 4, fkind[4], foff[4], x, y);
+  const float fz1 = faceCellValue<B>(a, h, off, X, 5, fkind[5], foff[5], x, y);
+  const float w6t = a.W2[fkind[5] == FK_SAME ? foff[5] + x + B * y : q0];
+  const bool m0 = x == 0 && fkind[0] >= FK_COARSER, m2 = y == 0 && fkind[2] >= FK_COARSER;
+  const bool m4 = fkind[4] >= FK_COARSER;
+  const float xOut = fkind[1] == FK_DOMAIN ? a.bcW[0] : 1.0f, yOut = fkind[3] == FK_DOMAIN ? a.bcW[1] : 1.0f;
+  const float zOut = fkind[5] == FK_SAME ? w6t : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
+  const bool xIn = x < B - 1 || fkind[1] == FK_SAME, yIn = y < B - 1 || fkind[3] == FK_SAME;
+  auto qx = [&](int z) { return x < B - 1 ? q0 + BB * z + 1 : (fkind[1] == FK_SAME ? foff[1] + B * y + BB * z : q0 + BB * z); };
+  auto qy = [&](int z) { return y < B - 1 ? q0 + BB * z + B : (fkind[3] == FK_SAME ? foff[3] + x + BB * z : q0 + BB * z); };
+
+  /* prologue: plane 0's coefficients, x and flags of planes 0 and 1 */
+  float xr0 = X[q0], xr1 = X[q0 + BB];
+  uint16_t fl0 = a.flags[q0], fl1 = a.flags[q0 + BB];
+  float w0 = a.W0[q0], w1 = a.W1[q0], w2 = a.W2[q0], w2n = a.W2[q0 + BB], dd = a.D[q0];
+  float w0x = a.W0[qx(0)], w1y = a.W1[qy(0)], bb = bsrc[q0];
+  float rr1, rr2;
+  ringLoad(0, rr1, rr2);
+  bool sk = segmentAllNonFluid<S>((fl0 & (CELL_SOLID | CELL_VOID)) != 0);
+  float xs = sk ? 0.0f : xr0 * h;   // h*x of plane z as the stencil reads it
+  float xsPrev = 0.0f;
+  double acc = 0.0;
+#pragma unroll
+  for (int z = 0; z < B; z++) {
+    /* next step's loads */
+    float xr2 = 0.0f, nw0 = 0.0f, nw1 = 0.0f, nw2n = 0.0f, ndd = 0.0f, nw0x = 0.0f, nw1y = 0.0f, nbb = 0.0f;
+    float nr1 = 0.0f, nr2 = 1.0f;
+    uint16_t fl2 = 0;
+    if (z + 2 < B) {
+      xr2 = X[q0 + BB * (z + 2)];
+      fl2 = a.flags[q0 + BB * (z + 2)];
+      nw2n = a.W2[q0 + BB * (z + 2)];
+    }
+    if (z + 1 < B) {
+      const int64_t q = q0 + BB * (z + 1);
+      nw0 = a.W0[q];
+      nw1 = a.W1[q];
+      ndd = a.D[q];
+      nw0x = a.W0[qx(z + 1)];
+      nw1y = a.W1[qy(z + 1)];
+      nbb = bsrc[q];
+      ringLoad(z + 1, nr1, nr2);
+    }
+    /* h*x of plane z+1 (the +z neighbour) and its skip state */
+    bool skn = false;
+    float xsn = 0.0f;
+    if (z + 1 < B) {
+      skn = segmentAllNonFluid<S>((fl1 & (CELL_SOLID | CELL_VOID)) != 0);
+      xsn = skn ? 0.0f : xr1 * h;
+    }
+    float *p = pl + (z & 1) * P2;
+    p[(y + 1) * P + x + 1] = xs;
+    if (ring) p[rpos] = ringValue(z, rr1, rr2);
+    ldsBarrier();
+    if (!sk) {
+      const float *c = p + (y + 1) * P + x + 1;
+      const float F0 = xs;
+      const float F1 = c[-1], F2 = c[1], F3 = c[-P], F4 = c[P];
+      const float F5 = z > 0 ? xsPrev : fz0;
+      const float F6 = z < B - 1 ? xsn : fz1;
+      const float W1 = m0 ? 1.0f : w0;
+      const float W3 = m2 ? 1.0f : w1;
+      const float W5 = (z == 0 && m4) ? 1.0f : w2;
+      const float W2 = xIn ? w0x : xOut;
+      const float W4 = yIn ? w1y : yOut;
+      const float W6 = z < B - 1 ? w2n : zOut;
+      float s = F1 * W1;
+      s = fmaf(F2, W2, s);
+      s = fmaf(F3, W3, s);
+      s = fmaf(F4, W4, s);
+      s = fmaf(F5, W5, s);
+      s = fmaf(F6, W6, s);
+      float r = dd != 0.0f ? fmaf(1.0f / dd, F0, -s) : 0.0f;
+      if (residual) r = bb - r;
+      Y[q0 + BB * z] = r;
+      acc += (double)F0 * (double)r;
+    }
+    /* rotate */
+    xsPrev = xs;
+    xs = xsn;
+    sk = skn;
+    xr1 = xr2;
+    fl1 = fl2;
+    w0 = nw0;
+    w1 = nw1;
+    w2 = w2n;
+    w2n = nw2n;
+    dd = ndd;
+    w0x = nw0x;
+    w1y = nw1y;
+    bb = nbb;
+    rr1 = nr1;
+    rr2 = nr2;
+  }
+  if (partials) {
+    const double sres = blockReduceSum(lev == 0 ? acc : 0.0, red);
+    if (threadIdx.x == 0) partials[partialBase + li] = sres;   // list order: the fixed summation order
+  }
+}
+
 /* Restriction into the coarse right-hand side of MG level Lc from the fine
  * residual of level Lc-1: 2x2x2 mean for coarsened blocks; for blocks with
  * the same level at both MG levels, a copy (first V-cycle on the channel
@@ -2233,7 +2458,11 @@ static int multiplyLaunch(const Grid &g, int L, bool residual, const float *X, c
       case 4: launchMultiplyB<4>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
       case 8: launchMultiplyB<8>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
       case 16: launchMultiplyB<16>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
-      case 32: launchMultiplyB<32>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
+      case 32:
+        if (l.n)
+          hipLaunchKernelGGL(k_multiply_z<32>, dim3(l.n), dim3(32 * 32), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y,
+                             residual, partials, base, done);
+        break;
     }
     base += l.n;
   }

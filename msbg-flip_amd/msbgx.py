@@ -574,15 +574,21 @@ class LearnedNet:
         _ck(lib.msbgx_net_set_weights(self._h, _ptr(w)), "net_set_weights")
 
     def save(self, path):
-        """Write the weights (the network's only persistent state) and its form
-        as an .npz file (plain arrays, no pickled objects)."""
-        np.savez(path, weights=self.weights, mode=np.int32(self.mode))
+        """Write the weights (the network's only persistent state), its form,
+        prolongation and scales as an .npz file (plain arrays, no pickled
+        objects)."""
+        np.savez(path, weights=self.weights, mode=np.int32(self.mode), prolongation=np.int32(self.prolongation),
+                 scales=np.int32(self.scales))
 
     @classmethod
     def load(cls, path):
         with np.load(path, allow_pickle=False) as f:
             net = cls(seed=1, mode=int(f["mode"]))
             net.weights = f["weights"]
+            if "prolongation" in f:
+                net.prolongation = int(f["prolongation"])
+            if "scales" in f:
+                net.scales = int(f["scales"])
         return net
 
     def set_samples(self, x, target, mask):
