@@ -1,13 +1,19 @@
 """Benchmark: time to 1e-6 relative residual of the MSBG FLIP pressure solve
-(MG-PCG) on one MI355X, plus the smoother's HBM roofline.
+on one MI355X, learned-preconditioned vs plain MG-PCG, plus the smoother's HBM
+roofline.
 
-Workload (default, --config c1): 256^3 effective domain, 16^3 blocks, one
-active level (3 MG levels); liquid = sphere of radius 0.3 at (0.5,0.5,0.4);
-air cells Dirichlet, domain walls Neumann; RHS = divergence of a seeded
-uniform [-1,1] MAC velocity; FP32 storage, FP64 dots.  A step = one solve
-from p = 0 to |r| <= 1e-6 |b| with the inputs resident in HBM.  --config c2
-selects the 512^3 multires workload (32^3 base blocks, 3 levels), whose plain
-solve does not reach 1e-6 under the reference semantics.
+Workload (default, --config c5): the full-scale particle workload -- 1024^3
+effective domain, 32^3 base blocks, 4 levels assigned by block distance to
+the liquid (fine within one block); liquid = dam-break box x<0.4, y<0.6 plus
+a sphere of radius 0.15 at (0.7,0.8,0.5); RHS = divergence of 8 seeded
+particles per liquid cell (uniform positions, N(0,0.5) velocities) splatted
+to the faces; FP16-stored smoother operands, FP32 Krylov operator, FP64 dots.
+The learned coarse correction is trained in the run on pairs from plain
+MG-PCG solves of seeded dam-break fields at 256^3 and validated on a held-out
+seed.  `value` is the learned solve's time from p = 0 to |r| <= 1e-6 |b| with
+the inputs resident in HBM; a step of the timed region (--steps) is one
+learned MG-PCG iteration on that system.  --config c1/c2/c4 time the plain
+MG-PCG on the smaller workloads instead.
 
 Prints one JSON line (see the repository's benchmark contract).
 """
@@ -27,33 +33,34 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c4", "c5", "c5b16"],
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c5", choices=["c1", "c2", "c4", "c5", "c5b16"],
                     help="c5b16: the full-scale particle workload on 16^3 base blocks")
     ap.add_argument("--n", type=int, default=0, help="override the domain size")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=2000)
-    ap.add_argument("--cpu-iters", type=int, default=0,
-                    help="PCG iterations in the CPU-baseline sample (0: the whole solve to --tol)")
+    ap.add_argument("--cpu-iters", type=int, default=-1,
+                    help="PCG iterations in the CPU-baseline sample (-1: 0 for c5 -- the initial V-cycle "
+                         "only --, else the whole solve to --tol)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-precision", action="store_true", help="skip the FP16/BF16 precision-form runs")
+    ap.add_argument("--precision", action="store_true", help="also run the FP32/FP16/BF16 precision-form runs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FLIP caller, particle-to-grid, compact, PDE and batched sections")
     ap.add_argument("--no-fp64-oracle", action="store_true", help="skip the CPU FP64 reference solve")
-    ap.add_argument("--secondary", action="store_true", help="also time the other workloads")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form (64 problems) runs")
-    ap.add_argument("--no-flip", action="store_true",
-                    help="skip the FLIP caller steps (velocity update, particle-to-grid transfer)")
     ap.add_argument("--p2g-n", type=int, default=1024, help="domain size of the particle-to-grid run")
     ap.add_argument("--batch-max-iter", type=int, default=400)
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner A/B")
     ap.add_argument("--learned-pairs", type=int, default=2000)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--learned-eval-n", type=int, default=512)
     ap.add_argument("--learned-eval-config", default="c4", choices=["c4", "c5", "c5b16"],
                     help="evaluation workload of the learned forms (c4: dam break at --learned-eval-n, seeds 501/502; "
                          "c5/c5b16: the 1024^3 particle workload on 32^3 / 16^3 blocks)")
-    ap.add_argument("--learned-forms", default="fine,coarse")
+    ap.add_argument("--learned-forms", default="", help="comma list of fine,coarse: also run the learned forms on "
+                    "the dam-break evaluation workload (--learned-eval-config)")
+    ap.add_argument("--scales", type=int, default=3, help="coarse form: scales of the network")
+    ap.add_argument("--prolongation", type=int, default=1, help="coarse form: 0 constant, 1 trilinear")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch PCG iterations eagerly (no hipGraph) in every section (profiling runs)")
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -112,10 +119,13 @@ def gpu_run(msbgx, cfg, args, log):
 def pmc_calibration(msbgx, cfg, log):
     """Workload for the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: one copy
     of the MG0 fluid blocks (k_copy_blocks: reads and writes 4 B per cell), ten
-    smoother colour passes over the largest same-size block list and ten
-    residual launches (k_multiply) over the level-0 fluid blocks."""
+    smoother colour passes (both colours in turn) of the block size with the
+    most relaxation cells and ten residual launches (k_multiply) over the
+    level-0 fluid blocks; the particle workload with FP16 smoother operands."""
     g = msbgx.build_grid(cfg)
     g.prepare_pressure_solve_flip(scene=cfg.scene)
+    if cfg.scene.particles_per_cell > 0 and cfg.bsx == 32:
+        g.set_precision(msbgx.PREC_FP16)
     g.copy_channel(msbgx.CH_DIVERGENCE, msbgx.CH_R)
     ms, by, cells = g.time_kernel(0, reps=10)
     ms_res, by_res, _ = g.time_kernel(1, reps=10)
@@ -220,14 +230,22 @@ def batched_runs(msbgx, args, log):
     return rec
 
 
-def learned_runs(msbgx, args, log, mode="fine"):
-    """Learned-preconditioner form (dam-break workload): training pairs from
-    plain MG-PCG solves of seeded dam-break fields at 256^3 (box x<0.4, y<0.6
-    plus a sphere r~U[0.1,0.2]), every iterate sampled (`blocks_per_pair`
-    random level-0 blocks each) until --learned-pairs iterates; Adam on the
-    GPU; then plain vs learned MG-PCG to 1e-6 on unseen seeds at 512^3."""
+def learned_train(msbgx, args, log, mode="coarse"):
+    """Train a learned correction as the learned-preconditioner workload
+    prescribes: pairs from plain MG-PCG solves of seeded dam-break fields at
+    256^3 (box x<0.4, y<0.6 plus a sphere r~U[0.1,0.2], 16^3 blocks, 3
+    levels), every iterate sampled until --learned-pairs iterates (fine form:
+    `blocks_per_pair` random level-0 blocks each; coarse form: 16^3 tiles of
+    the block-mean grids at each of the net's scales, random tile origins);
+    Adam on the GPU.  Coarse form: after each schedule stage the net is
+    validated on a held-out seed of the particle workload at 512^3 (seed 900:
+    neither a training seed nor the evaluation seed; FP16 operands) and the
+    weights with the fewest iterations are kept.  Returns (net, record)."""
     t0 = time.time()
     net = msbgx.LearnedNet(seed=2024, mode=msbgx.NET_COARSE if mode == "coarse" else msbgx.NET_FINE)
+    if mode == "coarse":
+        net.scales = args.scales
+        net.prolongation = args.prolongation
     pairs, seed = 0, 0
     while pairs < args.learned_pairs:
         seed += 1
@@ -235,7 +253,7 @@ def learned_runs(msbgx, args, log, mode="fine"):
         g = msbgx.build_grid(c)
         g.prepare_pressure_solve_flip(scene=c.scene)
         pairs += net.collect(g, every=1, max_pairs=args.learned_pairs - pairs, blocks_per_pair=16, seed=seed,
-                             tol=1e-7, max_iter=2000)   # coarse form: one sample per 16^3-block tile
+                             tol=1e-7, max_iter=2000)
         g.destroy()
     t_collect = time.time() - t0
     log(f"learned ({mode}): {pairs} pairs from {seed} solves, {net.num_samples} samples, {t_collect:.1f} s")
@@ -246,14 +264,12 @@ def learned_runs(msbgx, args, log, mode="fine"):
         sched = [(2000, 3e-3, 16), (2000, 1e-3, 16), (4000, 1e-3, 32), (4000, 3e-4, 32), (4000, 3e-4, 64),
                  (4000, 1e-4, 64)]
     train_ms, first, last = 0.0, None, None
-    # coarse form: after each schedule stage, validate on a held-out dam-break
-    # seed at 384^3 (not a training seed, not an evaluation seed) and keep the
-    # weights with the fewest iterations
     gv = None
     if mode == "coarse":
-        vc = msbgx.config_c4(384, seed=900)
+        vc = msbgx.config_c5(512, seed=900)
         gv = msbgx.build_grid(vc)
         gv.prepare_pressure_solve_flip(scene=vc.scene)
+        gv.set_precision(msbgx.PREC_FP16)
     selection, cum = [], 0
     for k, (steps, lr, bs) in enumerate(sched):
         a, b = net.train(steps=steps, batch=bs, lr=lr, seed=7 + k)
@@ -265,7 +281,8 @@ def learned_runs(msbgx, args, log, mode="fine"):
             gv.zero_channel(msbgx.CH_PRESSURE, 0)
             it_v, rr_v, fb_v = net.solve(gv, max_iter=args.max_iter, tol=args.tol)
             selection.append({"steps": cum, "val_iterations": it_v, "val_fallback": fb_v, "weights": net.weights})
-            log(f"learned ({mode}): after {cum} steps, validation (dam break 384^3, seed 900): {it_v} iterations")
+            log(f"learned ({mode}): after {cum} steps, validation (particle workload 512^3, seed 900): "
+                f"{it_v} iterations, fallback {fb_v}")
     chosen = None
     if selection:
         best = min(range(len(selection)), key=lambda i: (selection[i]["val_iterations"], -selection[i]["steps"]))
@@ -275,6 +292,23 @@ def learned_runs(msbgx, args, log, mode="fine"):
         log(f"learned ({mode}): keeping the weights after {chosen} steps")
     steps_total = sum(st for st, _, _ in sched)
     log(f"learned ({mode}): {steps_total} Adam steps, loss {first:.4f} -> {last:.4f}, {train_ms:.0f} ms")
+    rec = {"form": mode, "workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
+           "samples": net_samples, "collect_s": round(t_collect, 1), "steps": steps_total,
+           "schedule": [{"steps": st, "lr": lr, "batch": bs} for st, lr, bs in sched],
+           "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1),
+           "validation": [{k: v for k, v in e.items() if k != "weights"} for e in selection],
+           "kept_weights_after_steps": chosen, "wall_s": round(time.time() - t0, 1)}
+    if mode == "coarse":
+        rec["scales"] = net.scales
+        rec["prolongation"] = "trilinear" if net.prolongation == msbgx.PROLONG_LINEAR else "constant"
+    return net, rec
+
+
+def learned_runs(msbgx, args, log, mode="fine"):
+    """Learned form on the dam-break evaluation workload: the net of
+    learned_train, then plain vs learned MG-PCG to 1e-6 on unseen seeds at
+    --learned-eval-n (or the particle workload)."""
+    net, train = learned_train(msbgx, args, log, mode)
     evals = []
     if mode == "fine" and args.learned_eval_config == "c5":
         log("learned (fine): the 32^3-block workload needs the coarse form; skipped")
@@ -314,14 +348,8 @@ def learned_runs(msbgx, args, log, mode="fine"):
     log(f"learned: conv16 roofline {conv}")
     w = net.weights
     net.destroy()
-    return {"form": mode,
-            "train": {"workload": "dam break 256^3, 16^3 blocks, 3 levels", "pairs": pairs, "solves": seed,
-                      "samples": net_samples, "collect_s": round(t_collect, 1), "steps": steps_total,
-                      "schedule": [{"steps": st, "lr": lr, "batch": bs} for st, lr, bs in sched],
-                      "loss_first": first, "loss_last": last, "train_gpu_ms": round(train_ms, 1),
-                      "validation": [{k: v for k, v in e.items() if k != "weights"} for e in selection],
-                      "kept_weights_after_steps": chosen},
-            "eval": evals, "conv16_roofline": conv, "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
+    return {"form": mode, "train": train, "eval": evals, "conv16_roofline": conv,
+            "weights_l2": float((w.astype(float) ** 2).sum() ** 0.5)}
 
 
 def read_traffic(kernel, workload):
@@ -521,7 +549,14 @@ def p2g_runs(msbgx, args, log):
 
 def cpu_baseline(cfg, args, gpu_iters, log):
     """CPU oracle (C++ restatement of the reference solver) timed on the host
-    cores on a bounded sample: `cpu_iters` PCG iterations of the same solve."""
+    cores on a bounded sample of the same solve: `cpu_iters` PCG iterations
+    (0: the initial V-cycle alone, which is one iteration's work), scaled to
+    the GPU's plain iteration count.  For the particle workload the oracle is
+    prepared from the same scene with the velocity-divergence right-hand side
+    instead of the particle splat: topology, level map, cell flags and face
+    weights are those of the workload, and the work per iteration does not
+    depend on the right-hand side's values, while the oracle's serial
+    particle transfer would take tens of minutes at 1024^3."""
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     outdir = tempfile.mkdtemp(prefix="msbg_oracle_")
@@ -536,25 +571,36 @@ def cpu_baseline(cfg, args, gpu_iters, log):
         log(f"native oracle build failed ({e}); using oracle/libmsbg_oracle.so")
     import oracle_lib
     threads = min(args.cpu_threads, os.cpu_count() or 1)
+    particles = cfg.scene.particles_per_cell > 0
+    scene = type(cfg.scene).from_buffer_copy(cfg.scene)
+    if particles:
+        scene.particles_per_cell = 0
+    big = cfg.n >= 512
     o = oracle_lib.build_oracle(cfg, threads=threads)
+    if args.config == "c5":
+        o.set_smoother_precision(1)   # FP16 operands, as the GPU solve
     t0 = time.time()
-    o.prepare_scene(cfg.scene)
+    o.prepare_scene(scene)
     t_prep = time.time() - t0
-    k = args.cpu_iters if args.cpu_iters > 0 else max(1, gpu_iters)
+    k = args.cpu_iters if args.cpu_iters >= 0 else (0 if big else max(1, gpu_iters))
     t0 = time.time()
     it, rr, _ = o.solve(max_iter=k, tol=args.tol)
     t_k = time.time() - t0
+    del o
     # the solve runs one preconditioner application before the loop: count it as one iteration
     per_iter = t_k / (k + 1)
-    est = t_k if it == gpu_iters else per_iter * (gpu_iters + 1)
-    log(f"cpu oracle: prepare {t_prep:.1f} s, {it} iterations (rel {rr:.2e}) {t_k:.1f} s on {threads} threads, "
+    whole = it == gpu_iters and k > 0
+    est = t_k if whole else per_iter * (gpu_iters + 1)
+    log(f"cpu oracle: prepare {t_prep:.1f} s, {it} iterations (+1 initial V-cycle) {t_k:.1f} s on {threads} threads, "
         f"{per_iter*1e3:.1f} ms/iter; for the GPU's {gpu_iters} iterations: {est:.1f} s")
-    o1 = oracle_lib.build_oracle(cfg, threads=1)
-    o1.prepare_scene(cfg.scene)
-    t0 = time.time()
-    o1.solve(max_iter=1, tol=args.tol)
-    t1 = (time.time() - t0) / 2
-    log(f"cpu oracle, 1 thread: {t1*1e3:.0f} ms per iteration")
+    t1 = None
+    if not big:
+        o1 = oracle_lib.build_oracle(cfg, threads=1)
+        o1.prepare_scene(scene)
+        t0 = time.time()
+        o1.solve(max_iter=1, tol=args.tol)
+        t1 = (time.time() - t0) / 2
+        log(f"cpu oracle, 1 thread: {t1*1e3:.0f} ms per iteration")
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -564,32 +610,120 @@ def cpu_baseline(cfg, args, gpu_iters, log):
                     break
     except OSError:
         pass
-    whole = it == gpu_iters
+    rhs = ("; right-hand side: velocity divergence in place of the particle splat (same topology, levels, flags "
+           "and weights)" if particles else "")
     return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
-            "one_thread_ms_per_iteration": round(t1 * 1e3, 1),
+            "one_thread_ms_per_iteration": round(t1 * 1e3, 1) if t1 is not None else None,
             "sample": (f"oracle/msbg_oracle.cpp (C++ restatement of the reference solver, -O3 -march=native, OpenMP "
-                       f"over the blocks of a colour) on {threads} threads of {model}: "
+                       f"over the blocks of a colour) on {threads} threads of {model}, {cfg.name} {cfg.n}^3"
+                       + (", FP16 smoother operands" if args.config == "c5" else "") + ": "
                        + (f"the whole solve to {args.tol:g} ({it} PCG iterations, the GPU's count) timed"
                           if whole else
-                          f"{it} PCG iterations (+1 initial V-cycle) timed = {t_k:.1f} s, extrapolated linearly "
-                          f"to the GPU's {gpu_iters} iterations")
-                       + f"; setup {t_prep:.1f} s excluded; 1 thread: {t1*1e3:.0f} ms per iteration")}
+                          f"{it} PCG iterations + the initial V-cycle timed = {t_k:.1f} s, i.e. {per_iter:.2f} s per "
+                          f"iteration, scaled to the GPU plain solve's {gpu_iters} iterations")
+                       + f"; setup {t_prep:.1f} s excluded" + rhs)}
 
 
-def main():
-    args = parse()
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def c5_headline(msbgx, args, log):
+    """The headline: plain vs learned MG-PCG to --tol on the particle workload
+    (FP16 smoother operands), the timed region of --steps learned iterations,
+    and the smoother's roofline on the same grid."""
+    import numpy as np
+    cfg = make_cfg(msbgx, "c5", args.n)
+    t0 = time.time()
+    g = msbgx.build_grid(cfg)
     if args.no_graph:
-        os.environ["MSBGX_NO_GRAPH"] = "1"   # every handle created below launches eagerly
-    import msbgx
-    cfg = make_cfg(msbgx, args.config, args.n)
-    if args.pmc_calibration:
-        pmc_calibration(msbgx, cfg, log)
-        return
+        g.set_options(msbgx.OPT_NO_GRAPH)
+    g.prepare_pressure_solve_flip(scene=cfg.scene)
+    g.set_precision(msbgx.PREC_FP16)
+    lv, fl = g.block_info(0)
+    nb0 = len(g.level0_blocks())
+    log(f"prepare {cfg.name} n={cfg.n}: {time.time() - t0:.1f} s, {nb0} PCG blocks "
+        f"({nb0 * cfg.bsx ** 3} cells), {len(g.relax_list(0))} relaxation blocks at MG0")
+    net, train = learned_train(msbgx, args, log, "coarse")
+    # plain MG-PCG: graph capture on a short solve, then the whole solve
+    g.solve_pressure_flip(max_iter=3, tol=0.0)
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    it_p, rr_p = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
+    ms_p = g.last_solve_ms()
+    log(f"plain MG-PCG: {it_p} iterations, rel {rr_p:.3e}, {ms_p:.1f} ms ({ms_p / max(it_p, 1):.2f} ms/iteration)")
+    # timed region: --warmup untimed, then exactly --steps learned MG-PCG iterations
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    net.solve(g, max_iter=max(args.warmup, 1), tol=0.0)
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    w0 = time.perf_counter()
+    k_it, _, _ = net.solve(g, max_iter=args.steps, tol=0.0)   # returns after the device is idle
+    wall_ms = (time.perf_counter() - w0) * 1e3
+    steps_ms = g.last_solve_ms()
+    log(f"timed: {k_it} learned iterations in {steps_ms:.1f} ms (HIP events), {wall_ms:.1f} ms wall")
+    # the learned solve to the tolerance
+    g.zero_channel(msbgx.CH_PRESSURE, 0)
+    it_l, rr_l, fb = net.solve(g, max_iter=args.max_iter, tol=args.tol)
+    ms_l = g.last_solve_ms()
+    log(f"learned MG-PCG: {it_l} iterations, rel {rr_l:.3e}, fallback {fb}, {ms_l:.1f} ms; "
+        f"speedup {ms_p / ms_l:.2f}x")
+    ms_relax, bytes_relax, cells_relax = g.time_kernel(0, reps=20)
+    ms_res, bytes_res, _ = g.time_kernel(1, reps=20)
+    ms_upd, bytes_upd, _ = g.time_kernel(2, reps=20)
+    achieved = bytes_relax / (ms_relax * 1e-3) / 1e9
+    log(f"smoother: {ms_relax:.4f} ms per colour pass, {cells_relax} cells, {achieved:.0f} GB/s")
+    ws = net.weights
+    net.destroy()
+    rec = {
+        "metric": "time to 1e-6 rel. residual (learned vs plain MG-PCG, 1 GPU); smoother HBM GB/s",
+        "value": round(ms_l, 1),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(steps_ms / max(k_it, 1), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP16-stored smoother operands, FP64 dots)",
+        "data": "synthetic (seeded scene; particle-to-grid velocity divergence, 8 particles per liquid cell)",
+        "config": {
+            "workload": f"{cfg.name} {cfg.n}^3, {cfg.bsx}^3 base blocks, {cfg.levels} levels, FP16 smoother "
+                        f"operands; learned coarse correction ({train['scales']} scales, {train['prolongation']} "
+                        f"prolongation) vs plain MG-PCG, to {args.tol:g}",
+            "step": "one learned MG-PCG iteration on this system (value: the whole learned solve)",
+            "blocks_per_level": [int(c) for c in np.bincount(lv, minlength=cfg.levels)],
+            "pcg_blocks": nb0,
+            "pcg_cells": nb0 * cfg.bsx ** 3,
+            "relax_blocks_mg0": len(g.relax_list(0)),
+            "tol": args.tol,
+            "parallelism": "1 GPU",
+        },
+        "learned_vs_plain": {
+            "plain_ms": round(ms_p, 1), "plain_iterations": it_p, "plain_rel_residual": rr_p,
+            "plain_ms_per_iteration": round(ms_p / max(it_p, 1), 3),
+            "learned_ms": round(ms_l, 1), "learned_iterations": it_l, "learned_rel_residual": rr_l,
+            "learned_fallback_iteration": fb, "learned_ms_per_iteration": round(steps_ms / max(k_it, 1), 3),
+            "speedup": round(ms_p / ms_l, 3), "timed_region_wall_ms": round(wall_ms, 1),
+            "train": train, "weights_l2": float((ws.astype(float) ** 2).sum() ** 0.5),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_relax_c32h<FP16> (one red-black colour pass over 32^3 blocks: 4 local sweeps + Jacobi)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic("k_relax_c32h<1>", cfg.name),
+            "launch_ms": round(ms_relax, 4),
+            "cells_per_launch": cells_relax,
+            "bytes_per_cell": 22,
+        },
+        "other_kernels": {
+            "residual_k_multiply_z32_GBs": round(bytes_res / (ms_res * 1e-3) / 1e9, 1),
+            "pcg_update_GBs": round(bytes_upd / (ms_upd * 1e-3) / 1e9, 1),
+        },
+    }
+    return rec, g, it_p
+
+
+def plain_line(msbgx, cfg, args, log):
+    """--config c1/c2/c4/c5b16: plain MG-PCG time to --tol (FP32 storage)."""
     r = gpu_run(msbgx, cfg, args, log)
     ms = sorted(r["times"])[len(r["times"]) // 2]
     ms_relax, bytes_relax, cells_relax = r["relax"]
@@ -634,34 +768,49 @@ def main():
             "pcg_update_GBs": round(r["update"][1] / (r["update"][0] * 1e-3) / 1e9, 1),
         },
     }
-    if not args.no_precision:
-        line["precision_form"] = precision_runs(msbgx, r["g"], cfg, args, log)
-    if not args.no_flip:
-        line["flip_caller"] = flip_runs(msbgx, r["g"], cfg, args, log)
+    return line, r["g"], r["iters"][0]
+
+
+def main():
+    args = parse()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.no_graph:
+        os.environ["MSBGX_NO_GRAPH"] = "1"   # every handle created below launches eagerly
+    import msbgx
+    cfg = make_cfg(msbgx, args.config, args.n)
+    if args.pmc_calibration:
+        pmc_calibration(msbgx, cfg, log)
+        return
+    if args.config == "c5":
+        line, g, gpu_iters = c5_headline(msbgx, args, log)
+    else:
+        line, g, gpu_iters = plain_line(msbgx, cfg, args, log)
+    if args.precision:
+        line["precision_form"] = precision_runs(msbgx, g, cfg, args, log)
+    g.destroy()
+    if not args.no_extras:
+        c1 = msbgx.config_c1(256)
+        g1 = msbgx.build_grid(c1)
+        g1.prepare_pressure_solve_flip(scene=c1.scene)
+        line["flip_caller"] = flip_runs(msbgx, g1, c1, args, log)
         line["particle_to_grid"] = p2g_runs(msbgx, args, log)
-        line["compact_mic0_pcg"] = compact_runs(msbgx, r["g"], cfg, args, log)
+        line["compact_mic0_pcg"] = compact_runs(msbgx, g1, c1, args, log)
         line["pde_demo"] = pde_runs(msbgx, args, log)
-    if not args.no_batch:
         line["batched_form"] = batched_runs(msbgx, args, log)
-    if not args.no_learned:
-        forms = args.learned_forms.split(",")
-        if "fine" in forms:
-            line["learned_form"] = learned_runs(msbgx, args, log, "fine")
-        if "coarse" in forms:
-            line["learned_coarse_form"] = learned_runs(msbgx, args, log, "coarse")
+        g1.destroy()
+    if args.learned_forms:
+        for form in args.learned_forms.split(","):
+            line[f"learned_{form}_form_dambreak"] = learned_runs(msbgx, args, log, form)
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline(cfg, args, r["iters"][0], log)
+            line["cpu_baseline"] = cpu_baseline(cfg, args, gpu_iters, log)
         except Exception as e:
             log(f"cpu baseline failed: {e}")
             line["cpu_baseline"] = None
-    if args.secondary:
-        c1 = msbgx.config_c1(256)
-        a2 = argparse.Namespace(**vars(args))
-        a2.warmup, a2.steps = 1, 3
-        r1 = gpu_run(msbgx, c1, a2, log)
-        line["secondary"] = {"workload": "c1 256^3 single level 16^3 blocks", "ms": sorted(r1["times"])[1],
-                             "iterations": r1["iters"][0]}
     print(json.dumps(line), flush=True)
 
 

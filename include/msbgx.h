@@ -299,10 +299,13 @@ int msbgx_vcycle(msbgx_grid *g, int n_pre, int n_post, int n_coarse);
 
 /* Measurement: runs `reps` launches of the kernel `which` on the handle's
  * stream between HIP events and returns the mean duration per launch in ms
- * and the algorithmic HBM bytes one launch moves (FP32 storage).
- *   which 0: one red-black colour pass of the smoother over the largest
- *            same-size block list at MG0 (30 B per cell: x read+write, b, D,
- *            W_x, W_y, W_z, flags)
+ * and the algorithmic HBM bytes one launch moves.
+ *   which 0: red-black colour passes of the smoother over the block size
+ *            with the most relaxation cells at MG0, the two colour lists in
+ *            turn (reps rounded up to whole pairs); cells per launch are the
+ *            mean of the two lists; 30 B per cell with FP32 operands (x read
+ *            and written, b, D, W_x, W_y, W_z, flags), 22 B with FP16/BF16
+ *            operands (D and W 2 B each)
  *   which 1: residual r = b - A x over the MG0 fluid blocks of the finest
  *            level (30 B per cell)
  *   which 2: PCG update x += a d, r -= a q, |r|^2 (24 B per cell)

@@ -2439,8 +2439,12 @@ static void launchMultiplyB(const Grid &g, int L, const DevList &l, const float 
                             double *partials, int base, const int *done, hipStream_t s)
 {
   if (l.n == 0) return;
-  hipLaunchKernelGGL(k_multiply<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
-                     partials, base, done);
+  if constexpr (B == 32)
+    hipLaunchKernelGGL(k_multiply_z<32>, dim3(l.n), dim3(32 * 32), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
+                       partials, base, done);
+  else
+    hipLaunchKernelGGL(k_multiply<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
+                       partials, base, done);
 }
 
 /* multiplyLaplacianMatrixOpt over all non-NO_FLUID blocks of MG level L;
@@ -2458,11 +2462,7 @@ static int multiplyLaunch(const Grid &g, int L, bool residual, const float *X, c
       case 4: launchMultiplyB<4>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
       case 8: launchMultiplyB<8>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
       case 16: launchMultiplyB<16>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
-      case 32:
-        if (l.n)
-          hipLaunchKernelGGL(k_multiply_z<32>, dim3(l.n), dim3(32 * 32), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y,
-                             residual, partials, base, done);
-        break;
+      case 32: launchMultiplyB<32>(g, L, l, X, Bv, Y, residual, partials, base, done, s); break;
     }
     base += l.n;
   }
@@ -2785,17 +2785,20 @@ int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, in
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *nd = d.zeroFlag;
-  /* pick the largest same-size block list of one colour at MG0 */
-  int bestC = 0, bestL = 0, bestN = -1;
-  for (int c = 0; c < 2; c++)
-    for (int l = 0; l < g.nLevels; l++) {
-      const int b = g.bsxL(l);
-      const int n = m0.dRelax[c][l].n * b * b * b;
-      if (n > bestN) { bestN = n; bestC = c; bestL = l; }
-    }
+  /* the smoother: the block size with the most relaxation cells at MG0, its
+   * two colour lists launched in turn as a V-cycle does (one launch = one
+   * colour pass) */
+  int bestL = 0, bestN = -1;
+  for (int l = 0; l < g.nLevels; l++) {
+    const int b = g.bsxL(l);
+    const int n = (m0.dRelax[0][l].n + m0.dRelax[1][l].n) * b * b * b;
+    if (n > bestN) { bestN = n; bestL = l; }
+  }
+  int colour = 0;
   auto launch = [&]() {
     if (which == 0) {
-      launchRelaxLev(g, 0, bestL, m0.dRelax[bestC][bestL], m0.d_ch[C_Z], m0.d_ch[C_R], 0, nd, d.stream);
+      launchRelaxLev(g, 0, bestL, m0.dRelax[colour][bestL], m0.d_ch[C_Z], m0.d_ch[C_R], 0, nd, d.stream);
+      colour ^= 1;
     } else if (which == 1) {
       const DevList &l = m0.dAct[0];
       switch (g.bsx0) {
@@ -2813,7 +2816,9 @@ int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, in
   PcgState init{};
   init.alpha = 1e-3f;
   HIPCHK(hipMemcpyAsync(d.state, &init, sizeof(init), hipMemcpyHostToDevice, d.stream));
-  launch();  // warm-up
+  launch();  // warm-up (both colours for the smoother)
+  if (which == 0) launch();
+  if (which == 0 && (reps & 1)) reps++;   // whole colour pairs
   HIPCHK(hipEventRecord(d.ev0, d.stream));
   for (int i = 0; i < reps; i++) launch();
   HIPCHK(hipEventRecord(d.ev1, d.stream));
@@ -2824,7 +2829,10 @@ int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, in
   *ms = (double)t / reps;
   int64_t c;
   double bpc;
-  if (which == 0) { c = bestN; bpc = 30.0; }
+  /* smoother: cells per launch averaged over the two colours; algorithmic
+   * bytes per cell 30 with FP32 operands (flags 2, D and W_x,y,z 16, x read
+   * and written 8, b 4), 22 with 16-bit ones (D and W 8) */
+  if (which == 0) { c = bestN / 2; bpc = g.prec == MSBGX_PREC_FP32 ? 30.0 : 22.0; }
   else if (which == 1) { c = (int64_t)m0.dAct[0].n * g.bsx0 * g.bsx0 * g.bsx0; bpc = 30.0; }
   else { c = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0; bpc = 24.0; }
   *cells = (int)c;

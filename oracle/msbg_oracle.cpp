@@ -33,6 +33,7 @@
 #include <cstring>
 #include <limits>
 #include <deque>
+#include <mutex>
 #include <memory>
 #include <queue>
 #include <vector>
@@ -79,14 +80,25 @@ template <typename T> struct Grid {
   }
   T *get(int bid) const { return map[bid]; }
   bool isValue(int bid) const { return map[bid] && map[bid] != emptyBlock; }
-  /* alloc-on-write; fresh blocks are zero */
+  /* alloc-on-write; fresh blocks are zero.  Called from block-parallel
+   * loops (each block by one thread): the shared pool is appended under a
+   * lock, the block's own map slot needs none. */
   T *alloc(int bid)
   {
     if (!map[bid] || map[bid] == emptyBlock) {
-      pool.emplace_back(new T[nvox()]());
-      map[bid] = pool.back().get();
+      T *blk = new T[nvox()]();
+      {
+        std::lock_guard<std::mutex> lk(poolMutex());
+        pool.emplace_back(blk);
+      }
+      map[bid] = blk;
     }
     return map[bid];
+  }
+  static std::mutex &poolMutex()
+  {
+    static std::mutex m;
+    return m;
   }
   bool inRange(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < sx && y < sy && z < sz; }
   T getValue(int x, int y, int z) const
