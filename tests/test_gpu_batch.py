@@ -137,3 +137,59 @@ def test_batch_autotune_keeps_results():
     np.testing.assert_array_equal(rr, rr0)
     for a, b in zip(p0, pressure(batch)):
         np.testing.assert_array_equal(a, b)
+
+
+def test_full_batch_of_64_problems():
+    """The whole batched workload: 64 problems of sizes {64,128,192,256}^3,
+    one or two resolutions.  Every problem's iterations, residual and
+    pressure equal its own solve's, and a 192^3 and a 256^3 problem (one of
+    each resolution kind where the draw has them) match the oracle: same
+    iteration count, pressure within 1e-5 relative L2."""
+    from oracle_lib import build_oracle
+    print("full batch of 64 problems", flush=True)
+    cfgs = msbgx.config_c3(seed=1234, n_problems=64)
+    mi = 400
+    singles = [prepared(c) for c in cfgs]
+    single_res = []
+    for q, g in enumerate(singles):
+        it, rr = g.solve_pressure_flip(max_iter=mi, tol=1e-6)
+        single_res.append((it, rr, pressure(g)))
+    print("  individual solves done", flush=True)
+    batch = msbgx.BatchGrid.create(singles)
+    it_b, rr_b = batch.solve(max_iter=mi, tol=1e-6)
+    for q, (c, g) in enumerate(zip(cfgs, singles)):
+        it, rr, ps = single_res[q]
+        assert it == it_b[q] and rr == rr_b[q], f"problem {q} {c.name}: single {it} {rr} batch {it_b[q]} {rr_b[q]}"
+        pb = pressure(batch, batch.bases[q], g.n_blocks)
+        for b in range(g.n_blocks):
+            np.testing.assert_array_equal(ps[b], pb[b], err_msg=f"problem {q} block {b}")
+    print(f"  batch equals the individual solves; {int((it_b < mi).sum())}/64 converged", flush=True)
+    picks = []
+    for n in (192, 256):
+        for mode in (-1, 0):
+            cand = [q for q, c in enumerate(cfgs) if c.n == n and c.level_mode == mode and it_b[q] < mi]
+            if cand:
+                picks.append(cand[0])
+    assert any(cfgs[q].n == 256 for q in picks) and any(cfgs[q].n == 192 for q in picks)
+    for q in picks:
+        c = cfgs[q]
+        o = build_oracle(c)
+        o.prepare_scene(c.scene)
+        it_o, rr_o, _ = o.solve(max_iter=mi, tol=1e-6)
+        lv, _ = singles[q].block_info(0)
+        num = den = 0.0
+        pb = pressure(batch, batch.bases[q], singles[q].n_blocks)
+        for b in range(singles[q].n_blocks):
+            if lv[b] != 0:
+                continue
+            r = o.get_block(msbgx.CH_PRESSURE, 0, b, c.bsx).astype(np.float64)
+            num += float(((pb[b].astype(np.float64) - r) ** 2).sum())
+            den += float((r ** 2).sum())
+        rel = np.sqrt(num / max(den, 1e-300))
+        print(f"  problem {q} {c.name} mode {c.level_mode}: batch {it_b[q]} oracle {it_o}, pressure rel {rel:.2e}",
+              flush=True)
+        assert it_b[q] == it_o and rel <= 1e-5
+        del o
+    batch.destroy()
+    for g in singles:
+        g.destroy()
