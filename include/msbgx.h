@@ -422,9 +422,58 @@ int msbgx_compact_mic0(msbgx_grid *g, float *precon, float *z, int cap);
  * [0, 1], num_iter rounded up to even, time step time_step.  density is the
  * dense sx0*sy0*sz0 channel (x fastest), smoothed in place on the GPU; only
  * voxels of level-0 blocks are read or written.  Needs the level map, not a
- * prepare; block edge <= 16.  *ms (may be NULL) receives the device time. */
+ * prepare; block edge <= 32.  *ms (may be NULL) receives the device time. */
 int msbgx_pde_mean_curvature(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active, int num_iter,
                              float time_step, double *ms);
+
+/* Reference: MultiresSparseGrid::applyChannelPdeFast<Data_T>
+ * (src/msbg4.cpp:60-960, declared src/msbg.h:2176-2198) over a density
+ * channel, with the in-place multicolour block schemes.  The parameters keep
+ * the reference's meaning:
+ *   lapl_typ      the reference's encoding: -(type + MSBGX_PDE_OPT_8_COLOR)
+ *                 for 8 colour passes, -type for red-black passes (colour
+ *                 (bx+by+bz)&1 = 1 first); type 4 mean curvature
+ *                 (PDE_MEAN_CURVATURE), 2 bi-Laplacian (two-voxel halo),
+ *                 1 Laplacian times |grad|, 3 simple diffusion, 14
+ *                 Perona-Malik, 15 its Jacobi form, 17 Jacobi diffusion
+ *                 (src/msbg4.cpp:555-800);
+ *   do_constr_zero_one, dens_thr, dens_thr_tol1, dens_thr_tol2
+ *                 the [0, 1] clamp, or hard constraints around the input
+ *                 when dens_thr or dens_thr_tol1 > 1e-12
+ *                 (src/msbg4.cpp:806-820);
+ *   num_iter      rounded up to even; grid_spacing must be 1;
+ *   time_step, taubin (Taubin alternation for types 4 and 1),
+ *   anis_diff_alpha (types 14/15; < 0 selects the exponential conductance).
+ * Rejected with MSBGX_EINVAL, as undefined or not built: laplTyp >= 0 (the
+ * unordered double-buffered scheme), types 1/3/17 at block edges other than
+ * 16 (their fixed 18-voxel row stride, src/msbg4.cpp:688-693), red-black
+ * passes with types 2 and 4 (their stencils read same-colour diagonal
+ * blocks while those are written), type 2 at block edge 32, a uint16
+ * channel without a clamp, anis_diff_alpha 0 for types 14/15, negative
+ * num_iter or time step.  The reference's lambda, lambdaSDF, redist,
+ * velocity-dependent and stochastic-rounding (uint8) options are not built
+ * (lambda is forced to 0 there, src/msbg4.cpp:164).
+ * dtype MSBGX_PDE_U16: uint16 render density read as v/65535, stored
+ * rounded to nearest even; MSBGX_PDE_F32: float.  channel is the dense
+ * sx0*sy0*sz0 array (x fastest), smoothed in place on the GPU over the
+ * listed level-0 blocks; only voxels of level-0 blocks are read or
+ * written.  Needs the level map, not a prepare; block edge <= 32.  *ms
+ * (may be NULL) receives the device time. */
+#define MSBGX_PDE_OPT_8_COLOR (1 << 29)
+#define MSBGX_PDE_U16 0
+#define MSBGX_PDE_F32 1
+typedef struct msbgx_pde_params {
+  int lapl_typ;
+  int do_constr_zero_one;
+  float dens_thr, dens_thr_tol1, dens_thr_tol2;
+  int num_iter;
+  float grid_spacing;
+  float time_step;
+  float taubin;
+  float anis_diff_alpha;
+} msbgx_pde_params;
+int msbgx_pde_apply(msbgx_grid *g, int dtype, void *channel, const int *active_blocks, int n_active,
+                    const msbgx_pde_params *params, double *ms);
 
 /* Allocation-free variant of msbgx_pde_mean_curvature.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code

@@ -779,6 +779,17 @@ int msbgx_pde_mean_curvature(msbgx_grid *g, uint16_t *density, const int *active
   return devicePdeMeanCurvature(*g, density, active_blocks, n_active, num_iter, time_step, ms);
 }
 
+int msbgx_pde_apply(msbgx_grid *g, int dtype, void *channel, const int *active_blocks, int n_active,
+                    const msbgx_pde_params *params, double *ms)
+{
+  if (int rc = checkSingle(g)) return rc;
+  if (!channel || !params || n_active < 0 || (n_active > 0 && !active_blocks)) {
+    setError("bad argument");
+    return MSBGX_EINVAL;
+  }
+  return devicePdeApply(*g, dtype, channel, active_blocks, n_active, *params, ms);
+}
+
 int msbgx_pde_mean_curvature_noalloc(msbgx_grid *g, uint16_t *density, const int *active_blocks, int n_active,
                                      int num_iter, float time_step)
 {

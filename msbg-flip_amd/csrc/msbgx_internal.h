@@ -168,7 +168,9 @@ int p2gToVelocity(P2gCtx &c, Grid &g);
 /* compact MIC(0)-PCG comparator (compact.hip) */
 int deviceSolveCompact(Grid &g, int maxIter, float tol, int *iters, float *rel, double *ms);
 int deviceCompactApply(Grid &g, float *preconOut, float *zOut, int cap, int *nRows);
-/* msbg_demo mean-curvature smoothing (pde.hip) */
+/* msbg_demo PDE smoothing (pde.hip) */
+int devicePdeApply(const Grid &g, int dtype, void *dense, const int *active, int nActive, const msbgx_pde_params &p,
+                   double *ms);
 int devicePdeMeanCurvature(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter, float dt,
                            double *ms);
 int devicePdeMeanCurvatureNoAlloc(const Grid &g, uint16_t *dense, const int *active, int nActive, int numIter,

@@ -149,6 +149,19 @@ class P2GParams(C.Structure):
 PARTICLE_DTYPE = np.dtype([("pos", np.float32, 3), ("vel", np.float32, 3), ("phase", np.int32)])
 
 
+PDE_OPT_8_COLOR = 1 << 29          # OPT_8_COLOR_SCHEME (src/msbg.h:132)
+PDE_MEAN_CURVATURE, PDE_BIHARMONIC = 4, 2
+PDE_U16, PDE_F32 = 0, 1
+
+
+class PdeParams(C.Structure):
+    """msbgx_pde_params: applyChannelPdeFast's parameters (src/msbg.h:2176-2198)."""
+    _fields_ = [("lapl_typ", C.c_int), ("do_constr_zero_one", C.c_int), ("dens_thr", C.c_float),
+                ("dens_thr_tol1", C.c_float), ("dens_thr_tol2", C.c_float), ("num_iter", C.c_int),
+                ("grid_spacing", C.c_float), ("time_step", C.c_float), ("taubin", C.c_float),
+                ("anis_diff_alpha", C.c_float)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libmsbgx.so not built at {LIB_PATH}; run __graft_entry__.build()")
@@ -231,6 +244,7 @@ def _load():
         "msbgx_compact_mic0": (C.c_int, [P, P, P, C.c_int]),
         "msbgx_pde_mean_curvature": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_double)]),
         "msbgx_pde_mean_curvature_noalloc": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_float]),
+        "msbgx_pde_apply": (C.c_int, [P, C.c_int, P, P, C.c_int, C.POINTER(PdeParams), C.POINTER(C.c_double)]),
         "msbgx_flip_set_velocity": (C.c_int, [P, P, P, P]),
         "msbgx_flip_set_velocity_scene": (C.c_int, [P, C.POINTER(Scene)]),
         "msbgx_flip_get_velocity": (C.c_int, [P, P, P, P]),
@@ -266,7 +280,7 @@ EXPORTED = [
     "msbgx_p2g_destroy", "msbgx_p2g_run", "msbgx_p2g_run_scene", "msbgx_p2g_get", "msbgx_p2g_active_blocks",
     "msbgx_p2g_last_times", "msbgx_p2g_to_velocity",
     "msbgx_solve_pressure_compact", "msbgx_compact_mic0", "msbgx_pde_mean_curvature",
-    "msbgx_pde_mean_curvature_noalloc",
+    "msbgx_pde_mean_curvature_noalloc", "msbgx_pde_apply",
 ]
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
@@ -394,6 +408,23 @@ class MultiresSparseGrid:
         a = np.ascontiguousarray(active_blocks, np.int32)
         ms = C.c_double()
         _ck(lib.msbgx_pde_mean_curvature(self._h, _ptr(d), _ptr(a), len(a), num_iter, time_step, C.byref(ms)), "pde")
+        return d, ms.value
+
+    def pde_apply(self, channel, active_blocks, lapl_typ, do_constr_zero_one=1, dens_thr=0.0, dens_thr_tol1=0.0,
+                  dens_thr_tol2=0.0, num_iter=5, grid_spacing=1.0, time_step=0.025, taubin=0.0, anis_diff_alpha=0.0):
+        """applyChannelPdeFast (src/msbg4.cpp:60-960) on a [z][y][x] uint16
+        (render density) or float32 array, `lapl_typ` in the reference's
+        encoding (-(type + PDE_OPT_8_COLOR): 8 colours, -type: red-black);
+        returns (smoothed copy, device ms)."""
+        if channel.dtype == np.uint16:
+            d, dtype = np.ascontiguousarray(channel).copy(), PDE_U16
+        else:
+            d, dtype = np.ascontiguousarray(channel, np.float32).copy(), PDE_F32
+        a = np.ascontiguousarray(active_blocks, np.int32)
+        p = PdeParams(lapl_typ, do_constr_zero_one, dens_thr, dens_thr_tol1, dens_thr_tol2, num_iter, grid_spacing,
+                      time_step, taubin, anis_diff_alpha)
+        ms = C.c_double()
+        _ck(lib.msbgx_pde_apply(self._h, dtype, _ptr(d), _ptr(a), len(a), C.byref(p), C.byref(ms)), "pde_apply")
         return d, ms.value
 
     def pde_mean_curvature_noalloc(self, density, active_blocks, num_iter=5, time_step=0.05):

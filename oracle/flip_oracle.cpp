@@ -21,6 +21,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -480,69 +481,243 @@ extern "C" int orc_compact_pcg(int nx, int ny, int nz, int bsx, const int *block
 }
 
 /* ------------------------------------------------------------------------
- * Mean-curvature smoothing of a uint16 render-density channel, 8-colour
- * in-place block scheme: applyChannelPdeFast<RenderDensity> as msbg_demo
- * calls it (src/msbg4.cpp:60-960, src/msbg_demo.cpp:716-727), halo blocks
- * with OPT_BC_NEUMANN (src/halo.cpp:267-480, src/halo.h:267-290),
- * block colours (src/msbg.h:642-646).  rho is the dense nx*ny*nz channel
- * (x fastest); only voxels of level-0 blocks hold data.
+ * applyChannelPdeFast (src/msbg4.cpp:60-960) over a density channel of
+ * uint16 render density (dtype 0, read as v/65535, stored rounded to
+ * nearest even) or float (dtype 1), with the two in-place block schemes:
+ *   laplTyp < 0 carries the scheme (src/msbg4.cpp:100-112): with
+ *   OPT_8_COLOR_SCHEME (1<<29) set, 8 colour passes (colour (bx&1) +
+ *   2(by&1) + 4(bz&1), src/msbg.h:642-646), pass c updates colour c; else
+ *   red-black, pass p updates the blocks with (bx+by+bz)&1 != p
+ *   (src/msbg4.cpp:427-434), i.e. colour 1 first;
+ *   types (src/msbg4.cpp:555-800): 4 mean curvature, 2 bi-Laplacian (halo
+ *   2), 1 Laplacian times |grad|, 3 simple diffusion, 14 Perona-Malik, 15
+ *   its Jacobi form, 17 Jacobi diffusion; Taubin alternation for 4 and 1
+ *   (src/msbg4.cpp:621-628, 710-717); hard constraints [phi0 - tol1,
+ *   phi0 + tol2] within [0, 1] against the input copy, else the optional
+ *   [0, 1] clamp (src/msbg4.cpp:806-820); numIter rounded up to even;
+ *   halo blocks with OPT_BC_NEUMANN (src/halo.cpp:267-480,
+ *   src/halo.h:267-290).
+ * Returns -1 for what the reference does not define or this restatement
+ * leaves out: the unordered (laplTyp >= 0) scheme, types with the fixed
+ * 18-voxel row stride (1, 3, 17) at block edges other than 16
+ * (src/msbg4.cpp:688-693), red-black with stencils that reach diagonal
+ * neighbour blocks (2, 4: a same-colour block is then read while it is
+ * written), uint16 without a clamp, anisDiffAlpha 0, negative numIter or
+ * time step (src/msbg4.cpp:144-148, 178-185).
  * ---------------------------------------------------------------------- */
-extern "C" void orc_pde_mean_curvature(int nx, int ny, int nz, int bsx, const int *blockLevel, uint16_t *rho,
-                                       const int *active, int nActive, int numIter, float dt)
+namespace {
+
+/* FMA_FAST_EXP (src/fastmath.h:583-626): Cephes-style expf with the
+ * reference's own coefficient sequence (its PX2 term is added twice). */
+float pdeFastExp(float x0)
 {
+  const float MAXLOGF = 88.72283905206835f, MINLOGF = -88.f;
+  const float C1F = 0.693359375f, C2F = -2.12194440e-4f;
+  const float P1 = 1.9875691500E-4f, P2 = 1.3981999507E-3f, P3 = 8.3334519073E-3f, P4 = 4.1665795894E-2f,
+              P5 = 1.6666665459E-1f, P6 = 5.0000001201E-1f;
+  const float LOG2EF = 1.44269504088896341f;
+  float x = x0;
+  float z = std::floor(LOG2EF * x + 0.5f);
+  x -= z * C1F;
+  x -= z * C2F;
+  const int n = (int)z;
+  const float x2 = x * x;
+  z = x * P1;
+  z += P2;
+  z *= x;
+  z += P2;
+  z *= x;
+  z += P3;
+  z *= x;
+  z += P4;
+  z *= x;
+  z += P5;
+  z *= x;
+  z += P6;
+  z *= x2;
+  z += x + 1.0f;
+  uint32_t bits = (uint32_t)(n + 0x7f) << 23;
+  float scale;
+  std::memcpy(&scale, &bits, 4);
+  z *= scale;
+  if (x0 < MINLOGF) z = 0.0f;
+  if (x0 > MAXLOGF) z = INFINITY;
+  return z;
+}
+
+float pdeConductance(float f, float alpha)
+{
+  if (alpha < 0) {
+    const float t = std::fabs(f) * (1.0f / -alpha);
+    return pdeFastExp(-(t * t));
+  }
+  const float t = std::fabs(f) / alpha;
+  return 1.0f / (1.0f + t * t);
+}
+
+}  // namespace
+
+extern "C" int orc_pde_apply(int nx, int ny, int nz, int bsx, const int *blockLevel, int dtype, void *data,
+                             const int *active, int nActive, int laplTyp, int doConstrZeroOne, float densThr,
+                             float densThrTol1, float densThrTol2, int numIter, float timeStep, float taubin,
+                             float anisDiffAlpha)
+{
+  const int OPT_8_COLOR = 1 << 29;
+  bool color8 = false, redBlack = false;
+  if (laplTyp < 0) {
+    laplTyp = -laplTyp;
+    if (laplTyp & OPT_8_COLOR) {
+      laplTyp &= ~OPT_8_COLOR;
+      color8 = true;
+    } else {
+      redBlack = true;
+    }
+  }
+  const int typ = laplTyp;
+  if (!color8 && !redBlack) return -1;
+  if (typ != 1 && typ != 2 && typ != 3 && typ != 4 && typ != 14 && typ != 15 && typ != 17) return -1;
+  if ((typ == 1 || typ == 3 || typ == 17) && bsx != 16) return -1;
+  if (redBlack && (typ == 2 || typ == 4)) return -1;
+  if ((typ == 14 || typ == 15) && anisDiffAlpha == 0.0f) return -1;
+  if (numIter < 0 || timeStep < 0) return -1;
+  const bool doHard = densThr > 1e-12f || densThrTol1 > 1e-12f;
+  if (dtype == 0 && !doHard && !doConstrZeroOne) return -1;
+  const bool doTaubin = std::fabs(taubin) > 1e-20f;
+  const int hs = typ == 2 ? 2 : 1;
   const int nbx = nx / bsx, nby = ny / bsx;
-  const int hsx = bsx + 2, hsx2 = hsx * hsx;
+  const int hsx = bsx + 2 * hs, hsx2 = hsx * hsx;
+  uint16_t *rho16 = static_cast<uint16_t *>(data);
+  float *rhoF = static_cast<float *>(data);
   auto idx = [&](int x, int y, int z) { return ((int64_t)z * ny + y) * nx + x; };
   auto hasData = [&](int x, int y, int z) {
     if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return false;
     return blockLevel[x / bsx + nbx * (y / bsx + nby * (z / bsx))] == 0;
   };
+  auto get = [&](int64_t i) { return dtype == 0 ? rho16[i] * (1.f / 65535) : rhoF[i]; };
+  std::vector<float> phi0;
+  if (doHard) {
+    phi0.resize((size_t)nx * ny * nz);
+    for (int64_t i = 0; i < (int64_t)phi0.size(); i++) phi0[i] = get(i);
+  }
   int nMaxIter = numIter;
   if ((nMaxIter % 2) == 1) nMaxIter++;
-  std::vector<float> halo((size_t)hsx * hsx * hsx);
+  const float dt = timeStep;
+  std::vector<float> halo((size_t)hsx * hsx2);
   for (int iIter = 0; iIter < nMaxIter; iIter++)
-    for (int pass = 0; pass < 8; pass++)
+    for (int pass = 0; pass < (color8 ? 8 : 2); pass++)
       for (int t = 0; t < nActive; t++) {
         const int bid = active[t];
         const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
-        const int color = (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
-        if (color != pass) continue;
+        if (redBlack && ((bx + by + bz) & 1) == pass) continue;
+        if (color8 && (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2) != pass) continue;
         if (blockLevel[bid] != 0) continue;
         const int x0 = bx * bsx, y0 = by * bsx, z0 = bz * bsx;
         /* fillHaloBlock_ with OPT_BC_NEUMANN */
-        for (int z = z0 - 1; z <= z0 + bsx; z++)
-          for (int y = y0 - 1; y <= y0 + bsx; y++)
-            for (int x = x0 - 1; x <= x0 + bsx; x++) {
-              uint16_t val;
-              if (hasData(x, y, z)) val = rho[idx(x, y, z)];
-              else val = rho[idx(std::min(std::max(x, x0), x0 + bsx - 1), std::min(std::max(y, y0), y0 + bsx - 1),
-                                 std::min(std::max(z, z0), z0 + bsx - 1))];
-              halo[(size_t)(x - x0 + 1) + hsx * (y - y0 + 1) + hsx2 * (z - z0 + 1)] = val * (1.f / 65535);
+        for (int z = z0 - hs; z < z0 + bsx + hs; z++)
+          for (int y = y0 - hs; y < y0 + bsx + hs; y++)
+            for (int x = x0 - hs; x < x0 + bsx + hs; x++) {
+              int64_t src;
+              if (hasData(x, y, z)) src = idx(x, y, z);
+              else src = idx(std::min(std::max(x, x0), x0 + bsx - 1), std::min(std::max(y, y0), y0 + bsx - 1),
+                             std::min(std::max(z, z0), z0 + bsx - 1));
+              halo[(size_t)(x - x0 + hs) + hsx * (y - y0 + hs) + hsx2 * (z - z0 + hs)] = get(src);
             }
         for (int vz = 0; vz < bsx; vz++)
           for (int vy = 0; vy < bsx; vy++)
             for (int vx = 0; vx < bsx; vx++) {
-              const float *ph = halo.data() + (vx + 1) + hsx * (vy + 1) + hsx2 * (vz + 1);
+              const float *ph = halo.data() + (vx + hs) + hsx * (vy + hs) + hsx2 * (vz + hs);
               auto F_ = [&](int dx, int dy, int dz) { return ph[dx + dy * hsx + dz * hsx2]; };
               float phi = F_(0, 0, 0);
-              const float f0 = F_(0, 0, 0), f1 = F_(-1, 0, 0), f2 = F_(1, 0, 0), f3 = F_(0, -1, 0),
-                          f4 = F_(0, 1, 0), f5 = F_(0, 0, -1), f6 = F_(0, 0, 1);
-              const float fx = 0.5f * (f2 - f1), fy = 0.5f * (f4 - f3), fz = 0.5f * (f6 - f5);
-              const float fxx = f1 + f2 - 2 * f0, fyy = f3 + f4 - 2 * f0, fzz = f5 + f6 - 2 * f0;
-              const float fy_right = (F_(1, 1, 0) - F_(1, -1, 0)) * 0.5f, fy_left = (F_(-1, 1, 0) - F_(-1, -1, 0)) * 0.5f,
-                          fz_right = (F_(1, 0, 1) - F_(1, 0, -1)) * 0.5f, fz_left = (F_(-1, 0, 1) - F_(-1, 0, -1)) * 0.5f,
-                          fz_up = (F_(0, 1, 1) - F_(0, 1, -1)) * 0.5f, fz_down = (F_(0, -1, 1) - F_(0, -1, -1)) * 0.5f,
-                          fyx = (fy_right - fy_left) * 0.5f, fzx = (fz_right - fz_left) * 0.5f,
-                          fzy = (fz_up - fz_down) * 0.5f;
-              float H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
-                        2.0f * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
-              const float gradMagSq = fx * fx + fy * fy + fz * fz;
-              H = gradMagSq > 1e-7f ? H / gradMagSq : 0.0f;
-              float D = dt * H;
-              D = std::min(std::max(D, -0.1f), 0.1f);
-              phi += D;
-              phi = std::min(std::max(phi, 0.0f), 1.0f);
-              rho[idx(x0 + vx, y0 + vy, z0 + vz)] = (uint16_t)(int)std::nearbyint(phi * 65535);
+              if (typ == 4) {
+                const float f0 = F_(0, 0, 0), f1 = F_(-1, 0, 0), f2 = F_(1, 0, 0), f3 = F_(0, -1, 0),
+                            f4 = F_(0, 1, 0), f5 = F_(0, 0, -1), f6 = F_(0, 0, 1);
+                const float fx = 0.5f * (f2 - f1), fy = 0.5f * (f4 - f3), fz = 0.5f * (f6 - f5);
+                const float fxx = f1 + f2 - 2 * f0, fyy = f3 + f4 - 2 * f0, fzz = f5 + f6 - 2 * f0;
+                const float fy_right = (F_(1, 1, 0) - F_(1, -1, 0)) * 0.5f,
+                            fy_left = (F_(-1, 1, 0) - F_(-1, -1, 0)) * 0.5f,
+                            fz_right = (F_(1, 0, 1) - F_(1, 0, -1)) * 0.5f,
+                            fz_left = (F_(-1, 0, 1) - F_(-1, 0, -1)) * 0.5f,
+                            fz_up = (F_(0, 1, 1) - F_(0, 1, -1)) * 0.5f,
+                            fz_down = (F_(0, -1, 1) - F_(0, -1, -1)) * 0.5f, fyx = (fy_right - fy_left) * 0.5f,
+                            fzx = (fz_right - fz_left) * 0.5f, fzy = (fz_up - fz_down) * 0.5f;
+                float H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
+                          2.0f * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
+                const float gradMagSq = fx * fx + fy * fy + fz * fz;
+                H = gradMagSq > 1e-7f ? H / gradMagSq : 0.0f;
+                float D = dt * H;
+                D = std::min(std::max(D, -0.1f), 0.1f);
+                if (doTaubin) phi = (iIter & 1) ? phi + D : phi - taubin * D;
+                else phi += D;
+              } else if (typ == 2) {
+                const float F0 = F_(0, 0, 0);
+                const float H = 42.f * F_(0, 0, 0) -
+                                12.f * (F_(1, 0, 0) + F_(0, 1, 0) + F_(0, 0, 1) + F_(-1, 0, 0) + F_(0, -1, 0) +
+                                        F_(0, 0, -1)) +
+                                1.f * (F_(2, 0, 0) + F_(0, 2, 0) + F_(0, 0, 2) + F_(-2, 0, 0) + F_(0, -2, 0) +
+                                       F_(0, 0, -2)) +
+                                2.f * (F_(1, 0, 1) + F_(0, 1, 1) + F_(-1, 0, 1) + F_(0, -1, 1) + F_(1, 0, -1) +
+                                       F_(0, 1, -1) + F_(-1, 0, -1) + F_(0, -1, -1) + F_(1, 1, 0) + F_(-1, 1, 0) +
+                                       F_(-1, -1, 0) + F_(1, -1, 0));
+                const float GX = 0.5f * (F_(1, 0, 0) - F_(-1, 0, 0)), GY = 0.5f * (F_(0, 1, 0) - F_(0, -1, 0)),
+                            GZ = 0.5f * (F_(0, 0, 1) - F_(0, 0, -1));
+                const float G = std::sqrt(GX * GX + GY * GY + GZ * GZ);
+                float D = -dt * H * G;
+                D = std::min(std::max(D, -0.1f), 0.1f);
+                phi = F0 + D;
+              } else if (typ == 1 || typ == 3 || typ == 17) {
+                /* fixed strides of an 18-voxel halo row (bsx 16) */
+                const float F0 = phi, F1 = ph[-1], F2 = ph[1], F3 = ph[-18], F4 = ph[18], F5 = ph[-18 * 18],
+                            F6 = ph[18 * 18];
+                if (typ == 1) {
+                  const float L = F1 + F2 + F3 + F4 + F5 + F6 - 6.0f * F0;
+                  const float GX = 0.5f * (F2 - F1), GY = 0.5f * (F4 - F3), GZ = 0.5f * (F6 - F5);
+                  const float G = std::sqrt(GX * GX + GY * GY + GZ * GZ);
+                  const float D = dt * L * G;
+                  if (doTaubin) phi = (iIter & 1) ? phi + D : phi - taubin * D;
+                  else phi += D;
+                } else if (typ == 3) {
+                  const float S = (1.0f / 6.0f) * (F1 + F2 + F3 + F4 + F5 + F6);
+                  const float D = dt * (S - phi);
+                  phi += D;
+                } else {
+                  phi = (phi + dt * (F1 + F2 + F3 + F4 + F5 + F6)) * 1.0f / (1.0f + dt * 6.0f);
+                }
+              } else {   // 14, 15
+                const float f0 = F_(0, 0, 0), f1 = F_(-1, 0, 0), f2 = F_(1, 0, 0), f3 = F_(0, -1, 0),
+                            f4 = F_(0, 1, 0), f5 = F_(0, 0, -1), f6 = F_(0, 0, 1);
+                const float g1 = f1 - f0, g2 = f2 - f0, g3 = f3 - f0, g4 = f4 - f0, g5 = f5 - f0, g6 = f6 - f0;
+                const float G1 = pdeConductance(g1, anisDiffAlpha), G2 = pdeConductance(g2, anisDiffAlpha),
+                            G3 = pdeConductance(g3, anisDiffAlpha), G4 = pdeConductance(g4, anisDiffAlpha),
+                            G5 = pdeConductance(g5, anisDiffAlpha), G6 = pdeConductance(g6, anisDiffAlpha);
+                if (typ == 15) {
+                  const float sum = G1 * f1 + G2 * f2 + G3 * f3 + G4 * f4 + G5 * f5 + G6 * f6,
+                              wsum = G1 + G2 + G3 + G4 + G5 + G6;
+                  phi = (phi + dt * sum) * 1.0f / (1.0f + dt * wsum);
+                } else {
+                  const float sum = G1 * g1 + G2 * g2 + G3 * g3 + G4 * g4 + G5 * g5 + G6 * g6;
+                  phi += dt * sum;
+                }
+              }
+              const int64_t o = idx(x0 + vx, y0 + vy, z0 + vz);
+              if (doHard) {
+                const float phiMin = std::max(0.0f, phi0[o] - densThrTol1),
+                            phiMax = std::min(1.0f, phi0[o] + densThrTol2);
+                phi = std::max(std::min(phi, phiMax), phiMin);
+              } else if (doConstrZeroOne) {
+                phi = std::min(std::max(phi, 0.0f), 1.0f);
+              }
+              if (dtype == 0) rho16[o] = (uint16_t)(int)std::nearbyint(phi * 65535);
+              else rhoF[o] = phi;
             }
       }
+  return 0;
+}
+
+/* The msbg_demo call (src/msbg_demo.cpp:716-727): mean curvature, 8
+ * colours, [0, 1] clamp, uint16 render density. */
+extern "C" void orc_pde_mean_curvature(int nx, int ny, int nz, int bsx, const int *blockLevel, uint16_t *rho,
+                                       const int *active, int nActive, int numIter, float dt)
+{
+  orc_pde_apply(nx, ny, nz, bsx, blockLevel, 0, rho, active, nActive, -(4 + (1 << 29)), 1, 0.f, 0.f, 0.f, numIter,
+                dt, 0.f, 0.f);
 }

@@ -43,6 +43,8 @@ for name, res, args in [
     ("orc_flip_rhs", None, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, C.c_float, P]),
     ("orc_scene_particles", C.c_int64, [C.POINTER(Scene), P]),
     ("orc_pde_mean_curvature", None, [C.c_int] * 4 + [P, P, P, C.c_int, C.c_int, C.c_float]),
+    ("orc_pde_apply", C.c_int, [C.c_int] * 4 + [P, C.c_int, P, P, C.c_int, C.c_int, C.c_int] + [C.c_float] * 3 +
+     [C.c_int] + [C.c_float] * 3),
     ("orc_compact_pcg", C.c_int, [C.c_int] * 4 + [P, P, C.c_float, P, P, P, P, C.c_float, C.c_int, P, P, P,
                                                    C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("orc_p2g", None, [C.c_int] * 4 + [P, C.c_int64, P] + [C.c_float] * 5 + [C.c_int] + [P] * 8),
@@ -277,3 +279,19 @@ def pde_mean_curvature(block_level, bsx, density, active, num_iter=5, dt=0.05):
     a = np.ascontiguousarray(active, np.int32)
     _lib.orc_pde_mean_curvature(nx, ny, nz, bsx, _p(lv), _p(d), _p(a), len(a), num_iter, dt)
     return d
+
+
+def pde_apply(block_level, bsx, data, active, lapl_typ, do_constr_zero_one=1, dens_thr=0.0, dens_thr_tol1=0.0,
+              dens_thr_tol2=0.0, num_iter=5, time_step=0.025, taubin=0.0, anis_diff_alpha=0.0):
+    """applyChannelPdeFast on a uint16 or float32 [z][y][x] array; returns a
+    copy, or None where the oracle declines the combination."""
+    nz, ny, nx = data.shape
+    lv = _levels(block_level, nx, ny, nz, bsx)
+    if data.dtype == np.uint16:
+        d, dtype = np.ascontiguousarray(data).copy(), 0
+    else:
+        d, dtype = np.ascontiguousarray(data, np.float32).copy(), 1
+    a = np.ascontiguousarray(active, np.int32)
+    rc = _lib.orc_pde_apply(nx, ny, nz, bsx, _p(lv), dtype, _p(d), _p(a), len(a), lapl_typ, do_constr_zero_one,
+                            dens_thr, dens_thr_tol1, dens_thr_tol2, num_iter, time_step, taubin, anis_diff_alpha)
+    return d if rc == 0 else None
