@@ -921,18 +921,23 @@ __global__ void __launch_bounds__(1024) k_pack_col32h(LevelArgs a, const int *__
 /* Register-resident smoother for 32^3 blocks with 16-bit operands (the
  * precision form; the same schedule and arithmetic as k_relax_col, so the
  * same bits).  The FP32 record kernel re-reads its 32-byte coefficient
- * record at every one of the five local sweeps.  Here a workgroup of 512
- * threads takes the block, each thread the two z-columns (x, y) and
- * (x, y + 16): both sit at register slot hp mod 32 at hyperplane step hp.
- * A thread keeps the coefficients of its 64 cells as packed 16-bit pairs in
- * registers, loaded once from LevelArgs::pack16 -- {D, W-x} per cell, W-z
- * two cells per register -- and b of the 64 cells; {W-y, W+y} of the step's
- * cells streams from LevelArgs::pack16y, one contiguous plane per step,
- * loaded K steps ahead (a ring of K registers).  W+x of a cell is W-x of its
- * x+1 neighbour, which the next lane holds (a lane shuffle), except on the
+ * record at every one of the five local sweeps.  Here a workgroup of 1024
+ * threads takes the block, each thread its z-column (x, y) (or, built with
+ * MSBGX_C32_COLUMNS=2, 512 threads with the columns (x, y) and (x, y + 16),
+ * both at register slot hp mod 32 at hyperplane step hp).  A thread keeps
+ * the coefficients of its 32 cells as packed 16-bit pairs in registers,
+ * loaded once from LevelArgs::pack16 -- {D, W-x} per cell, W-z two cells
+ * per register -- and b of the 32 cells; {W-y, W+y} of the step's cells
+ * streams from LevelArgs::pack16y, one contiguous plane per step, loaded K
+ * steps ahead (a ring of K registers).  W+x of a cell is W-x of its x+1
+ * neighbour, which the next lane holds (a lane shuffle), except on the
  * block's +x face, whose 32^2 values sit in LDS.  The block's h*x values
  * with the x/y halo faces as a one-cell ring live in LDS (34^2 * 32 floats =
- * 145 KB of gfx950's 160 KB).
+ * 145 KB of gfx950's 160 KB), so one block runs per CU: one column per
+ * thread gives 16 waves (4 per SIMD, 128 registers each) to overlap each
+ * step's LDS round trips, against 8 waves (2 per SIMD) with two columns,
+ * which run their two cells' round trips one after the other (C5 colour
+ * pass 2.74 ms against 3.57 ms).
  *
  * Registers are indexed by compile-time slot, so a sweep's 32-step period is
  * unrolled.  One unrolled body serves all four sweeps: a reverse sweep visits
@@ -941,8 +946,13 @@ __global__ void __launch_bounds__(1024) k_pack_col32h(LevelArgs a, const int *__
  * code runs with the direction a run-time value.  That keeps the hot code
  * (about 35 KB) within the instruction cache; four unrolled copies did not
  * fit and every sweep streamed its code from L2. */
+#ifndef MSBGX_C32_COLUMNS
+#define MSBGX_C32_COLUMNS 1
+#endif
 struct C32 {
-  static constexpr int B = 32, S = 8, NT = 512, NC = 2, T = Cfg<32>::T, P = B + 2, P2 = P * P, N = P2 * B;
+  /* NC z-columns per thread: (x, y + YS c) for c < NC */
+  static constexpr int NC = MSBGX_C32_COLUMNS, YS = 32 / NC;
+  static constexpr int B = 32, S = 8, NT = 1024 / NC, T = Cfg<32>::T, P = B + 2, P2 = P * P, N = P2 * B;
   static constexpr int K = 4;
 };
 
@@ -990,7 +1000,7 @@ __device__ __forceinline__ uint32_t c32Cell(int x, int y, int z) { return (uint3
  * dword of the block's contiguous record plane sl. */
 template <int CI> __device__ __forceinline__ uint32_t c32LoadWy(const C32Ctx &c, int sl)
 {
-  return __builtin_amdgcn_raw_buffer_load_b32(c.rw, c32Cell(c.tx, c.ty0 + 16 * CI, 0), ((uint32_t)sl & 31u) * 4096u,
+  return __builtin_amdgcn_raw_buffer_load_b32(c.rw, c32Cell(c.tx, c.ty0 + C32::YS * CI, 0), ((uint32_t)sl & 31u) * 4096u,
                                               0);
 }
 
@@ -1030,7 +1040,7 @@ __device__ __forceinline__ void c32Cell(C32Col &r, float *h3, const C32Ctx &c, i
 {
   constexpr int B = C32::B, P = C32::P, P2 = C32::P2, K = C32::K;
   constexpr int IP = (I + 1) % B, IM = (I + B - 1) % B;
-  const int o = c.o0 + 16 * CI;
+  const int o = c.o0 + C32::YS * CI;
   const int z = hp - o;
   const int sl = hp & (B - 1);
   uint32_t sk = c.skip[CI];   // opaque copy: the slot test is made here, not hoisted as lane masks
@@ -1048,23 +1058,29 @@ __device__ __forceinline__ void c32Cell(C32Col &r, float *h3, const C32Ctx &c, i
   const float Bb = r.bb[I];
   const uint32_t wy = r.wq[iring];
   if (active) {
+    /* every LDS read of the cell issued at once, without a condition: the
+     * +x face weight and the z neighbours at clamped planes, selected after */
+    float *p = h3 + (c.h30 + C32::YS * CI * P) + z * P2;
+    const uint16_t wxb = c.wxp[z * B + c.ty0 + C32::YS * CI];
+    const float F0 = p[0], F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
+    const float F5r = p[z > 0 ? -P2 : 0], F6r = p[z < B - 1 ? P2 : 0];
     const float D = lo16<PREC>(k0), w1 = hi16<PREC>(k0);
     const float w3 = lo16<PREC>(wy), w4 = hi16<PREC>(wy);
     const float w5 = c32Wz<PREC, I>(k5);
     const float w6n = fwd ? c32Wz<PREC, IP>(k5n) : c32Wz<PREC, IM>(k5n);
     const float w6 = z < B - 1 ? w6n : c.w6top[CI];
-    const float w2 = c.tx == B - 1 ? fromBits16(c.wxp[z * B + c.ty0 + 16 * CI], PREC) : hi16<PREC>(got);
-    float *p = h3 + (c.h30 + 16 * CI * P) + z * P2;
-    const float F0 = p[0], F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
-    const float F5 = z > 0 ? p[-P2] : c.fz0[CI];
-    const float F6 = z < B - 1 ? p[P2] : c.fz1[CI];
+    const float w2 = c.tx == B - 1 ? fromBits16(wxb, PREC) : hi16<PREC>(got);
+    const float F5 = z > 0 ? F5r : c.fz0[CI];
+    const float F6 = z < B - 1 ? F6r : c.fz1[CI];
     float Sm = F1 * w1;
     Sm = fmaf(F2, w2, Sm);
     Sm = fmaf(F3, w3, Sm);
     Sm = fmaf(F4, w4, Sm);
     Sm = fmaf(F5, w5, Sm);
     Sm = fmaf(F6, w6, Sm);
-    *p = D != 0.0f ? fmaf(c.om * (Bb + Sm), D, c.omm * F0) : F0;
+    float Y = fmaf(c.om * (Bb + Sm), D, c.omm * F0);
+    asm volatile("" : "+v"(Y));   // computed for every active lane: a select, not a branch, on D
+    *p = D != 0.0f ? Y : F0;
   }
   /* refill the ring entry after its value's last use, so the load can target
    * the same register (a copy would wait for the load) */
@@ -1083,7 +1099,7 @@ template <int PREC, int I, int END> struct C32Period {
     int hp = fwd ? step : C32::T - 1 - step;
     asm volatile("" : "+s"(hp));
     c32Cell<PREC, I, 0>(r[0], h3, c, hp, fwd, I % C32::K);
-    c32Cell<PREC, I, 1>(r[1], h3, c, hp, fwd, I % C32::K);
+    if constexpr (C32::NC > 1) c32Cell<PREC, I, 1>(r[1], h3, c, hp, fwd, I % C32::K);
     ldsBarrier();
     C32Period<PREC, I + 1, END>::run(r, h3, c, base, fwd);
   }
@@ -1103,7 +1119,7 @@ __device__ __forceinline__ void c32Sweep(C32Col (&r)[C32::NC], float *h3, const 
   for (int s = 0; s < K; s++) {
     const int hp = fwd ? s : T - 1 - s;
     r[0].wq[s] = c32LoadWy<0>(c, hp);
-    r[1].wq[s] = c32LoadWy<1>(c, hp);
+    if constexpr (C32::NC > 1) r[1].wq[s] = c32LoadWy<1>(c, hp);
   }
 #pragma unroll 1
   for (int base = 0; base < FULL; base += B) C32Period<PREC, 0, B>::run(r, h3, c, base, fwd);
@@ -1118,7 +1134,7 @@ template <int PREC, int SL, int CI>
 __device__ __forceinline__ void c32JacobiCell(C32Col &r, const float *h3, const C32Ctx &c, int slv)
 {
   constexpr int B = C32::B, P = C32::P, P2 = C32::P2, K = C32::K, SLP = (SL + 1) % B;
-  const int o = c.o0 + 16 * CI;
+  const int o = c.o0 + C32::YS * CI;
   const int z = (slv - o + 4 * B) & (B - 1);
   uint32_t k0 = r.c0[SL], k0n = r.c0[SLP], k5 = r.c5[SL / 2], k5n = r.c5[SLP / 2];
   asm volatile("" : "+v"(k0), "+v"(k0n), "+v"(k5), "+v"(k5n));
@@ -1132,8 +1148,8 @@ __device__ __forceinline__ void c32JacobiCell(C32Col &r, const float *h3, const 
     const float w3 = lo16<PREC>(wy), w4 = hi16<PREC>(wy);
     const float w5 = c32Wz<PREC, SL>(k5);
     const float w6 = z < B - 1 ? c32Wz<PREC, SLP>(k5n) : c.w6top[CI];
-    const float w2 = c.tx == B - 1 ? fromBits16(c.wxp[z * B + c.ty0 + 16 * CI], PREC) : hi16<PREC>(got);
-    const float *p = h3 + (c.h30 + 16 * CI * P) + z * P2;
+    const float w2 = c.tx == B - 1 ? fromBits16(c.wxp[z * B + c.ty0 + C32::YS * CI], PREC) : hi16<PREC>(got);
+    const float *p = h3 + (c.h30 + C32::YS * CI * P) + z * P2;
     const float F0 = p[0], F1 = p[-1], F2 = p[1], F3 = p[-P], F4 = p[P];
     const float F5 = z > 0 ? p[-P2] : c.fz0[CI];
     const float F6 = z < B - 1 ? p[P2] : c.fz1[CI];
@@ -1144,7 +1160,7 @@ __device__ __forceinline__ void c32JacobiCell(C32Col &r, const float *h3, const 
     Sm = fmaf(F5, w5, Sm);
     Sm = fmaf(F6, w6, Sm);
     const float Y = D != 0.0f ? fmaf(c.om * (r.bb[SL] + Sm), D, c.omm * F0) : F0;
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(Y * c.HR), c.rx, c32Cell(c.tx, c.ty0 + 16 * CI, z), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(Y * c.HR), c.rx, c32Cell(c.tx, c.ty0 + C32::YS * CI, z), 0, 0);
   }
   r.wq[SL % K] = c32LoadWy<CI>(c, slv + K);
 }
@@ -1154,7 +1170,7 @@ template <int PREC, int SL> struct C32JacobiAll {
     int slv = SL;   // laundered: z and the addresses are formed here
     asm volatile("" : "+s"(slv));
     c32JacobiCell<PREC, SL, 0>(r[0], h3, c, slv);
-    c32JacobiCell<PREC, SL, 1>(r[1], h3, c, slv);
+    if constexpr (C32::NC > 1) c32JacobiCell<PREC, SL, 1>(r[1], h3, c, slv);
     C32JacobiAll<PREC, SL + 1>::run(r, h3, c);
   }
 };
@@ -1170,7 +1186,7 @@ __device__ __forceinline__ void c32LoadRecords(C32Col &r, const __amdgpu_buffer_
                                                uint32_t &w6bits)
 {
   constexpr int B = C32::B, S = C32::S, NF = B * B;
-  const int ty = c.ty0 + 16 * CI, o = c.o0 + 16 * CI;
+  const int ty = c.ty0 + C32::YS * CI, o = c.o0 + C32::YS * CI;
   uint4 v[8];
 #pragma unroll
   for (int k = 0; k < 8; k++)
@@ -1203,25 +1219,26 @@ template <int CI>
 __device__ __forceinline__ void c32StageX(C32Col &r, float *h3, const C32Ctx &c, float h)
 {
   constexpr int B = C32::B, P = C32::P, P2 = C32::P2;
-  const int o = c.o0 + 16 * CI;
+  const int o = c.o0 + C32::YS * CI;
 #pragma unroll
   for (int sl = 0; sl < B; sl++) {
     int slv = sl;
     asm volatile("" : "+s"(slv));
     const int z = (slv - o + 4 * B) & (B - 1);
-    const uint32_t q = c32Cell(c.tx, c.ty0 + 16 * CI, z);
+    const uint32_t q = c32Cell(c.tx, c.ty0 + C32::YS * CI, z);
     const float xv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rx, q, 0, 0));
     r.bb[sl] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rb, q, 0, 0));
-    h3[c.h30 + 16 * CI * P + z * P2] = ((c.skip[CI] >> sl) & 1u) ? 0.0f : xv * h;
+    h3[c.h30 + C32::YS * CI * P + z * P2] = ((c.skip[CI] >> sl) & 1u) ? 0.0f : xv * h;
   }
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(512) k_relax_c32h(LevelArgs a, const int *__restrict__ blocks,
-                                                    float *__restrict__ X, const float *__restrict__ Bv,
-                                                    float omega, int reverse, const int *done)
+__global__ void __launch_bounds__(C32::NT) k_relax_c32h(LevelArgs a, const int *__restrict__ blocks,
+                                                        float *__restrict__ X, const float *__restrict__ Bv,
+                                                        float omega, int reverse, const int *done)
 {
-  constexpr int B = C32::B, S = C32::S, NF = B * B, P = C32::P, P2 = C32::P2, NT = C32::NT;
+  constexpr int B = C32::B, S = C32::S, NF = B * B, P = C32::P, P2 = C32::P2, NT = C32::NT, NC = C32::NC;
+  constexpr int YS = C32::YS;
   __shared__ float h3[C32::N];
   __shared__ uint16_t wxp[NF];
   const int bid = blocks[xcdListIndex()];
@@ -1231,27 +1248,29 @@ __global__ void __launch_bounds__(512) k_relax_c32h(LevelArgs a, const int *__re
   asm volatile("" ::"s"(lev), "s"(isDone));
   if (isDone) return;
   const float h = (float)(1 << lev);
-  const int tx = threadIdx.x % B, ty = threadIdx.x / B;   // ty in [0, 16)
-  /* halo face values of this thread's face cells (u, w) = (tx, ty) and (tx, ty + 16) */
+  const int tx = threadIdx.x % B, ty = threadIdx.x / B;   // ty in [0, YS)
+  /* halo face values of this thread's face cells (u, w) = (tx, ty + YS k) */
   int fkind[6];
   int64_t foff[6];
-  float fv[2][6];
-  const bool same = sameLevelFaces<B>(a, bid, lev, h, off, X, fv[0], fkind, foff, threadIdx.x) &&
-                    sameLevelFaces<B>(a, bid, lev, h, off, X, fv[1], fkind, foff, threadIdx.x + NT);
+  float fv[NC][6];
+  bool same = true;
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    same = same && sameLevelFaces<B>(a, bid, lev, h, off, X, fv[k], fkind, foff, threadIdx.x + NT * k);
   if (same) {
 #pragma unroll
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < NC; k++)
 #pragma unroll
       for (int f = 0; f < 6; f++) fv[k][f] = sameLevelFaceValue(fv[k][f], fkind[f], h);
   } else {
     float *scratch = h3 + (C32::N - 6 * NF);   // overwritten by x values below
-    stageFaces<B>(scratch, a, bid, lev, h, off, X, fkind, foff, threadIdx.x);
-    stageFaces<B>(scratch, a, bid, lev, h, off, X, fkind, foff, threadIdx.x + NT);
+#pragma unroll
+    for (int k = 0; k < NC; k++) stageFaces<B>(scratch, a, bid, lev, h, off, X, fkind, foff, threadIdx.x + NT * k);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < NC; k++)
 #pragma unroll
-      for (int f = 0; f < 6; f++) fv[k][f] = scratch[f * NF + tx + B * (ty + 16 * k)];
+      for (int f = 0; f < 6; f++) fv[k][f] = scratch[f * NF + tx + B * (ty + YS * k)];
     __syncthreads();
   }
   C32Ctx c;
@@ -1266,30 +1285,34 @@ __global__ void __launch_bounds__(512) k_relax_c32h(LevelArgs a, const int *__re
   if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
   c.om = omega; c.omm = 1.0f - omega; c.HR = 1.0f / h;
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NC; k++) {
     c.fz0[k] = fv[k][4];
     c.fz1[k] = fv[k][5];
     c.skip[k] = 0;
   }
   /* coefficient records, then x and b */
-  C32Col r[C32::NC];
-  uint32_t w6b0 = 0, w6b1 = 0;
+  C32Col r[NC];
+  uint32_t w6b[NC];
   const __amdgpu_buffer_rsrc_t rr = blockRsrc(a.pack16 + off, B * B * B * 16);
-  c32LoadRecords<0, 0>(r[0], rr, c, wxp, w6b0);
-  c32LoadRecords<8, 0>(r[0], rr, c, wxp, w6b0);
-  c32LoadRecords<16, 0>(r[0], rr, c, wxp, w6b0);
-  c32LoadRecords<24, 0>(r[0], rr, c, wxp, w6b0);
-  c32LoadRecords<0, 1>(r[1], rr, c, wxp, w6b1);
-  c32LoadRecords<8, 1>(r[1], rr, c, wxp, w6b1);
-  c32LoadRecords<16, 1>(r[1], rr, c, wxp, w6b1);
-  c32LoadRecords<24, 1>(r[1], rr, c, wxp, w6b1);
-  c.w6top[0] = hi16<PREC>(w6b0);
-  c.w6top[1] = hi16<PREC>(w6b1);
-  c32StageX<0>(r[0], h3, c, h);
-  c32StageX<1>(r[1], h3, c, h);
+  w6b[0] = 0;
+  c32LoadRecords<0, 0>(r[0], rr, c, wxp, w6b[0]);
+  c32LoadRecords<8, 0>(r[0], rr, c, wxp, w6b[0]);
+  c32LoadRecords<16, 0>(r[0], rr, c, wxp, w6b[0]);
+  c32LoadRecords<24, 0>(r[0], rr, c, wxp, w6b[0]);
+  if constexpr (NC > 1) {
+    w6b[1] = 0;
+    c32LoadRecords<0, 1>(r[1], rr, c, wxp, w6b[1]);
+    c32LoadRecords<8, 1>(r[1], rr, c, wxp, w6b[1]);
+    c32LoadRecords<16, 1>(r[1], rr, c, wxp, w6b[1]);
+    c32LoadRecords<24, 1>(r[1], rr, c, wxp, w6b[1]);
+  }
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const int w = ty + 16 * k;
+  for (int k = 0; k < NC; k++) c.w6top[k] = hi16<PREC>(w6b[k]);
+  c32StageX<0>(r[0], h3, c, h);
+  if constexpr (NC > 1) c32StageX<1>(r[1], h3, c, h);
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    const int w = ty + YS * k;
     h3[w * P2 + (tx + 1) * P] = fv[k][0];
     h3[w * P2 + (tx + 1) * P + B + 1] = fv[k][1];
     h3[w * P2 + tx + 1] = fv[k][2];
@@ -1302,19 +1325,19 @@ __global__ void __launch_bounds__(512) k_relax_c32h(LevelArgs a, const int *__re
   for (int k = 0; k < 4; k++) {
     const bool fwd = (reverse != 0) != ((k & 1) != 0);
     if (!fwd) {
-      c32Mirror(r[0]);
-      c32Mirror(r[1]);
+#pragma unroll
+      for (int q = 0; q < NC; q++) c32Mirror(r[q]);
     }
     c32Sweep<PREC>(r, h3, c, fwd);
     if (!fwd) {
-      c32Mirror(r[0]);
-      c32Mirror(r[1]);
+#pragma unroll
+      for (int q = 0; q < NC; q++) c32Mirror(r[q]);
     }
   }
 #pragma unroll
   for (int k = 0; k < C32::K; k++) {
     r[0].wq[k] = c32LoadWy<0>(c, k);
-    r[1].wq[k] = c32LoadWy<1>(c, k);
+    if constexpr (NC > 1) r[1].wq[k] = c32LoadWy<1>(c, k);
   }
   C32JacobiAll<PREC, 0>::run(r, h3, c);
 }
