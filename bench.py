@@ -59,7 +59,7 @@ def parse():
                          "c5/c5b16: the 1024^3 particle workload on 32^3 / 16^3 blocks)")
     ap.add_argument("--learned-forms", default="", help="comma list of fine,coarse: also run the learned forms on "
                     "the dam-break evaluation workload (--learned-eval-config)")
-    ap.add_argument("--scales", type=int, default=3, help="coarse form: scales of the network")
+    ap.add_argument("--scales", type=int, default=2, help="coarse form: scales of the network (1x, 2x, 4x block-mean grids)")
     ap.add_argument("--prolongation", type=int, default=1, help="coarse form: 0 constant, 1 trilinear")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch PCG iterations eagerly (no hipGraph) in every section (profiling runs)")
@@ -343,6 +343,7 @@ def learned_runs(msbgx, args, log, mode="fine"):
         g.destroy()
     ms_c, fl_c = net.time_conv(blocks=2048, reps=20)
     conv = {"kernel": "k_nn_conv16<false> (16->16, v_mfma_f32_16x16x32_bf16)", "blocks": 2048,
+            "flop_per_launch": fl_c, "flop_counted": "16^3 cells x 27 taps x 16 x 16 channels x 2 per block",
             "launch_ms": round(ms_c, 4), "achieved": round(fl_c / (ms_c * 1e-3) / 1e12, 1),
             "peak": 2500.0, "unit": "TFLOP/s", "frac": round(fl_c / (ms_c * 1e-3) / 1e12 / 2500.0, 4)}
     log(f"learned: conv16 roofline {conv}")
@@ -667,6 +668,13 @@ def c5_headline(msbgx, args, log):
     ms_upd, bytes_upd, _ = g.time_kernel(2, reps=20)
     achieved = bytes_relax / (ms_relax * 1e-3) / 1e9
     log(f"smoother: {ms_relax:.4f} ms per colour pass, {cells_relax} cells, {achieved:.0f} GB/s")
+    ms_c, fl_c = net.time_conv(blocks=2048, reps=20)
+    conv = {"kernel": "k_nn_conv16<false> (16->16 forward layer, v_mfma_f32_16x16x32_bf16)", "blocks": 2048,
+            "launch_ms": round(ms_c, 4), "flop_per_launch": fl_c,
+            "flop_counted": "16^3 cells x 27 taps x 16 x 16 channels x 2 per block",
+            "achieved": round(fl_c / (ms_c * 1e-3) / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(fl_c / (ms_c * 1e-3) / 1e12 / 2500.0, 4)}
+    log(f"learned: conv16 {conv}")
     ws = net.weights
     net.destroy()
     rec = {
@@ -717,6 +725,7 @@ def c5_headline(msbgx, args, log):
         "other_kernels": {
             "residual_k_multiply_z32_GBs": round(bytes_res / (ms_res * 1e-3) / 1e9, 1),
             "pcg_update_GBs": round(bytes_upd / (ms_upd * 1e-3) / 1e9, 1),
+            "mfma_conv16": conv,
         },
     }
     return rec, g, it_p

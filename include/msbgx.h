@@ -117,7 +117,10 @@ int msbgx_prepare_pressure_solve_scene(msbgx_grid *g, const msbgx_scene *scene);
  * |r|/|b| in *rel_residual. */
 int msbgx_solve_pressure(msbgx_grid *g, const msbgx_solve_params *p, int *iters, float *rel_residual);
 /* Precision form (no reference counterpart): selects the smoother operand
- * storage (MSBGX_PREC_*); takes effect immediately, also after prepare. */
+ * storage (MSBGX_PREC_*); takes effect immediately, also after prepare.
+ * With FP16/BF16 operands the V-cycle's coarsest "solve" (n_coarse relax
+ * iterations, src/msbg3.cpp:2423-2430) runs in FP64 (x and arithmetic in
+ * double over the FP32 coefficients) for coarsest block edges up to 16. */
 int msbgx_set_precision(msbgx_grid *g, int prec);
 
 /* Mixed-precision iterative refinement around the MG-PCG (no reference
@@ -253,7 +256,7 @@ double msbgx_net_last_train_ms(const msbgx_net *n);
 int msbgx_net_apply(msbgx_net *n, msbgx_grid *g, int ch_in, int ch_out);
 /* Measurement: mean duration of one forward 16 -> 16 MFMA layer launch over
  * `blocks` patches (HIP events, `reps` launches) and its algorithmic FLOP
- * (blocks * 18^3 * 27 * 16 * 16 * 2). */
+ * (blocks * 16^3 cells * 27 * 16 * 16 * 2). */
 int msbgx_net_time_conv(msbgx_net *n, int blocks, int reps, double *ms_per_launch, double *flop_per_launch);
 /* MG-PCG with the learned correction (flexible beta).  A non-finite network
  * output, a corrected direction with r.z <= 0, or a stalled residual (see

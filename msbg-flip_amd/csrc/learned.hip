@@ -1408,7 +1408,9 @@ double netLastTrainMs(const Net &n) { return n.lastTrainMs; }
 /* Measurement: `reps` launches of the forward 16 -> 16 MFMA layer over `nb`
  * patches (the network's current weights, zero activations) between HIP
  * events on the network's stream; algorithmic FLOP per launch counts the
- * 18^3 patch positions: nb * 5832 * 27 * 16 * 16 * 2. */
+ * 16^3 cells of each patch, 27 taps x 16 x 16 channels x 2:
+ * nb * 4096 * 27 * 16 * 16 * 2 (the layer computes the 18^3 positions a
+ * following layer's halo needs, 1.42x that). */
 int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLaunch)
 {
   NetBuffers b;
@@ -1432,7 +1434,7 @@ int netTimeConv(Net &n, int nb, int reps, double *msPerLaunch, double *flopPerLa
   (void)hipEventDestroy(e1);
   freeBuffers(b);
   *msPerLaunch = ms / reps;
-  *flopPerLaunch = (double)nb * 5832.0 * nn::T * nn::C * nn::C * 2.0;
+  *flopPerLaunch = (double)nb * 4096.0 * nn::T * nn::C * nn::C * 2.0;
   return MSBGX_OK;
 }
 

@@ -340,6 +340,110 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax(LevelArgs a, const int *__
   }
 }
 
+/* Precision form, coarsest MG level: the reference's coarsest "solve", nCoarse
+ * relax iterations (src/msbg3.cpp:2423-2430), in FP64 -- x and the arithmetic
+ * in double, b, W and D read as FP32 -- with k_relax's schedule, SIMD-segment
+ * semantics and local sweeps.  At the coarsest MG level every block has the
+ * coarsest level, so a halo face holds a same-level neighbour's h*x or, outside
+ * the domain, zero, and a +face weight is the neighbour's (bcW outside). */
+template <int B>
+__global__ void __launch_bounds__(Cfg<B>::NT) k_relax64(LevelArgs a, const int *__restrict__ blocks,
+                                                        double *__restrict__ X, const float *__restrict__ Bv,
+                                                        float omega, int reverse, int nIter, const int *done)
+{
+  if (*done) return;
+  constexpr int S = Cfg<B>::S, T = Cfg<B>::T, NIN = Cfg<B>::NIN, NF = Cfg<B>::NF;
+  __shared__ double tile[NIN + 6 * NF];
+  const int bid = blocks[xcdListIndex()];
+  const int lev = a.level[bid];
+  const int64_t off = a.off[bid];
+  const double h = (double)(1 << lev);
+  const int x = threadIdx.x % B, y = threadIdx.x / B;
+  uint32_t skipmask = 0;
+  for (int z = 0; z < B; z++) {
+    const int v = x + B * y + B * B * z;
+    const bool skip = segmentAllNonFluid<S>((a.flags[off + v] & (CELL_SOLID | CELL_VOID)) != 0);
+    skipmask |= (skip ? 1u : 0u) << z;
+    tile[v] = skip ? 0.0 : X[off + v] * h;
+  }
+  bool same[6];
+  int64_t foff[6];
+#pragma unroll
+  for (int face = 0; face < 6; face++) {
+    const int dir = face >> 1, side = face & 1;
+    const int l2 = a.nbd[bid].lev[face];
+    same[face] = l2 == lev;
+    foff[face] = same[face] ? a.nbd[bid].off[face] : 0;
+    const int e = side ? 0 : B - 1;
+    const int src = dir == 0 ? e + B * x + B * B * y : (dir == 1 ? x + B * e + B * B * y : x + B * y + B * B * e);
+    tile[NIN + face * NF + x + B * y] = same[face] ? X[foff[face] + src] * h : 0.0;
+  }
+  __syncthreads();
+  if (a.omegaOf) omega = a.omegaOf[a.probOf[bid]];
+  const double om = (double)omega, omm = 1.0 - om;
+  const int o = x / S + y;
+  auto stencil = [&](int z, double &F0, double &Sm, double &D) {
+    const int v = x + B * y + B * B * z;
+    F0 = tile[v];
+    const double F1 = x > 0 ? tile[v - 1] : tile[NIN + 0 * NF + y + B * z];
+    const double F2 = x < B - 1 ? tile[v + 1] : tile[NIN + 1 * NF + y + B * z];
+    const double F3 = y > 0 ? tile[v - B] : tile[NIN + 2 * NF + x + B * z];
+    const double F4 = y < B - 1 ? tile[v + B] : tile[NIN + 3 * NF + x + B * z];
+    const double F5 = z > 0 ? tile[v - B * B] : tile[NIN + 4 * NF + x + B * y];
+    const double F6 = z < B - 1 ? tile[v + B * B] : tile[NIN + 5 * NF + x + B * y];
+    const int64_t c = off + v;
+    const double W1 = a.W0[c], W3 = a.W1[c], W5 = a.W2[c];
+    const double W2 = x < B - 1 ? a.W0[c + 1] : (same[1] ? a.W0[foff[1] + B * y + B * B * z] : a.bcW[0]);
+    const double W4 = y < B - 1 ? a.W1[c + B] : (same[3] ? a.W1[foff[3] + x + B * B * z] : a.bcW[1]);
+    const double W6 = z < B - 1 ? a.W2[c + B * B] : (same[5] ? a.W2[foff[5] + x + B * y] : a.bcW[2]);
+    double sm = F1 * W1;
+    sm = fma(F2, W2, sm);
+    sm = fma(F3, W3, sm);
+    sm = fma(F4, W4, sm);
+    sm = fma(F5, W5, sm);
+    sm = fma(F6, W6, sm);
+    Sm = sm;
+    D = a.D[c];
+  };
+  for (int k = 0; k < nIter - 1; k++) {
+    const bool rev = (reverse != 0) == ((k & 1) != 0);
+    for (int t = 0; t < T; t++) {
+      const int hp = rev ? T - 1 - t : t;
+      const int z = hp - o;
+      if (z >= 0 && z < B && !((skipmask >> z) & 1u)) {
+        double F0, Sm, D;
+        stencil(z, F0, Sm, D);
+        const double bb = Bv[off + x + B * y + B * B * z];
+        tile[x + B * y + B * B * z] = D != 0.0 ? fma(om * (bb + Sm), D, omm * F0) : F0;
+      }
+      __syncthreads();
+    }
+  }
+  const double HR = 1.0 / h;
+  for (int z = 0; z < B; z++) {
+    if ((skipmask >> z) & 1u) continue;
+    double F0, Sm, D;
+    stencil(z, F0, Sm, D);
+    const int64_t c = off + x + B * y + B * B * z;
+    const double Y = D != 0.0 ? fma(om * ((double)Bv[c] + Sm), D, omm * F0) : F0;
+    X[c] = Y * HR;
+  }
+}
+
+__global__ void k_fill64(double *__restrict__ x, int64_t n, const int *done)
+{
+  if (*done) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+__global__ void k_x64_to_f32(const double *__restrict__ x64, float *__restrict__ x, int64_t n, const int *done)
+{
+  if (*done) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = (float)x64[i];
+}
+
 /* The halo faces of stageFaces, into registers, for a block whose six face
  * neighbours all have its own level or lie outside the domain (every block
  * of a single-level grid, the interior of a level): the six neighbour levels
@@ -2086,6 +2190,9 @@ struct Dev {
   GraphKey graphKey;
   /* iterative refinement (precision form) */
   double *x64 = nullptr;
+  /* precision form: x of the coarsest MG level in FP64 (its pool's cells) */
+  double *coarse64 = nullptr;
+  int64_t coarse64Cap = 0;
   float *b0 = nullptr;
   int *dActive0 = nullptr;
   int64_t refCap = 0;
@@ -2175,6 +2282,7 @@ void deviceFree(Grid &g)
     if (d.hist) (void)hipFree(d.hist);
     if (d.zeroFlag) (void)hipFree(d.zeroFlag);
     if (d.x64) (void)hipFree(d.x64);
+    if (d.coarse64) (void)hipFree(d.coarse64);
     if (d.b0) (void)hipFree(d.b0);
     if (d.dActive0) (void)hipFree(d.dActive0);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
@@ -2457,6 +2565,27 @@ static void relaxLaunch(const Grid &g, int L, bool reverse, int nIter, float *X,
     }
 }
 
+/* The coarsest level's relax in FP64 (precision form): nIter outer
+ * iterations of two colour passes over the relaxation lists, x in X64. */
+static void relax64Launch(const Grid &g, int L, int nIter, double *X64, const float *Bv, const int *done,
+                          hipStream_t s)
+{
+  const MgLevel &m = g.mg[L];
+  if (!m.anyRelax) return;
+  const LevelArgs a = levelArgs(g, L);
+  for (int it = 0; it < nIter; it++)
+    for (int pass = 0; pass < 2; pass++) {
+      const DevList &l = m.dRelax[1 - pass][L];
+      if (l.n == 0) continue;
+      switch (g.bsx0 >> L) {
+        case 2: hipLaunchKernelGGL(k_relax64<2>, dim3(l.n), dim3(Cfg<2>::NT), 0, s, a, l.ptr, X64, Bv, g.omega, 0, 5, done); break;
+        case 4: hipLaunchKernelGGL(k_relax64<4>, dim3(l.n), dim3(Cfg<4>::NT), 0, s, a, l.ptr, X64, Bv, g.omega, 0, 5, done); break;
+        case 8: hipLaunchKernelGGL(k_relax64<8>, dim3(l.n), dim3(Cfg<8>::NT), 0, s, a, l.ptr, X64, Bv, g.omega, 0, 5, done); break;
+        case 16: hipLaunchKernelGGL(k_relax64<16>, dim3(l.n), dim3(Cfg<16>::NT), 0, s, a, l.ptr, X64, Bv, g.omega, 0, 5, done); break;
+      }
+    }
+}
+
 template <int B>
 static void launchMultiplyB(const Grid &g, int L, const DevList &l, const float *X, const float *Bv, float *Y, int residual,
                             double *partials, int base, const int *done, hipStream_t s)
@@ -2537,6 +2666,14 @@ static void vcycleLaunch(Grid &g, int L, int nPre, int nPost, int nCoarse, const
   MgLevel &m = g.mg[L];
   float *X = m.d_ch[C_Z], *Bv = m.d_ch[C_R], *R = m.d_ch[C_Q];
   if (L >= g.nLevels - 1) {
+    Dev &d = dev(g);
+    if (g.prec != MSBGX_PREC_FP32 && d.coarse64 && (g.bsx0 >> L) <= 16) {
+      /* precision form: the coarsest solve in FP64, from x = 0 as the zeroed channel */
+      hipLaunchKernelGGL(k_fill64, dim3(256), dim3(256), 0, s, d.coarse64, m.ncells, done);
+      relax64Launch(g, L, nCoarse, d.coarse64, Bv, done, s);
+      hipLaunchKernelGGL(k_x64_to_f32, dim3(256), dim3(256), 0, s, d.coarse64, X, m.ncells, done);
+      return;
+    }
     relaxLaunch(g, L, false, nCoarse, X, Bv, done, s);
     return;
   }
@@ -2894,6 +3031,16 @@ int deviceSetPrecision(Grid &g, int prec)
       if (!m.d_H[k])
         if (int rc = dalloc(&m.d_H[k], (size_t)m.ncells)) return rc;
       hipLaunchKernelGGL(k_to16, dim3(1024), dim3(256), 0, d.stream, src[k], m.d_H[k], m.ncells, prec);
+    }
+  }
+  if (prec != MSBGX_PREC_FP32) {
+    const int64_t nc = g.mg[g.nLevels - 1].ncells;
+    if (d.coarse64Cap < nc) {
+      if (d.coarse64) (void)hipFree(d.coarse64);
+      d.coarse64 = nullptr;
+      d.coarse64Cap = 0;
+      if (int rc = dalloc(&d.coarse64, (size_t)std::max<int64_t>(nc, 1))) return rc;
+      d.coarse64Cap = nc;
     }
   }
   HIPCHK(hipGetLastError());

@@ -186,6 +186,8 @@ struct Oracle {
   std::vector<CellFlags> zeroBlkF;
   float omegaDefault = 6.f / 7.f, omega = 6.f / 7.f;
   int smPrec = 0;   // precision form: 16-bit smoother operands (0 FP32, 1 FP16, 2 BF16)
+  /* precision form: x of the coarsest MG level's blocks in FP64 during its relax */
+  std::vector<std::vector<double>> x64;
   int nBlockIter = 5;
   int nThreads = 1;
 
@@ -827,6 +829,140 @@ struct Oracle {
       }
   }
 
+  /* Precision form: one block of the coarsest MG level's relax in FP64 --
+   * processBlock's OP_RELAX path with x and the arithmetic in double and b, W,
+   * D read as FP32.  At the coarsest MG level every block has the coarsest
+   * level, so a halo face is a same-level neighbour's h*x (zero outside the
+   * domain) and a +face weight the neighbour's (0 or 1 outside, by the
+   * boundary kind). */
+  void processBlock64(bool reverse, int L, int bid, int chB, int nIter)
+  {
+    const BlockInfo &bi = binfo[L][bid];
+    const int level = bi.level, b = bsxL(level), b2 = b * b, nv = b2 * b, S = simdWidth(b);
+    const double h = (double)(1 << level);
+    const int hs = b + 2, hs2 = hs * hs;
+    auto H = [&](int x, int y, int z) { return (x + 1) + hs * (y + 1) + hs2 * (z + 1); };
+    std::vector<double> X((size_t)hs * hs * hs, 0.0);
+    std::vector<double> Wt[6];
+    for (int k = 0; k < 6; k++) Wt[k].assign(nv, 0.0);
+    std::vector<uint8_t> skipSeg(nv / S, 0);
+    const CellFlags *F = flags[level][L].get(bid);
+    const float *dataB = fch[chB][level][L].get(bid);
+    const float *dataD = fch[CH_DIAG][level][L].get(bid);
+    const float *Wd[3] = {fch[CH_W0][level][L].get(bid), fch[CH_W1][level][L].get(bid), fch[CH_W2][level][L].get(bid)};
+    std::vector<double> &xb = x64[bid];
+    for (int vz = 0, iS = 0; vz < b; vz++)
+      for (int vy = 0; vy < b; vy++)
+        for (int vx = 0; vx < b; vx += S, iS++) {
+          const int v0 = vx + b * vy + b2 * vz;
+          bool allNonFluid = true;
+          for (int k = 0; k < S; k++)
+            if (!(F[v0 + k] & (CELL_SOLID | CELL_VOID))) allNonFluid = false;
+          skipSeg[iS] = allNonFluid;
+          for (int k = 0; k < S; k++) {
+            const int v = v0 + k;
+            X[H(vx + k, vy, vz)] = allNonFluid ? 0.0 : xb[v] * h;
+            Wt[0][v] = Wd[0][v];
+            if (vx + k < b - 1) Wt[1][v] = Wd[0][v + 1];
+            Wt[2][v] = Wd[1][v];
+            if (vy < b - 1) Wt[3][v] = Wd[1][v + b];
+            Wt[4][v] = Wd[2][v];
+            if (vz < b - 1) Wt[5][v] = Wd[2][v + b2];
+          }
+        }
+    int bx, by, bz;
+    blockCoords(bid, bx, by, bz);
+    static const int offs[3][2][3] = {{{-1, 0, 0}, {1, 0, 0}}, {{0, -1, 0}, {0, 1, 0}}, {{0, 0, -1}, {0, 0, 1}}};
+    for (int iDir = 0; iDir < 3; iDir++)
+      for (int iFace = 0; iFace < 2; iFace++) {
+        const int *o = offs[iDir][iFace];
+        const int bx2 = bx + o[0], by2 = by + o[1], bz2 = bz + o[2];
+        const bool inside = blockInRange(bx2, by2, bz2);
+        const int bid2 = inside ? blockIndex(bx2, by2, bz2) : -1;
+        const float *fa2 = inside ? fch[CH_W0 + iDir][level][L].get(bid2) : nullptr;
+        const float wOut = domainBc[iDir][iFace] != DBC_OPEN ? 0.0f : 1.0f;
+        for (int u = 0; u < b; u++)
+          for (int v = 0; v < b; v++) {
+            int src, hidx, wcell;
+            if (iDir == 0) {
+              src = (iFace ? 0 : b - 1) + b * u + b2 * v;
+              hidx = H(iFace ? b : -1, u, v);
+              wcell = (b - 1) + b * u + b2 * v;
+            } else if (iDir == 1) {
+              src = u + b * (iFace ? 0 : b - 1) + b2 * v;
+              hidx = H(u, iFace ? b : -1, v);
+              wcell = u + b * (b - 1) + b2 * v;
+            } else {
+              src = u + b * v + b2 * (iFace ? 0 : b - 1);
+              hidx = H(u, v, iFace ? b : -1);
+              wcell = u + b * v + b2 * (b - 1);
+            }
+            X[hidx] = inside ? x64[bid2][src] * h : 0.0;
+            if (iFace) Wt[2 * iDir + 1][wcell] = inside ? (fa2 ? fa2[src] : 0.0f) : wOut;
+          }
+      }
+    const double om = (double)omega, omm = 1.0 - om, HR = 1.0 / h;
+    const int nSeg = b / S;
+    double Y[8];
+    for (int it = 0; it < nIter; it++) {
+      const bool rev = reverse == ((it & 1) != 0);
+      for (int zz = 0; zz < b; zz++)
+        for (int yy = 0; yy < b; yy++)
+          for (int ss = 0; ss < nSeg; ss++) {
+            const int vz = rev ? b - 1 - zz : zz, vy = rev ? b - 1 - yy : yy, sg = rev ? nSeg - 1 - ss : ss;
+            const int vx0 = sg * S, v0 = vx0 + b * vy + b2 * vz;
+            if (skipSeg[v0 / S]) continue;
+            for (int k = 0; k < S; k++) {
+              const int v = v0 + k, x = vx0 + k;
+              const double F0 = X[H(x, vy, vz)];
+              double Sm = X[H(x - 1, vy, vz)] * Wt[0][v];
+              Sm = std::fma(X[H(x + 1, vy, vz)], Wt[1][v], Sm);
+              Sm = std::fma(X[H(x, vy - 1, vz)], Wt[2][v], Sm);
+              Sm = std::fma(X[H(x, vy + 1, vz)], Wt[3][v], Sm);
+              Sm = std::fma(X[H(x, vy, vz - 1)], Wt[4][v], Sm);
+              Sm = std::fma(X[H(x, vy, vz + 1)], Wt[5][v], Sm);
+              const double D = dataD[v];
+              const double B = dataB ? (double)dataB[v] : 0.0;
+              Y[k] = D != 0.0 ? std::fma(om * (B + Sm), D, omm * F0) : F0;
+            }
+            for (int k = 0; k < S; k++) {
+              if (it < nIter - 1) X[H(vx0 + k, vy, vz)] = Y[k];
+              else xb[v0 + k] = Y[k] * HR;
+            }
+          }
+    }
+  }
+
+  /* Precision form: the coarsest MG level's "solve" (nCoarse relax
+   * iterations, src/msbg3.cpp:2423-2430) in FP64 from x = 0; chX receives x
+   * rounded to FP32 on the relaxed blocks. */
+  void relaxCoarse64(int L, int chX, int chB, int nIter)
+  {
+    x64.assign(nBlocks, std::vector<double>());
+    for (int bid = 0; bid < nBlocks; bid++) x64[bid].assign((size_t)bsxL(binfo[L][bid].level) * bsxL(binfo[L][bid].level) *
+                                                                  bsxL(binfo[L][bid].level), 0.0);
+    const std::vector<int> &list = relaxList[L];
+    for (int it = 0; it < nIter; it++)
+      for (int pass = 0; pass < 2; pass++) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nThreads)
+        for (int i = 0; i < (int)list.size(); i++) {
+          const int bid = list[i];
+          const BlockInfo &bi = binfo[L][bid];
+          if (bi.flags & BLK_FIXED) continue;
+          int bx, by, bz;
+          blockCoords(bid, bx, by, bz);
+          if (((bx + by + bz) & 1) == pass) continue;
+          processBlock64(false, L, bid, chB, nBlockIter);
+        }
+      }
+    for (int bid : list) {
+      const BlockInfo &bi = binfo[L][bid];
+      float *dst = fch[chX][bi.level][L].alloc(bid);
+      const int nv = bsxL(bi.level) * bsxL(bi.level) * bsxL(bi.level);
+      for (int v = 0; v < nv; v++) dst[v] = (float)x64[bid][v];
+    }
+  }
+
   /* src/msbg3.cpp:2375-2399 */
   void zeroChannelInPlace(int ch, int L)
   {
@@ -920,7 +1056,11 @@ struct Oracle {
   /* src/msbg3.cpp:2412-2465 */
   void vCycle(int L, int chX, int chB, int chR, int nPre, int nPost, int nCoarse)
   {
-    if (L >= nLevels - 1) { relax(false, L, chX, chB, nCoarse); return; }
+    if (L >= nLevels - 1) {
+      if (smPrec && bsxL(L) <= 16) relaxCoarse64(L, chX, chB, nCoarse);   // precision form: FP64 coarsest solve
+      else relax(false, L, chX, chB, nCoarse);
+      return;
+    }
     relax(false, L, chX, chB, nPre);
     multiply(true, L, chX, chB, chR);
     zeroChannelInPlace(chB, L + 1);
@@ -1269,6 +1409,8 @@ int orc_set_block(void *h, int ch, int L, int bid, const float *src)
   return b;
 }
 void orc_relax(void *h, int L, int reverse, int nIter, int chX, int chB) { ((Oracle *)h)->relax(reverse != 0, L, chX, chB, nIter); }
+/* Precision form: the coarsest level's FP64 relax (from x = 0) into chX. */
+void orc_relax_coarse64(void *h, int L, int nIter, int chX, int chB) { ((Oracle *)h)->relaxCoarse64(L, chX, chB, nIter); }
 void orc_multiply(void *h, int L, int residual, int chX, int chB, int chDst) { ((Oracle *)h)->multiply(residual != 0, L, chX, chB, chDst); }
 void orc_downsample(void *h, int Lc, int chSrc, int chDst) { ((Oracle *)h)->downsample(Lc, chSrc, chDst); }
 void orc_prolongate(void *h, int Lf, int ch) { ((Oracle *)h)->prolongate(Lf, ch); }

@@ -31,6 +31,7 @@ for name, res, args in [
     ("orc_get_block", C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
     ("orc_set_block", C.c_int, [P, C.c_int, C.c_int, C.c_int, P]),
     ("orc_relax", None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("orc_relax_coarse64", None, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("orc_multiply", None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("orc_downsample", None, [P, C.c_int, C.c_int, C.c_int]),
     ("orc_prolongate", None, [P, C.c_int, C.c_int]),
@@ -118,6 +119,11 @@ class Oracle:
         """Precision form: the relax reads its coefficients rounded to 16 bits
         (0 FP32, 1 FP16, 2 BF16)."""
         _lib.orc_set_smoother_precision(self._h, prec)
+
+    def relax_coarse64(self, L, n_iter, ch_x, ch_b):
+        """Precision form: the coarsest level's relax in FP64 from x = 0
+        (x rounded to FP32 into ch_x on the relaxed blocks)."""
+        _lib.orc_relax_coarse64(self._h, L, n_iter, ch_x, ch_b)
 
     def block_info(self, mg=0):
         lv = np.zeros(self.n_blocks, np.int32)

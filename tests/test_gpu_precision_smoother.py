@@ -86,3 +86,44 @@ def test_16bit_solve_matches_oracle(name, make):
     print(f"16-bit solve {name}: gpu {it_g} it rel {rr_g:.3e}; oracle {it_o} it rel {rr_o:.3e}; "
           f"pressure rel L2 {rel:.2e}", flush=True)
     assert it_g == it_o and abs(rr_g - rr_o) <= 1e-3 * rr_o + 1e-12 and rel <= 1e-5
+
+
+@pytest.mark.parametrize("bsx0", [8, 16, 32])
+def test_16bit_vcycle_fp64_coarsest_bit_exact(bsx0):
+    """Precision form: the V-cycle's coarsest "solve" (24 relax iterations)
+    runs in FP64 on both sides.  Under the reference's coarse metadata (Q1)
+    the workloads' coarsest levels relax no block, so the grid here is built
+    for it: a 4x4x4 raster, two MG levels, block 0 at level 0 and the others
+    at level 1, OPEN walls, dense inputs -- the coarsest level relaxes the
+    blocks whose doubled-coordinate samples reach level-1 data, and its
+    right-hand side is the restricted MG 0 residual (coarsest block edges 4,
+    8 and 16).  Two V-cycles give bit-identical z at both levels."""
+    n = 4 * bsx0
+    print(f"16-bit V-cycle with FP64 coarsest level, bsx0 {bsx0}", flush=True)
+    rng = np.random.default_rng(5 + bsx0)
+    flags = np.zeros((n, n, n), np.uint16)
+    w = [rng.uniform(0.5, 1.5, (n, n, n)).astype(np.float32) for _ in range(3)]
+    rhs = rng.standard_normal((n, n, n)).astype(np.float32)
+    lv = np.ones(64, np.int32)
+    lv[0] = 0
+    g = msbgx.MultiresSparseGrid.create(n, n, n, bsx0, 2)
+    g.set_domain_boundary_spec(*([msbgx.DBC_OPEN] * 6))
+    g.set_refinement_map(lv)
+    g.prepare_pressure_solve_flip(dense=(flags, *w, rhs))
+    g.set_precision(msbgx.PREC_FP16)
+    g.set_options(msbgx.OPT_LITERAL_VCYCLE)
+    o = orc.Oracle(n, n, n, bsx0, 2)
+    o.set_domain_boundary_spec(*([msbgx.DBC_OPEN] * 6))
+    o.set_refinement_map(lv)
+    o.prepare_dense((n, n, n), flags, *w, rhs)
+    o.set_smoother_precision(msbgx.PREC_FP16)
+    assert len(g.relax_list(1)) > 0 and np.array_equal(g.relax_list(1), o.relax_list(1))
+    for side in (g, o):
+        side.copy_channel(msbgx.CH_DIVERGENCE, msbgx.CH_R)
+        side.vcycle(3, 3, 24)
+        side.vcycle(3, 3, 24)
+    for mg in (1, 0):
+        blocks, lvb = blocks_of(g, mg, "fluid")
+        nmis, md, mv = compare_channel(g, o, msbgx.CH_Z, mg, blocks, lvb)
+        assert nmis == 0 and mv > 0, f"MG {mg}: {nmis} mismatches (max diff {md})"
+    g.destroy()
