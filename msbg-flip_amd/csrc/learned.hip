@@ -609,14 +609,17 @@ __global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__
   const int bid = blocks[blockIdx.x];
   const int64_t o = off[bid];
   double acc = 0.0, cnt = 0.0;
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const bool unk = !(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f;
-    if (unknownOnly && !unk) continue;
-    acc += (double)src[o + c];
-    cnt += 1.0;
+  if (unknownOnly) {
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+      if ((flags[o + c] & (CELL_SOLID | CELL_VOID)) || D[o + c] == 0.0f) continue;
+      acc += (double)src[o + c];
+      cnt += 1.0;
+    }
+  } else {   // every cell: the flags and D are not read
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) acc += (double)src[o + c];
   }
   const double sum = blockReduceSum(acc, red);
-  const double num = blockReduceSum(cnt, red);
+  const double num = unknownOnly ? blockReduceSum(cnt, red) : (double)ncell;
   if (threadIdx.x == 0) coarse[bid] = num > 0.0 ? (float)(sum / num) : 0.0f;
 }
 
@@ -813,6 +816,112 @@ struct DirSet {
 };
 constexpr int kGramBlocks = 512;
 
+/* The scales' coarse grids, PCG-block weights and outputs for one pass of
+ * k_nn_prolong_lin_b. */
+struct ProlongScales {
+  const float *coarse[kMaxScales];
+  const float *wgt[kMaxScales];
+  float *z[kMaxScales];
+  int f[kMaxScales], sx[kMaxScales], sy[kMaxScales], sz[kMaxScales];
+};
+
+/* k_nn_prolong_lin for NS scales at once on B^3 blocks (B = 16, 32), so the
+ * flags and D are read once for all of them: thread t takes the (x, y)
+ * columns t, t + 256, ... and walks z.  The interpolation factors are per
+ * axis, so per column and scale the x/y corners of each of the three z
+ * layers of the neighbourhood are summed once (masked weight and weighted
+ * value); a cell then blends the two layers around it and divides.  The same
+ * weights and mask as k_nn_prolong_lin, summed in another grouping (the
+ * results agree to rounding). */
+template <int B, int NS>
+__global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int nbx, int nby,
+                                                          const int *__restrict__ blocks,
+                                                          const int64_t *__restrict__ off,
+                                                          const uint16_t *__restrict__ flags,
+                                                          const float *__restrict__ D, PcgState *st)
+{
+  static_assert(B * B % 256 == 0, "whole columns per thread");
+  if (st && st->skipNet) return;
+  __shared__ float cv[NS][27], cm[NS][27];
+  const int bid = blocks[blockIdx.x];
+  const int b3[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
+  if (threadIdx.x < 27 * NS) {
+    const int k = threadIdx.x / 27, j = threadIdx.x % 27, f = ps.f[k];
+    const int nx = b3[0] / f + j % 3 - 1, ny = b3[1] / f + (j / 3) % 3 - 1, nz = b3[2] / f + j / 9 - 1;
+    float v = 0.0f, m = 0.0f;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < ps.sx[k] && ny < ps.sy[k] && nz < ps.sz[k]) {
+      const int nc = nx + ps.sx[k] * (ny + ps.sy[k] * nz);
+      if (ps.wgt[k][nc] > 0.0f) {
+        v = ps.coarse[k][nc];
+        m = 1.0f;
+        if (!isfinite(v)) {
+          if (st) st->bad = 1;
+          v = 0.0f;
+        }
+      }
+    }
+    cv[k][j] = v;
+    cm[k][j] = m;
+  }
+  __syncthreads();
+  const int64_t o = off[bid];
+  /* per scale: offset of the block inside its scale cell along each axis,
+   * and 1 / (f B) */
+  int base[NS][3];
+  float inv[NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const int f = ps.f[k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) base[k][a] = (b3[a] - f * (b3[a] / f)) * B;
+    inv[k] = 1.0f / (float)(f * B);
+  }
+  auto axis = [&](int k, int a, int v, int &lo, float &t) {
+    const float u = ((float)(base[k][a] + v) + 0.5f) * inv[k] - 0.5f;
+    lo = u < 0.0f ? 0 : 1;
+    t = u < 0.0f ? u + 1.0f : u;
+  };
+  for (int col = threadIdx.x; col < B * B; col += 256) {
+    const int x = col % B, y = col / B;
+    float SW[NS][3], SV[NS][3];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      int lx, ly;
+      float tx, ty;
+      axis(k, 0, x, lx, tx);
+      axis(k, 1, y, ly, ty);
+#pragma unroll
+      for (int iz = 0; iz < 3; iz++) {
+        float sw = 0.0f, sv = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const float w = ((e & 1) ? tx : 1.0f - tx) * ((e & 2) ? ty : 1.0f - ty);
+          const int q = lx + (e & 1) + 3 * (ly + ((e >> 1) & 1)) + 9 * iz;
+          sw = fmaf(w, cm[k][q], sw);
+          sv = fmaf(w * cm[k][q], cv[k][q], sv);
+        }
+        SW[k][iz] = sw;
+        SV[k][iz] = sv;
+      }
+    }
+#pragma unroll 4
+    for (int z = 0; z < B; z++) {
+      const int64_t c = o + col + (int64_t)B * B * z;
+      const bool unk = !(flags[c] & (CELL_SOLID | CELL_VOID)) && D[c] != 0.0f;
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        int lz;
+        float tz;
+        axis(k, 2, z, lz, tz);
+        const float w0 = SW[k][0], w1 = SW[k][1], w2 = SW[k][2], v0 = SV[k][0], v1 = SV[k][1], v2 = SV[k][2];
+        const float swl = lz ? w1 : w0, swh = lz ? w2 : w1, svl = lz ? v1 : v0, svh = lz ? v2 : v1;
+        const float sw = fmaf(tz, swh, (1.0f - tz) * swl), sv = fmaf(tz, svh, (1.0f - tz) * svl);
+        ps.z[k][c] = unk ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
+      }
+    }
+  }
+}
+
 /* Partial Galerkin sums over [0, n) in FP64, unless *skip: for a <= b,
  * <z_a, A z_b> = <z_a, r - q_b>, then <z_a, r>; value j of workgroup w at
  * partials[j * gridDim.x + w]. */
@@ -948,6 +1057,28 @@ __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, in
   if (skip && *skip) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     z[i] += zc[i];
+}
+
+/* z += zc unless *skip, then (unless *done) the partial <r, z> of the
+ * result, its products summed in k_dot's order */
+__global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ z, const float *__restrict__ zc,
+                                                            const float *__restrict__ r, int64_t n, const int *skip,
+                                                            const int *done, double *partials)
+{
+  if (*done) return;
+  __shared__ double red[kVecThreads / 64];
+  const bool add = !*skip;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float zv = z[i];
+    if (add) {
+      zv += zc[i];
+      z[i] = zv;
+    }
+    acc += (double)r[i] * (double)zv;
+  }
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 /* Scale the correction by the Galerkin step gamma over the whole MG0 pool:
@@ -1545,7 +1676,9 @@ static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
   if (!c.zc) {
     if (int rc = dalloc(&c.zc, nc)) return rc;
     if (int rc = dalloc(&c.rs, nc)) return rc;
-    if (int rc = dalloc(&c.gram, (size_t)(kMaxScales * (kMaxScales + 3) / 2) * kGramBlocks)) return rc;
+    /* Galerkin partials: per workgroup of k_nn_gram, or per level-0 block of the fused residual pass */
+    const size_t nPart = (size_t)std::max(kGramBlocks, g.nActive0);
+    if (int rc = dalloc(&c.gram, (size_t)(kMaxScales * (kMaxScales + 3) / 2) * nPart)) return rc;
     HIPCHK(hipMemset(c.zc, 0, nc * sizeof(float)));
     HIPCHK(hipMemset(c.rs, 0, nc * sizeof(float)));
   }
@@ -1626,7 +1759,38 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
     netForward123(n, c.ib.X0, nullptr, c.ib, nt, skip, s);
     hipLaunchKernelGGL(k_nn_out_dense, dim3(nt, 8), dim3(256), 0, s, c.ib.A3, n.w4f, q.sx, q.sy, q.sz, q.ntx, q.nty,
                        q.cout, skip);
-    if (n.prolong == MSBGX_NET_PROLONG_LINEAR)
+  }
+  const bool linear = n.prolong == MSBGX_NET_PROLONG_LINEAR;
+  if (linear && (g.bsx0 == 16 || g.bsx0 == 32)) {   // every scale in one pass
+    ProlongScales ps{};
+    for (int k = 0; k < n.scales; k++) {
+      const LearnedCtx::Scale &q = c.sc[k];
+      ps.coarse[k] = q.cout;
+      ps.wgt[k] = q.w;
+      ps.z[k] = q.z;
+      ps.f[k] = q.f;
+      ps.sx[k] = q.sx;
+      ps.sy[k] = q.sy;
+      ps.sz[k] = q.sz;
+    }
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(g.nActive0), dim3(256), 0, s, ps, g.nbx, g.nby, d.dActive0, m0.d_off, m0.d_flags,
+                         m0.d_D, st);
+    };
+    const int sel = (g.bsx0 == 32 ? 3 : 0) + n.scales - 1;
+    switch (sel) {
+      case 0: launch(k_nn_prolong_lin_b<16, 1>); break;
+      case 1: launch(k_nn_prolong_lin_b<16, 2>); break;
+      case 2: launch(k_nn_prolong_lin_b<16, 3>); break;
+      case 3: launch(k_nn_prolong_lin_b<32, 1>); break;
+      case 4: launch(k_nn_prolong_lin_b<32, 2>); break;
+      default: launch(k_nn_prolong_lin_b<32, 3>); break;
+    }
+    return;
+  }
+  for (int k = 0; k < n.scales; k++) {
+    LearnedCtx::Scale &q = c.sc[k];
+    if (linear)
       hipLaunchKernelGGL(k_nn_prolong_lin, dim3(g.nActive0), dim3(256), 0, s, q.cout, q.w, q.f, g.nbx, g.nby, q.sx,
                          q.sy, q.sz, d.dActive0, m0.d_off, m0.d_flags, m0.d_D, g.bsx0, q.z, st);
     else
@@ -1635,8 +1799,11 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
   }
 }
 
-/* Galerkin combination of K > 1 directions: zc and rs = r - A zc with the
- * coefficients that minimise the A-norm error over their span. */
+/* Galerkin combination of K > 1 directions: their residuals q_a = r - A z_a
+ * and the Galerkin sums (for two directions on 32^3 level-0 blocks in the
+ * same pass), then zc
+ * and rs = r - A zc with the coefficients that minimise the A-norm error
+ * over their span. */
 template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s)
 {
   Dev &d = dev(g);
@@ -1646,9 +1813,41 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
     ds.z[a] = c.sc[a].z;
     ds.q[a] = c.sc[a].q;
   }
-  hipLaunchKernelGGL(k_nn_gram<K>, dim3(kGramBlocks), dim3(256), 0, s, ds, m0.d_ch[C_R], c.n0, c.gram,
-                     &d.state->skipNet);
-  hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, kGramBlocks, d.state);
+  const int *skip = &d.state->skipNet;
+  const DevList &l0 = m0.dAct[0];
+  bool fused = false;
+  if constexpr (K == 2) {   // three directions' state does not fit the registers of 1024 threads
+    fused = g.bsx0 == 32 && l0.n == g.nActive0;
+    if (fused) {
+      /* level-0 blocks: both directions' residuals and the Galerkin partials
+       * in one pass; the other MG0 blocks one direction at a time */
+      MulVecs<K> mv{};
+      for (int a = 0; a < K; a++) {
+        mv.x[a] = c.sc[a].z;
+        mv.y[a] = c.sc[a].q;
+      }
+      if (l0.n)
+        hipLaunchKernelGGL((k_multiply_z<32, K>), dim3(l0.n), dim3(32 * 32), 0, s, levelArgs(g, 0), l0.ptr, mv,
+                           m0.d_ch[C_R], 1, c.gram, 0, skip);
+      for (int a = 0; a < K; a++)
+        for (int lev = 1; lev < g.nLevels; lev++) {
+          const DevList &l = m0.dAct[lev];
+          switch (g.bsx0 >> lev) {
+            case 2: launchMultiplyB<2>(g, 0, l, c.sc[a].z, m0.d_ch[C_R], c.sc[a].q, 1, nullptr, 0, skip, s); break;
+            case 4: launchMultiplyB<4>(g, 0, l, c.sc[a].z, m0.d_ch[C_R], c.sc[a].q, 1, nullptr, 0, skip, s); break;
+            case 8: launchMultiplyB<8>(g, 0, l, c.sc[a].z, m0.d_ch[C_R], c.sc[a].q, 1, nullptr, 0, skip, s); break;
+            case 16: launchMultiplyB<16>(g, 0, l, c.sc[a].z, m0.d_ch[C_R], c.sc[a].q, 1, nullptr, 0, skip, s); break;
+          }
+        }
+      hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, std::max(l0.n, 1), d.state);
+    }
+  }
+  if (!fused) {
+    for (int a = 0; a < K; a++)
+      multiplyLaunch(g, 0, true, c.sc[a].z, m0.d_ch[C_R], c.sc[a].q, nullptr, skip, s);
+    hipLaunchKernelGGL(k_nn_gram<K>, dim3(kGramBlocks), dim3(256), 0, s, ds, m0.d_ch[C_R], c.n0, c.gram, skip);
+    hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, kGramBlocks, d.state);
+  }
   hipLaunchKernelGGL(k_nn_combine<K>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.zc, c.rs, m0.d_ch[C_R],
                      (int64_t)m0.ncells, d.state);
 }
@@ -1656,16 +1855,15 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
 /* Coarse form, learned path (skipped while skipNet): directions
  * z_f = P_f N(R_f r) at the net's scales, zc their Galerkin combination
  * (one scale: the step gamma = <zc, r> / <zc, A zc>); then
- * z = zc + V(r - A zc). */
+ * z = zc + V(r - A zc), and the partials of <r, z> for the next finalize. */
 static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p, hipStream_t s)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *skip = &d.state->skipNet;
   coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s);
-  for (int k = 0; k < n.scales; k++)
-    multiplyLaunch(g, 0, true, c.sc[k].z, m0.d_ch[C_R], c.sc[k].q, nullptr, skip, s);
   if (n.scales == 1) {
+    multiplyLaunch(g, 0, true, c.sc[0].z, m0.d_ch[C_R], c.sc[0].q, nullptr, skip, s);
     /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
     hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, c.zc, m0.d_ch[C_R], c.n0, d.partials, skip);
     finalize(d, kVecBlocks, FIN_ZCR, nullptr);
@@ -1683,7 +1881,8 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   zeroActive(g, 0, m0.d_ch[C_Z], &d.state->done, s);
   vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->done, s);
   std::swap(m0.d_ch[C_R], c.rs);
-  hipLaunchKernelGGL(k_nn_add, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], c.zc, c.n0, skip);
+  hipLaunchKernelGGL(k_nn_add_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], c.zc, m0.d_ch[C_R], c.n0,
+                     skip, &d.state->done, d.partials);
 }
 
 static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p)
@@ -1691,20 +1890,15 @@ static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbg
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *done = &d.state->done;
-  const int *skip = &d.state->skipNet;
   const int64_t n0 = c.n0;
   const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
   finalize(d, nPart, FIN_ALPHA, nullptr);
-  hipLaunchKernelGGL(k_update_xr, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
-                     m0.d_ch[C_D], m0.d_ch[C_Q], n0, d.state, d.partials);
+  /* <r_k+1, z_k> of the flexible beta comes with the update, before z_k is replaced */
+  hipLaunchKernelGGL(k_update_xr_rz, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                     m0.d_ch[C_D], m0.d_ch[C_Q], m0.d_ch[C_Z], n0, d.state, d.partials);
   finalize(d, kVecBlocks, FIN_CONV, d.hist);
-  hipLaunchKernelGGL(k_nn_copy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_Z], c.zprev, n0, skip, 0);
+  finalizeAt(d, kVecBlocks, kVecBlocks, FIN_RZOLD, nullptr);
   coarseLearnedPath(g, n, c, p, d.stream);   // its V-cycle is the plain one while the network is off
-  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zprev, n0, d.partials,
-                     skip);
-  finalize(d, kVecBlocks, FIN_RZOLD, nullptr);
-  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
-                     done);
   finalize(d, kVecBlocks, FIN_BETA, nullptr);
   /* a rejected corrected z restarts along r */
   hipLaunchKernelGGL(k_nn_axpy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z],
@@ -1795,10 +1989,10 @@ int deviceSolveLearned(Grid &g, Net &n, const msbgx_solve_params &p, int *iters,
     hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], c.zplain, c.n0,
                        d.partials, skip);
     finalize(d, kVecBlocks, FIN_RZPLAIN, nullptr);
+    hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], c.n0,
+                       d.partials, done);
   }
-  hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], c.n0,
-                     d.partials, done);
-  finalize(d, kVecBlocks, FIN_RHO0, nullptr);
+  finalize(d, kVecBlocks, FIN_RHO0, nullptr);   // the coarse path left the partials of <r, z>
   /* d = z on every MG0 block; the PCG range takes the plain z (fine form) or
    * r (coarse form) if the corrected one was rejected */
   for (int lev = 0; lev < g.nLevels; lev++)

@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "msbgx_internal.h"
 
@@ -1658,6 +1659,14 @@ __device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int6
   return (float)(sv * (double)h);
 }
 
+/* Vectors of one k_multiply_z launch: y_v = A x_v (or r_v = b - A x_v) for
+ * v < NV, sharing the coefficient, flag and b loads. */
+template <int NV>
+struct MulVecs {
+  const float *x[NV];
+  float *y[NV];
+};
+
 /* y = A x (or r = b - A x) for B = 32: one workgroup of B*B threads per
  * block marches through the block's z-planes.  Thread (x, y) owns its
  * z-column; each step publishes the plane's h*x (0 on all-non-fluid
@@ -1666,15 +1675,20 @@ __device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int6
  * issues the next step's loads (x and flags two planes ahead, coefficients
  * and b one plane ahead, the ring's face values one plane ahead) before it
  * computes, so that they are in flight across the barrier.  Cells, face
- * values, operation order and the FP64 partial are those of k_multiply. */
-template <int B>
-__global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__restrict__ blocks,
-                                                      const float *__restrict__ X, const float *__restrict__ Bv,
-                                                      float *__restrict__ Y, int residual, double *partials,
+ * values, operation order and the FP64 partial are those of k_multiply.
+ *
+ * NV > 1 (residual form, the coarse correction's directions): NV vectors
+ * in one pass, and instead of the <x, y> partial the Galerkin partials of
+ * level-0 blocks in FP64, for a <= b <x_a, b - y_b> (= <x_a, A x_b>) then
+ * <x_a, b>, value j at partials[j * gridDim.x + list position]. */
+template <int B, int NV>
+__global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__restrict__ blocks, MulVecs<NV> mv,
+                                                      const float *__restrict__ Bv, int residual, double *partials,
                                                       int partialBase, const int *done)
 {
   constexpr int S = Cfg<B>::S, P = B + 2, P2 = P * P, BB = B * B;
-  __shared__ float pl[2 * P2];
+  constexpr int NG = NV > 1 ? NV * (NV + 1) / 2 + NV : 1;   // partial sums per workgroup
+  __shared__ float pl[NV][2 * P2];
   __shared__ double red[16];
   const int li = xcdListIndex();
   const int bid = blocks[li];
@@ -1694,7 +1708,7 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
     fkind[face] = l2 < 0 ? FK_DOMAIN : (l2 == lev ? FK_SAME : (l2 > lev ? FK_COARSER : FK_FINER));
     foff[face] = l2 < 0 ? 0 : nd.off[face];
   }
-  const float *bsrc = residual ? Bv : X;   // X stands in when b is not read
+  const float *bsrc = residual ? Bv : mv.x[0];   // x stands in when b is not read
   const int64_t q0 = off + x + B * y;
   /* ring role: threads 0 .. 4B-1 load the x/y halo of each plane, face
    * rf = t / B at face cell (ri, z) */
@@ -1709,23 +1723,26 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
   const float *rW = rdir == 0 ? a.W0 : a.W1;
   /* raw loads of the ring value at plane z: same level x; coarser x and own
    * -face coefficient */
-  auto ringLoad = [&](int z, float &r1, float &r2) {
-    r1 = 0.0f;
+  auto ringLoad = [&](int z, float (&r1)[NV], float &r2) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) r1[v] = 0.0f;
     r2 = 1.0f;
     if (rkind == FK_SAME) {
       const int e = rside ? 0 : B - 1;
       const int src = rdir == 0 ? e + B * ri + BB * z : ri + B * e + BB * z;
-      r1 = X[roff + src];
+#pragma unroll
+      for (int v = 0; v < NV; v++) r1[v] = mv.x[v][roff + src];
     } else if (rkind == FK_COARSER) {
       constexpr int Bc = B / 2;
       const int e = rside ? 0 : Bc - 1;
       const int cu = ri / 2, cw = z / 2;
       const int src = rdir == 0 ? e + Bc * cu + Bc * Bc * cw : cu + Bc * e + Bc * Bc * cw;
-      r1 = X[roff + src];
+#pragma unroll
+      for (int v = 0; v < NV; v++) r1[v] = mv.x[v][roff + src];
       if (!rside) r2 = rW[off + (rdir == 0 ? B * ri + BB * z : ri + BB * z)];
     }
   };
-  auto ringValue = [&](int z, float r1, float r2) -> float {
+  auto ringValue = [&](int v, int z, float r1, float r2) -> float {
     if (rkind == FK_SAME) return r1 * h;
     if (rkind == FK_COARSER) {
       float aa = (float)(1.0 * 2 * (1. / 3.));
@@ -1733,11 +1750,15 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
       const double sv = (double)(aa * r1);
       return (float)(sv * (double)h);
     }
-    if (rkind == FK_FINER) return faceCellValue<B>(a, h, off, X, rf, rkind, roff, ri, z);
+    if (rkind == FK_FINER) return faceCellValue<B>(a, h, off, mv.x[v], rf, rkind, roff, ri, z);
     return 0.0f;
   };
-  const float fz0 = faceCellValue<B>(a, h, off, X, 4, fkind[4], foff[4], x, y);
-  const float fz1 = faceCellValue<B>(a, h, off, X, 5, fkind[5], foff[5], x, y);
+  float fz0[NV], fz1[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    fz0[v] = faceCellValue<B>(a, h, off, mv.x[v], 4, fkind[4], foff[4], x, y);
+    fz1[v] = faceCellValue<B>(a, h, off, mv.x[v], 5, fkind[5], foff[5], x, y);
+  }
   const float w6t = a.W2[fkind[5] == FK_SAME ? foff[5] + x + B * y : q0];
   const bool m0 = x == 0 && fkind[0] >= FK_COARSER, m2 = y == 0 && fkind[2] >= FK_COARSER;
   const bool m4 = fkind[4] >= FK_COARSER;
@@ -1748,24 +1769,42 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
   auto qy = [&](int z) { return y < B - 1 ? q0 + BB * z + B : (fkind[3] == FK_SAME ? foff[3] + x + BB * z : q0 + BB * z); };
 
   /* prologue: plane 0's coefficients, x and flags of planes 0 and 1 */
-  float xr0 = X[q0], xr1 = X[q0 + BB];
+  float xr0[NV], xr1[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    xr0[v] = mv.x[v][q0];
+    xr1[v] = mv.x[v][q0 + BB];
+  }
   uint16_t fl0 = a.flags[q0], fl1 = a.flags[q0 + BB];
   float w0 = a.W0[q0], w1 = a.W1[q0], w2 = a.W2[q0], w2n = a.W2[q0 + BB], dd = a.D[q0];
   float w0x = a.W0[qx(0)], w1y = a.W1[qy(0)], bb = bsrc[q0];
-  float rr1, rr2;
+  float rr1[NV], rr2;
   ringLoad(0, rr1, rr2);
   bool sk = segmentAllNonFluid<S>((fl0 & (CELL_SOLID | CELL_VOID)) != 0);
-  float xs = sk ? 0.0f : xr0 * h;   // h*x of plane z as the stencil reads it
-  float xsPrev = 0.0f;
-  double acc = 0.0;
+  float xs[NV], xsPrev[NV];   // h*x of plane z as the stencil reads it, and of plane z-1
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    xs[v] = sk ? 0.0f : xr0[v] * h;
+    xsPrev[v] = 0.0f;
+  }
+  double acc[NG];
+#pragma unroll
+  for (int j = 0; j < NG; j++) acc[j] = 0.0;
 #pragma unroll
   for (int z = 0; z < B; z++) {
     /* next step's loads */
-    float xr2 = 0.0f, nw0 = 0.0f, nw1 = 0.0f, nw2n = 0.0f, ndd = 0.0f, nw0x = 0.0f, nw1y = 0.0f, nbb = 0.0f;
-    float nr1 = 0.0f, nr2 = 1.0f;
+    float xr2[NV], nr1[NV];
+    float nw0 = 0.0f, nw1 = 0.0f, nw2n = 0.0f, ndd = 0.0f, nw0x = 0.0f, nw1y = 0.0f, nbb = 0.0f;
+    float nr2 = 1.0f;
     uint16_t fl2 = 0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      xr2[v] = 0.0f;
+      nr1[v] = 0.0f;
+    }
     if (z + 2 < B) {
-      xr2 = X[q0 + BB * (z + 2)];
+#pragma unroll
+      for (int v = 0; v < NV; v++) xr2[v] = mv.x[v][q0 + BB * (z + 2)];
       fl2 = a.flags[q0 + BB * (z + 2)];
       nw2n = a.W2[q0 + BB * (z + 2)];
     }
@@ -1781,43 +1820,76 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
     }
     /* h*x of plane z+1 (the +z neighbour) and its skip state */
     bool skn = false;
-    float xsn = 0.0f;
+    float xsn[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) xsn[v] = 0.0f;
     if (z + 1 < B) {
       skn = segmentAllNonFluid<S>((fl1 & (CELL_SOLID | CELL_VOID)) != 0);
-      xsn = skn ? 0.0f : xr1 * h;
+#pragma unroll
+      for (int v = 0; v < NV; v++) xsn[v] = skn ? 0.0f : xr1[v] * h;
     }
-    float *p = pl + (z & 1) * P2;
-    p[(y + 1) * P + x + 1] = xs;
-    if (ring) p[rpos] = ringValue(z, rr1, rr2);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      float *p = pl[v] + (z & 1) * P2;
+      p[(y + 1) * P + x + 1] = xs[v];
+      if (ring) p[rpos] = ringValue(v, z, rr1[v], rr2);
+    }
     ldsBarrier();
     if (!sk) {
-      const float *c = p + (y + 1) * P + x + 1;
-      const float F0 = xs;
-      const float F1 = c[-1], F2 = c[1], F3 = c[-P], F4 = c[P];
-      const float F5 = z > 0 ? xsPrev : fz0;
-      const float F6 = z < B - 1 ? xsn : fz1;
       const float W1 = m0 ? 1.0f : w0;
       const float W3 = m2 ? 1.0f : w1;
       const float W5 = (z == 0 && m4) ? 1.0f : w2;
       const float W2 = xIn ? w0x : xOut;
       const float W4 = yIn ? w1y : yOut;
       const float W6 = z < B - 1 ? w2n : zOut;
-      float s = F1 * W1;
-      s = fmaf(F2, W2, s);
-      s = fmaf(F3, W3, s);
-      s = fmaf(F4, W4, s);
-      s = fmaf(F5, W5, s);
-      s = fmaf(F6, W6, s);
-      float r = dd != 0.0f ? fmaf(1.0f / dd, F0, -s) : 0.0f;
-      if (residual) r = bb - r;
-      Y[q0 + BB * z] = r;
-      acc += (double)F0 * (double)r;
+      float res[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const float *c = pl[v] + (z & 1) * P2 + (y + 1) * P + x + 1;
+        const float F0 = xs[v];
+        const float F1 = c[-1], F2 = c[1], F3 = c[-P], F4 = c[P];
+        const float F5 = z > 0 ? xsPrev[v] : fz0[v];
+        const float F6 = z < B - 1 ? xsn[v] : fz1[v];
+        float s = F1 * W1;
+        s = fmaf(F2, W2, s);
+        s = fmaf(F3, W3, s);
+        s = fmaf(F4, W4, s);
+        s = fmaf(F5, W5, s);
+        s = fmaf(F6, W6, s);
+        float r = dd != 0.0f ? fmaf(1.0f / dd, F0, -s) : 0.0f;
+        if (residual) r = bb - r;
+        mv.y[v][q0 + BB * z] = r;
+        res[v] = r;
+        if constexpr (NV == 1) acc[0] += (double)F0 * (double)r;
+      }
+      if constexpr (NV > 1) {
+        /* Galerkin sums of this cell: x_a with b - y_b, then with b */
+        const double bd = bb;
+        double xv[NV], ax[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          xv[v] = xr0[v];
+          ax[v] = bd - (double)res[v];
+        }
+        int j = 0;
+#pragma unroll
+        for (int u = 0; u < NV; u++)
+#pragma unroll
+          for (int v = u; v < NV; v++) acc[j++] += xv[u] * ax[v];
+#pragma unroll
+        for (int u = 0; u < NV; u++) acc[j++] += xv[u] * bd;
+      }
     }
     /* rotate */
-    xsPrev = xs;
-    xs = xsn;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      xsPrev[v] = xs[v];
+      xs[v] = xsn[v];
+      xr0[v] = xr1[v];
+      xr1[v] = xr2[v];
+      rr1[v] = nr1[v];
+    }
     sk = skn;
-    xr1 = xr2;
     fl1 = fl2;
     w0 = nw0;
     w1 = nw1;
@@ -1827,12 +1899,17 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
     w0x = nw0x;
     w1y = nw1y;
     bb = nbb;
-    rr1 = nr1;
     rr2 = nr2;
   }
   if (partials) {
-    const double sres = blockReduceSum(lev == 0 ? acc : 0.0, red);
-    if (threadIdx.x == 0) partials[partialBase + li] = sres;   // list order: the fixed summation order
+#pragma unroll
+    for (int j = 0; j < NG; j++) {
+      const double sres = blockReduceSum(lev == 0 ? acc[j] : 0.0, red);
+      if (threadIdx.x == 0) {
+        if constexpr (NV == 1) partials[partialBase + li] = sres;   // list order: the fixed summation order
+        else partials[(size_t)j * gridDim.x + li] = sres;
+      }
+    }
   }
 }
 
@@ -1960,6 +2037,44 @@ __global__ void __launch_bounds__(kVecThreads) k_update_xr(float *__restrict__ x
   }
   const double s = blockReduceSum(acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+/* k_update_xr and, unless st->skipNet, the partial <r_new, z> of the
+ * z that is about to be replaced (the flexible PCG's <r_k+1, z_k>) at
+ * partials[gridDim.x + w], its products summed in k_dot's order. */
+__global__ void __launch_bounds__(kVecThreads) k_update_xr_rz(float *__restrict__ x, float *__restrict__ r,
+                                                              const float *__restrict__ d, const float *__restrict__ q,
+                                                              const float *__restrict__ z, int64_t n,
+                                                              const PcgState *st, double *partials)
+{
+  if (st->done) return;
+  __shared__ double red[kVecThreads / 64];
+  const float alpha = st->alpha;
+  const bool withZ = !st->skipNet;
+  double acc = 0.0, acz = 0.0;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
+  if (withZ) {
+    for (int64_t i = i0; i < n; i += di) {
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+      acz += (double)rv * (double)z[i];
+    }
+  } else {
+    for (int64_t i = i0; i < n; i += di) {
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (withZ) {
+    const double sz = blockReduceSum(acz, red);
+    if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = sz;
+  }
 }
 
 /* d = z + beta d (src/msbg3.cpp:3224-3235) */
@@ -2439,7 +2554,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   d.learned = nullptr;
   freeFlip(d);
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
-  int maxBlocksPerLaunch = std::max(g.nBlocks, kVecBlocks);
+  int maxBlocksPerLaunch = std::max(g.nBlocks, 2 * kVecBlocks);   // k_update_xr_rz writes two sets
   if (d.partialsCap < maxBlocksPerLaunch) {
     if (d.partials) (void)hipFree(d.partials);
     d.partials = nullptr;
@@ -2591,10 +2706,13 @@ static void launchMultiplyB(const Grid &g, int L, const DevList &l, const float 
                             double *partials, int base, const int *done, hipStream_t s)
 {
   if (l.n == 0) return;
-  if constexpr (B == 32)
-    hipLaunchKernelGGL(k_multiply_z<32>, dim3(l.n), dim3(32 * 32), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
-                       partials, base, done);
-  else
+  if constexpr (B == 32) {
+    MulVecs<1> mv;
+    mv.x[0] = X;
+    mv.y[0] = Y;
+    hipLaunchKernelGGL((k_multiply_z<32, 1>), dim3(l.n), dim3(32 * 32), 0, s, levelArgs(g, L), l.ptr, mv, Bv,
+                       residual, partials, base, done);
+  } else
     hipLaunchKernelGGL(k_multiply<B>, dim3(l.n), dim3(Cfg<B>::NT), 0, s, levelArgs(g, L), l.ptr, X, Bv, Y, residual,
                        partials, base, done);
 }
@@ -2698,6 +2816,12 @@ static void vcycleLaunch(Grid &g, int L, int nPre, int nPost, int nCoarse, const
 static void finalize(Dev &d, int n, int op, double *out)
 {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.state, d.partials, n, op, out);
+}
+
+/* finalize over the partials at d.partials + first */
+static void finalizeAt(Dev &d, int first, int n, int op, double *out)
+{
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.state, d.partials + first, n, op, out);
 }
 
 /* One PCG iteration (src/msbg3.cpp:3149-3236), every kernel a no-op once
