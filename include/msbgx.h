@@ -77,6 +77,10 @@ typedef struct msbgx_solve_params {
                                           captured hipGraph                                  */
 #define MSBGX_OPT_NO_COMPACT 0x4u      /* batched solves: keep converged problems' blocks in the
                                           launches instead of dropping them                  */
+#define MSBGX_OPT_UNFUSED_COARSE 0x8u  /* learned coarse form with two directions on 32^3
+                                          blocks: their residuals one at a time and the
+                                          Galerkin sums in a pass of their own (the form of
+                                          other block sizes and scale counts)                */
 
 const char *msbgx_last_error(void);
 void msbgx_default_solve_params(msbgx_solve_params *p);
@@ -193,7 +197,7 @@ typedef struct msbgx_net msbgx_net;
  *                     and z = zc + V(r - A zc).  Trained on block means of
  *                     r_k (input) and of x* - x_k over the unknowns (target).
  * Setting the form also sets its default stall test (fine: 5 iterations,
- * 10 %; coarse: 16 iterations, 2 %). */
+ * 10 %; coarse: 32 iterations, 2 %). */
 #define MSBGX_NET_FINE 0
 #define MSBGX_NET_COARSE 1
 /* He-normal initial weights from `seed`. */
@@ -237,9 +241,11 @@ int msbgx_net_get_prolongation(const msbgx_net *n);
 #define MSBGX_NET_MAX_SCALES 3
 int msbgx_net_set_coarse_scales(msbgx_net *n, int scales);
 int msbgx_net_get_coarse_scales(const msbgx_net *n);
-/* Stall test of msbgx_solve_pressure_learned: fall back when |r| drops by
- * less than `drop` (relative) over `window` iterations (0 <= window <= 16;
- * 0 disables the test; defaults per form, see above). */
+/* Stall test of msbgx_solve_pressure_learned: a stall is |r| dropping by
+ * less than `drop` (relative) over `window` iterations (0 <= window <= 32;
+ * 0 disables the test; defaults per form, see above).  The fine form falls
+ * back at its first stall; the coarse form restarts its directions at the
+ * first and falls back at the second. */
 int msbgx_net_set_stall(msbgx_net *n, int window, float drop);
 int msbgx_net_num_samples(const msbgx_net *n);
 /* Adam on random minibatches of `batch` samples; first-step and smoothed

@@ -560,7 +560,7 @@ int msbgx_net_set_mode(msbgx_net *n, int mode)
 
 int msbgx_net_set_stall(msbgx_net *n, int window, float drop)
 {
-  if (!n || window < 0 || window > 16 || !(drop >= 0.0f && drop < 1.0f)) { setError("bad argument"); return MSBGX_EINVAL; }
+  if (!n || window < 0 || window > 32 || !(drop >= 0.0f && drop < 1.0f)) { setError("bad argument"); return MSBGX_EINVAL; }
   return netSetStall(*netOf(n), window, drop);
 }
 

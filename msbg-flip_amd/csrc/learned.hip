@@ -1506,7 +1506,7 @@ int netSetMode(Net &n, int mode)
     /* default stall tests: the fine correction is not step-length
      * controlled and is dropped early; the coarse one is scaled by its
      * Galerkin step and only a real stall switches it off */
-    n.stallWin = mode == MSBGX_NET_COARSE ? 16 : 5;
+    n.stallWin = mode == MSBGX_NET_COARSE ? 32 : 5;
     n.stallDrop = mode == MSBGX_NET_COARSE ? 0.02f : 0.1f;
   }
   return MSBGX_OK;
@@ -1817,7 +1817,7 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
   const DevList &l0 = m0.dAct[0];
   bool fused = false;
   if constexpr (K == 2) {   // three directions' state does not fit the registers of 1024 threads
-    fused = g.bsx0 == 32 && l0.n == g.nActive0;
+    fused = g.bsx0 == 32 && l0.n == g.nActive0 && !(g.options & MSBGX_OPT_UNFUSED_COARSE);
     if (fused) {
       /* level-0 blocks: both directions' residuals and the Galerkin partials
        * in one pass; the other MG0 blocks one direction at a time */

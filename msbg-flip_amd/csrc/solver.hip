@@ -102,6 +102,8 @@ __device__ __forceinline__ float smCoef(const LevelArgs &a, int k, int64_t q)
   return fromBits16(a.H[k][q], a.prec);
 }
 
+constexpr int kMaxStallWindow = 32;
+
 struct PcgState {
   double bnorm2, rho, rhoNew, dq, rr, tolSq;
   float alpha, beta;
@@ -110,19 +112,19 @@ struct PcgState {
    * non-finite network output seen, kernels of the correction skip while
    * skipNet is set, iteration of the fallback, the direction update takes
    * the plain V-cycle output (usePlain); r.z_old for the flexible beta and
-   * r.V(r) for the fallback; |r|^2 of the last kStallWindow iterations */
+   * r.V(r) for the fallback; |r|^2 of the last stallWin iterations */
   int learned, restart, bad, skipNet, fellBack, usePlain;
-  int coarseMode, pad5;        // coarse-correction form
-  int stallWin, pad3;          // stall window (0: no stall test), at most kMaxStallWindow
+  int coarseMode, stallRestarts;   // coarse-correction form; stalls answered by a restart so far
+  int stallWin, winBase;       // stall window (0: no stall test), at most kMaxStallWindow; its start
+
   double stallFactor;          // |r|^2 must fall below stallFactor * |r|^2 of stallWin iterations ago
   double rrBest;               // smallest |r|^2 before the stall window
   double zcr;                  // coarse form: <zc, r>
   float gamma, pad4;           // coarse form: Galerkin step <zc, r> / <zc, A zc> of the correction
   double rzOld, rzPlain;
-  double rrWin[16];
+  double rrWin[kMaxStallWindow];
   float coef[4];               // coarse form, several scales: Galerkin coefficients of the directions
 };
-constexpr int kMaxStallWindow = 16;
 
 template <int B> struct Cfg {
   static constexpr int S = B >= 8 ? 8 : B;     // SIMD segment width (src/msbg3.cpp:1700-1743)
@@ -2173,18 +2175,30 @@ __device__ void applyFinalize(PcgState *st, double s, int op, double *out)
       if (st->coarseMode) st->rzPlain = s;   // fallback direction r has r.r = |r|^2
       if (st->learned && st->stallWin > 0) {
         /* stall test of the learned correction (residuals of a flexible CG
-         * need not decrease monotonically): fall back to the plain
-         * preconditioner and restart the directions when no |r|^2 of the
-         * last stallWin iterations fell below stallFactor times the
-         * smallest one before them */
-        const int W = st->stallWin;
-        double &slot = st->rrWin[st->iter % W];
-        if (st->iter > W) st->rrBest = st->iter == W + 1 ? slot : fmin(st->rrBest, slot);
+         * need not decrease monotonically): no |r|^2 of the last stallWin
+         * iterations fell below stallFactor times the smallest one before
+         * them (since the window's start).  The coarse form answers its
+         * first stall by restarting the directions (beta = 0) with the
+         * correction kept and the window started afresh; otherwise, and at
+         * the next stall, it falls back to the plain preconditioner and
+         * restarts the directions */
+        const int W = st->stallWin, k = st->iter - st->winBase;
+        double &slot = st->rrWin[k % W];
+        if (k > W) st->rrBest = k == W + 1 ? slot : fmin(st->rrBest, slot);
         slot = s;
-        if (st->iter > W) {
+        if (k > W) {
           double m = st->rrWin[0];
-          for (int k = 1; k < W; k++) m = fmin(m, st->rrWin[k]);
-          if (!(m < st->stallFactor * st->rrBest)) { st->learned = 0; st->restart = 1; st->fellBack = st->iter; }
+          for (int j = 1; j < W; j++) m = fmin(m, st->rrWin[j]);
+          if (!(m < st->stallFactor * st->rrBest)) {
+            st->restart = 1;
+            if (st->coarseMode && st->stallRestarts == 0) {
+              st->stallRestarts = 1;
+              st->winBase = st->iter;
+            } else {
+              st->learned = 0;
+              st->fellBack = st->iter;
+            }
+          }
         }
       }
       break;
@@ -2265,7 +2279,7 @@ static GraphKey graphKey(const Grid &g, const msbgx_solve_params &p)
   k.nPre = p.n_pre;
   k.nPost = p.n_post;
   k.nCoarse = p.n_coarse;
-  k.opts = g.options & MSBGX_OPT_LITERAL_VCYCLE;
+  k.opts = g.options & (MSBGX_OPT_LITERAL_VCYCLE | MSBGX_OPT_UNFUSED_COARSE);
   return k;
 }
 

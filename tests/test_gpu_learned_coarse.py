@@ -160,8 +160,8 @@ CASES = [("c4_384_b16", lambda: msbgx.config_c4(n=384)), ("c5_128_b32", lambda: 
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prolong,scales", [(0, 1), (1, 1), (0, 3), (1, 3)], ids=["constant", "linear",
-                                                                                "constant_3scales", "linear_3scales"])
+@pytest.mark.parametrize("prolong,scales", [(0, 1), (1, 1), (1, 2), (0, 3), (1, 3)],
+                         ids=["constant", "linear", "linear_2scales", "constant_3scales", "linear_3scales"])
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
 def test_coarse_apply_matches_restatement(name, make, prolong, scales):
     cfg = make()
@@ -226,4 +226,55 @@ def test_linear_prolongation_solve(name, make, scales):
         assert rr <= 1e-6
         if fill is not None:
             assert (fb >= 0) == expect_fb and it <= it0 + 3
+    g.destroy()
+
+
+@pytest.mark.gpu
+def test_two_direction_pass_matches_separate_passes():
+    """Two scales on 32^3 blocks: the residual pass that serves both
+    directions and accumulates the Galerkin sums gives the iterates of the
+    separate residual passes and Galerkin pass (MSBGX_OPT_UNFUSED_COARSE) --
+    the residuals are the same cell by cell and the FP64 sums differ only in
+    their order, so a wrong or missing Galerkin term would show at once."""
+    cfg = msbgx.config_c5(n=256)
+    print(f"two-direction pass {cfg.name} n={cfg.n}", flush=True)
+    g = msbgx.build_grid(cfg)
+    g.prepare_pressure_solve_flip(scene=cfg.scene)
+    g.set_precision(msbgx.PREC_FP16)
+    net = msbgx.LearnedNet(seed=3, mode=msbgx.NET_COARSE)
+    net.prolongation = msbgx.PROLONG_LINEAR
+    net.scales = 2
+    net.set_stall(0, 0.0)
+    runs = []
+    for opts in (0, msbgx.OPT_UNFUSED_COARSE):
+        g.set_options(opts)
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr, fb = net.solve(g, max_iter=40, tol=0.0)
+        x = np.concatenate([g.get_block(msbgx.CH_PRESSURE, 0, b, cfg.bsx).ravel() for b in g.level0_blocks()])
+        runs.append((it, fb, np.array(g.residual_history()), x.astype(np.float64)))
+        print(f"  options {opts}: {it} iterations rel {rr:.3e} fallback {fb}", flush=True)
+    (it_a, fb_a, h_a, x_a), (it_b, fb_b, h_b, x_b) = runs
+    assert it_a == it_b == 40 and fb_a == fb_b == -1
+    np.testing.assert_allclose(h_a, h_b, rtol=1e-4)
+    assert rel(x_a, x_b) <= 1e-4
+    g.destroy()
+
+
+@pytest.mark.gpu
+def test_stall_restarts_the_coarse_form_before_falling_back():
+    """A stall test no solve can pass (|r| to drop 100x over 2 iterations):
+    the fine form falls back at its first test (iteration 3); the coarse form
+    restarts its directions there, starts the window afresh and falls back at
+    the next test (iteration 6); both then converge on the plain V-cycle."""
+    cfg = msbgx.config_c4(n=128)
+    g = msbgx.build_grid(cfg)
+    g.prepare_pressure_solve_flip(scene=cfg.scene)
+    for mode, expect in ((msbgx.NET_FINE, 3), (msbgx.NET_COARSE, 6)):
+        net = msbgx.LearnedNet(seed=2, mode=mode)
+        net.set_stall(2, 0.99)
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6)
+        print(f"  mode {mode}: {it} iterations rel {rr:.2e} fallback at {fb}", flush=True)
+        assert fb == expect and rr <= 1e-6
+        net.destroy()
     g.destroy()
