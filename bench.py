@@ -39,6 +39,9 @@ def parse():
                     help="c5b16: the full-scale particle workload on 16^3 base blocks")
     ap.add_argument("--n", type=int, default=0, help="override the domain size")
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--n-pre", type=int, default=1, help="particle workload: pre-smoothing relaxations of the "
+                    "V-cycle of both solves compared (the plain solve at the reference's 3/3 is reported beside)")
+    ap.add_argument("--n-post", type=int, default=1, help="particle workload: post-smoothing relaxations")
     ap.add_argument("--max-iter", type=int, default=2000)
     ap.add_argument("--cpu-iters", type=int, default=-1,
                     help="PCG iterations in the CPU-baseline sample (-1: 0 for c5 -- the initial V-cycle "
@@ -54,6 +57,8 @@ def parse():
     ap.add_argument("--learned-pairs", type=int, default=2000)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--learned-eval-n", type=int, default=512)
+    ap.add_argument("--learned-val-n", type=int, default=1024, help="coarse form: size of the held-out validation "
+                    "grid of the particle workload")
     ap.add_argument("--learned-eval-config", default="c4", choices=["c4", "c5", "c5b16"],
                     help="evaluation workload of the learned forms (c4: dam break at --learned-eval-n, seeds 501/502; "
                          "c5/c5b16: the 1024^3 particle workload on 32^3 / 16^3 blocks)")
@@ -238,14 +243,19 @@ def learned_train(msbgx, args, log, mode="coarse"):
     `blocks_per_pair` random level-0 blocks each; coarse form: 16^3 tiles of
     the block-mean grids at each of the net's scales, random tile origins);
     Adam on the GPU.  Coarse form: after each schedule stage the net is
-    validated on a held-out seed of the particle workload at 512^3 (seed 900:
-    neither a training seed nor the evaluation seed; FP16 operands) and the
-    weights with the fewest iterations are kept.  Returns (net, record)."""
+    validated on a held-out seed of the particle workload at the evaluation
+    size (--learned-val-n, seed 900: neither a training seed nor the
+    evaluation seed; FP16 operands) with the V-cycle of the timed solve
+    (--n-pre/--n-post), and the weights with the fewest iterations are kept.
+    The training pairs come from solves at the reference's V-cycle (3/3).
+    Returns (net, record)."""
     t0 = time.time()
     net = msbgx.LearnedNet(seed=2024, mode=msbgx.NET_COARSE if mode == "coarse" else msbgx.NET_FINE)
     if mode == "coarse":
         net.scales = args.scales
         net.prolongation = args.prolongation
+    # the coarse form's validation solves use the V-cycle of the solve it is timed in
+    vcyc = {"n_pre": args.n_pre, "n_post": args.n_post} if mode == "coarse" else {}
     pairs, seed = 0, 0
     while pairs < args.learned_pairs:
         seed += 1
@@ -266,7 +276,7 @@ def learned_train(msbgx, args, log, mode="coarse"):
     train_ms, first, last = 0.0, None, None
     gv = None
     if mode == "coarse":
-        vc = msbgx.config_c5(512, seed=900)
+        vc = msbgx.config_c5(args.learned_val_n, seed=900)
         gv = msbgx.build_grid(vc)
         gv.prepare_pressure_solve_flip(scene=vc.scene)
         gv.set_precision(msbgx.PREC_FP16)
@@ -279,9 +289,9 @@ def learned_train(msbgx, args, log, mode="coarse"):
         cum += steps
         if gv is not None:
             gv.zero_channel(msbgx.CH_PRESSURE, 0)
-            it_v, rr_v, fb_v = net.solve(gv, max_iter=args.max_iter, tol=args.tol)
+            it_v, rr_v, fb_v = net.solve(gv, max_iter=args.max_iter, tol=args.tol, **vcyc)
             selection.append({"steps": cum, "val_iterations": it_v, "val_fallback": fb_v, "weights": net.weights})
-            log(f"learned ({mode}): after {cum} steps, validation (particle workload 512^3, seed 900): "
+            log(f"learned ({mode}): after {cum} steps, validation (particle workload {args.learned_val_n}^3, seed 900): "
                 f"{it_v} iterations, fallback {fb_v}")
     chosen = None
     if selection:
@@ -299,6 +309,8 @@ def learned_train(msbgx, args, log, mode="coarse"):
            "validation": [{k: v for k, v in e.items() if k != "weights"} for e in selection],
            "kept_weights_after_steps": chosen, "wall_s": round(time.time() - t0, 1)}
     if mode == "coarse":
+        rec["pairs_v_cycle"] = "V(3,3) (the reference's nPre/nPost)"
+        rec["validation_grid"] = f"particle workload {args.learned_val_n}^3, seed 900, V({args.n_pre},{args.n_post})"
         rec["scales"] = net.scales
         rec["prolongation"] = "trilinear" if net.prolongation == msbgx.PROLONG_LINEAR else "constant"
     return net, rec
@@ -584,8 +596,9 @@ def cpu_baseline(cfg, args, gpu_iters, log):
     o.prepare_scene(scene)
     t_prep = time.time() - t0
     k = args.cpu_iters if args.cpu_iters >= 0 else (0 if big else max(1, gpu_iters))
+    vc = {"n_pre": args.n_pre, "n_post": args.n_post} if args.config == "c5" else {}   # the GPU plain solve's V-cycle
     t0 = time.time()
-    it, rr, _ = o.solve(max_iter=k, tol=args.tol)
+    it, rr, _ = o.solve(max_iter=k, tol=args.tol, **vc)
     t_k = time.time() - t0
     del o
     # the solve runs one preconditioner application before the loop: count it as one iteration
@@ -617,7 +630,8 @@ def cpu_baseline(cfg, args, gpu_iters, log):
             "one_thread_ms_per_iteration": round(t1 * 1e3, 1) if t1 is not None else None,
             "sample": (f"oracle/msbg_oracle.cpp (C++ restatement of the reference solver, -O3 -march=native, OpenMP "
                        f"over the blocks of a colour) on {threads} threads of {model}, {cfg.name} {cfg.n}^3"
-                       + (", FP16 smoother operands" if args.config == "c5" else "") + ": "
+                       + (f", FP16 smoother operands, V({args.n_pre},{args.n_post})" if args.config == "c5" else "")
+                       + ": "
                        + (f"the whole solve to {args.tol:g} ({it} PCG iterations, the GPU's count) timed"
                           if whole else
                           f"{it} PCG iterations + the initial V-cycle timed = {t_k:.1f} s, i.e. {per_iter:.2f} s per "
@@ -627,8 +641,11 @@ def cpu_baseline(cfg, args, gpu_iters, log):
 
 def c5_headline(msbgx, args, log):
     """The headline: plain vs learned MG-PCG to --tol on the particle workload
-    (FP16 smoother operands), the timed region of --steps learned iterations,
-    and the smoother's roofline on the same grid."""
+    (FP16 smoother operands), both with the V-cycle V(--n-pre, --n-post) so
+    that the ratio is the network's own gain; the plain MG-PCG with the
+    reference's V(3,3) is timed beside them.  Then the timed region of
+    --steps learned iterations and the smoother's roofline on the same
+    grid."""
     import numpy as np
     cfg = make_cfg(msbgx, "c5", args.n)
     t0 = time.time()
@@ -642,27 +659,37 @@ def c5_headline(msbgx, args, log):
     log(f"prepare {cfg.name} n={cfg.n}: {time.time() - t0:.1f} s, {nb0} PCG blocks "
         f"({nb0 * cfg.bsx ** 3} cells), {len(g.relax_list(0))} relaxation blocks at MG0")
     net, train = learned_train(msbgx, args, log, "coarse")
-    # plain MG-PCG: graph capture on a short solve, then the whole solve
-    g.solve_pressure_flip(max_iter=3, tol=0.0)
-    g.zero_channel(msbgx.CH_PRESSURE, 0)
-    it_p, rr_p = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol)
-    ms_p = g.last_solve_ms()
-    log(f"plain MG-PCG: {it_p} iterations, rel {rr_p:.3e}, {ms_p:.1f} ms ({ms_p / max(it_p, 1):.2f} ms/iteration)")
+    vc = {"n_pre": args.n_pre, "n_post": args.n_post}
+    vname = f"V({args.n_pre},{args.n_post})"
+    # plain MG-PCG: graph capture on a short solve, then the whole solve; the
+    # compared V-cycle, then the reference's V(3,3)
+    plain = {}
+    for name, kw in ((vname, vc), ("V(3,3)", {"n_pre": 3, "n_post": 3})):
+        if name in plain:
+            continue
+        g.solve_pressure_flip(max_iter=3, tol=0.0, **kw)
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr = g.solve_pressure_flip(max_iter=args.max_iter, tol=args.tol, **kw)
+        ms = g.last_solve_ms()
+        plain[name] = (it, rr, ms)
+        log(f"plain MG-PCG {name}: {it} iterations, rel {rr:.3e}, {ms:.1f} ms ({ms / max(it, 1):.2f} ms/iteration)")
+    it_p, rr_p, ms_p = plain[vname]
+    it_r, rr_r, ms_r = plain["V(3,3)"]
     # timed region: --warmup untimed, then exactly --steps learned MG-PCG iterations
     g.zero_channel(msbgx.CH_PRESSURE, 0)
-    net.solve(g, max_iter=max(args.warmup, 1), tol=0.0)
+    net.solve(g, max_iter=max(args.warmup, 1), tol=0.0, **vc)
     g.zero_channel(msbgx.CH_PRESSURE, 0)
     w0 = time.perf_counter()
-    k_it, _, _ = net.solve(g, max_iter=args.steps, tol=0.0)   # returns after the device is idle
+    k_it, _, _ = net.solve(g, max_iter=args.steps, tol=0.0, **vc)   # returns after the device is idle
     wall_ms = (time.perf_counter() - w0) * 1e3
     steps_ms = g.last_solve_ms()
     log(f"timed: {k_it} learned iterations in {steps_ms:.1f} ms (HIP events), {wall_ms:.1f} ms wall")
     # the learned solve to the tolerance
     g.zero_channel(msbgx.CH_PRESSURE, 0)
-    it_l, rr_l, fb = net.solve(g, max_iter=args.max_iter, tol=args.tol)
+    it_l, rr_l, fb = net.solve(g, max_iter=args.max_iter, tol=args.tol, **vc)
     ms_l = g.last_solve_ms()
-    log(f"learned MG-PCG: {it_l} iterations, rel {rr_l:.3e}, fallback {fb}, {ms_l:.1f} ms; "
-        f"speedup {ms_p / ms_l:.2f}x")
+    log(f"learned MG-PCG {vname}: {it_l} iterations, rel {rr_l:.3e}, fallback {fb}, {ms_l:.1f} ms; "
+        f"speedup {ms_p / ms_l:.2f}x over plain {vname}, {ms_r / ms_l:.2f}x over plain V(3,3)")
     ms_relax, bytes_relax, cells_relax = g.time_kernel(0, reps=20)
     ms_res, bytes_res, _ = g.time_kernel(1, reps=20)
     ms_upd, bytes_upd, _ = g.time_kernel(2, reps=20)
@@ -693,7 +720,8 @@ def c5_headline(msbgx, args, log):
         "config": {
             "workload": f"{cfg.name} {cfg.n}^3, {cfg.bsx}^3 base blocks, {cfg.levels} levels, FP16 smoother "
                         f"operands; learned coarse correction ({train['scales']} scales, {train['prolongation']} "
-                        f"prolongation) vs plain MG-PCG, to {args.tol:g}",
+                        f"prolongation) vs plain MG-PCG, both with the V-cycle {vname}, to {args.tol:g}",
+            "v_cycle": {"n_pre": args.n_pre, "n_post": args.n_post, "n_coarse": 24},
             "step": "one learned MG-PCG iteration on this system (value: the whole learned solve)",
             "blocks_per_level": [int(c) for c in np.bincount(lv, minlength=cfg.levels)],
             "pcg_blocks": nb0,
@@ -703,8 +731,12 @@ def c5_headline(msbgx, args, log):
             "parallelism": "1 GPU",
         },
         "learned_vs_plain": {
+            "v_cycle": vname,
             "plain_ms": round(ms_p, 1), "plain_iterations": it_p, "plain_rel_residual": rr_p,
             "plain_ms_per_iteration": round(ms_p / max(it_p, 1), 3),
+            "plain_reference_v_cycle": {"v_cycle": "V(3,3) (FlipPressureSolveParams defaults, src/msbg.h:837-844)",
+                                        "ms": round(ms_r, 1), "iterations": it_r, "rel_residual": rr_r,
+                                        "learned_speedup": round(ms_r / ms_l, 3)},
             "learned_ms": round(ms_l, 1), "learned_iterations": it_l, "learned_rel_residual": rr_l,
             "learned_fallback_iteration": fb, "learned_ms_per_iteration": round(steps_ms / max(k_it, 1), 3),
             "speedup": round(ms_p / ms_l, 3), "timed_region_wall_ms": round(wall_ms, 1),

@@ -623,6 +623,57 @@ __global__ void __launch_bounds__(256) k_nn_block_mean(const float *__restrict__
   if (threadIdx.x == 0) coarse[bid] = num > 0.0 ? (float)(sum / num) : 0.0f;
 }
 
+/* The coarse form's PCG update, one workgroup per PCG block (blocks[w],
+ * its ncell cells at off[]): x += alpha d, r -= alpha q, the partial |r|^2
+ * at partials[w] and, unless st->skipNet, the partial <r_new, z> of the z
+ * about to be replaced at partials[gridDim.x + w] and the block's mean of
+ * r_new into coarse[block] (k_nn_block_mean's sum, in its order), so that
+ * the network's input needs no second pass over r. */
+__global__ void __launch_bounds__(256) k_update_xr_rz_mean(float *__restrict__ x, float *__restrict__ r,
+                                                           const float *__restrict__ d, const float *__restrict__ q,
+                                                           const float *__restrict__ z, const int *__restrict__ blocks,
+                                                           const int64_t *__restrict__ off, int ncell,
+                                                           const PcgState *st, double *partials,
+                                                           float *__restrict__ coarse)
+{
+  if (st->done) return;
+  __shared__ double red[4];
+  const int bid = blocks[blockIdx.x];
+  const int64_t o = off[bid];
+  const float alpha = st->alpha;
+  const bool withZ = !st->skipNet;
+  double acc = 0.0, acz = 0.0, am = 0.0;
+  if (withZ) {
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+      const int64_t i = o + c;
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+      acz += (double)rv * (double)z[i];
+      am += (double)rv;
+    }
+  } else {
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+      const int64_t i = o + c;
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double s = blockReduceSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (withZ) {
+    const double sz = blockReduceSum(acz, red);
+    const double sm = blockReduceSum(am, red);
+    if (threadIdx.x == 0) {
+      partials[gridDim.x + blockIdx.x] = sz;
+      coarse[bid] = (float)(sm / (double)ncell);
+    }
+  }
+}
+
 /* Patch of tile t of the dense block-resolution grid: tile (tx, ty, tz)
  * covers coarse cells 16 t - o .. 16 t - o + 15 per axis (o: tile origin
  * offset, 0 at inference, random in training so that the domain boundary
@@ -967,12 +1018,20 @@ __global__ void __launch_bounds__(256) k_nn_gram_solve(const double *partials, i
 {
   if (st->done || st->skipNet) return;
   constexpr int NG = K * (K + 1) / 2;
-  __shared__ double v[NG + K];
+  __shared__ double v[NG + K], red[4];
+  /* every sum in one pass over the partials (each in sumPartials' order) */
+  double acc[NG + K];
+#pragma unroll
+  for (int j = 0; j < NG + K; j++) acc[j] = 0.0;
+  for (int i = threadIdx.x; i < nPart; i += blockDim.x)
+#pragma unroll
+    for (int j = 0; j < NG + K; j++) acc[j] += partials[(size_t)j * nPart + i];
+#pragma unroll
   for (int j = 0; j < NG + K; j++) {
-    const double t = sumPartials(partials + (size_t)j * nPart, nPart);
+    const double t = blockReduceSum(acc[j], red);
     if (threadIdx.x == 0) v[j] = t;
-    __syncthreads();
   }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double G[K][K], L[K][K], y[K], c[K];
   bool use[K];
@@ -1020,12 +1079,12 @@ __global__ void __launch_bounds__(256) k_nn_gram_solve(const double *partials, i
   for (int a = 0; a < K; a++) st->coef[a] = isfinite(c[a]) ? (float)c[a] : 0.0f;
 }
 
-/* zc = sum_a c_a z_a and rs = r - A zc = (1 - sum_a c_a) r + sum_a c_a q_a
- * over the MG0 pool (z[0] may be zc and q[0] rs); with the network off,
- * rs = r exactly (see k_nn_gamma). */
+/* rs = r - A zc = (1 - sum_a c_a) r + sum_a c_a q_a over the MG0 pool for
+ * zc = sum_a c_a z_a, which k_nn_add_dot forms where it adds it (q[0] may be
+ * rs); with the network off, rs = r exactly (see k_nn_gamma). */
 template <int K>
-__global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *zc, float *rs, const float *__restrict__ r,
-                                                    int64_t n, const PcgState *st)
+__global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const float *__restrict__ r, int64_t n,
+                                                    const PcgState *st)
 {
   if (st->done) return;
   const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
@@ -1040,13 +1099,9 @@ __global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *zc, float 
     cs += cf[a];
   }
   for (int64_t i = i0; i < n; i += di) {
-    float zv = 0.0f, rv = (1.0f - cs) * r[i];
+    float rv = (1.0f - cs) * r[i];
 #pragma unroll
-    for (int a = 0; a < K; a++) {
-      zv = fmaf(cf[a], ds.z[a][i], zv);
-      rv = fmaf(cf[a], ds.q[a][i], rv);
-    }
-    zc[i] = zv;
+    for (int a = 0; a < K; a++) rv = fmaf(cf[a], ds.q[a][i], rv);
     rs[i] = rv;
   }
 }
@@ -1059,20 +1114,33 @@ __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, in
     z[i] += zc[i];
 }
 
-/* z += zc unless *skip, then (unless *done) the partial <r, z> of the
- * result, its products summed in k_dot's order */
-__global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ z, const float *__restrict__ zc,
-                                                            const float *__restrict__ r, int64_t n, const int *skip,
-                                                            const int *done, double *partials)
+/* z += zc unless st->skipNet, then (unless st->done) the partial <r, z> of
+ * the result, its products summed in k_dot's order.  zc: the scaled
+ * direction ds.z[0] (K = 1, k_nn_gamma), else sum_a c_a z_a formed here. */
+template <int K>
+__global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ z, DirSet ds,
+                                                            const float *__restrict__ r, int64_t n,
+                                                            const PcgState *st, double *partials)
 {
-  if (*done) return;
+  if (st->done) return;
   __shared__ double red[kVecThreads / 64];
-  const bool add = !*skip;
+  const bool add = !st->skipNet;
+  float cf[K];
+#pragma unroll
+  for (int a = 0; a < K; a++) cf[a] = K == 1 ? 1.0f : st->coef[a];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float zv = z[i];
     if (add) {
-      zv += zc[i];
+      float zc;
+      if constexpr (K == 1) {
+        zc = ds.z[0][i];
+      } else {
+        zc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < K; a++) zc = fmaf(cf[a], ds.z[a][i], zc);
+      }
+      zv += zc;
       z[i] = zv;
     }
     acc += (double)r[i] * (double)zv;
@@ -1627,7 +1695,8 @@ static void netCorrect(Grid &g, Net &n, LearnedCtx &c, hipStream_t s)
 }
 
 static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s);
-static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s);
+static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s,
+                          bool meanDone = false);
 
 /* Per-kernel entry point on the level-0 PCG blocks: fine form
  * ch_out += N(ch_in); coarse form ch_out += sum over the net's scales of
@@ -1734,8 +1803,11 @@ static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
  * dense block grid (0 for blocks without PCG unknowns), R_2f the mean of
  * R_f over the PCG blocks of each 2^3 cells, N the network over 16^3 tiles
  * of each grid, P_f the net's prolongation onto the PCG blocks' unknowns
- * (skipped while st->skipNet; st may be null). */
-static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s)
+ * (skipped while st->skipNet; st may be null).  meanDone: R_1 src is in
+ * place already (k_update_xr_rz_mean; the grid's other cells are still 0
+ * from the first call). */
+static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s,
+                          bool meanDone)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
@@ -1743,9 +1815,11 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
   const int ncell = g.bsx0 * g.bsx0 * g.bsx0;
   if (g.nActive0 == 0) return;
   LearnedCtx::Scale &s0 = c.sc[0];
-  hipLaunchKernelGGL(k_nn_fill, dim3((s0.ncoarse + 255) / 256), dim3(256), 0, s, s0.cin, s0.ncoarse, 0.0f, skip);
-  hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, src, d.dActive0, m0.d_off, m0.d_flags,
-                     m0.d_D, ncell, 0, s0.cin, skip);
+  if (!meanDone) {
+    hipLaunchKernelGGL(k_nn_fill, dim3((s0.ncoarse + 255) / 256), dim3(256), 0, s, s0.cin, s0.ncoarse, 0.0f, skip);
+    hipLaunchKernelGGL(k_nn_block_mean, dim3(g.nActive0), dim3(256), 0, s, src, d.dActive0, m0.d_off, m0.d_flags,
+                       m0.d_D, ncell, 0, s0.cin, skip);
+  }
   for (int k = 0; k < n.scales; k++) {
     LearnedCtx::Scale &q = c.sc[k];
     if (k > 0) {
@@ -1848,7 +1922,7 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
     hipLaunchKernelGGL(k_nn_gram<K>, dim3(kGramBlocks), dim3(256), 0, s, ds, m0.d_ch[C_R], c.n0, c.gram, skip);
     hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, kGramBlocks, d.state);
   }
-  hipLaunchKernelGGL(k_nn_combine<K>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.zc, c.rs, m0.d_ch[C_R],
+  hipLaunchKernelGGL(k_nn_combine<K>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.rs, m0.d_ch[C_R],
                      (int64_t)m0.ncells, d.state);
 }
 
@@ -1856,12 +1930,13 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
  * z_f = P_f N(R_f r) at the net's scales, zc their Galerkin combination
  * (one scale: the step gamma = <zc, r> / <zc, A zc>); then
  * z = zc + V(r - A zc), and the partials of <r, z> for the next finalize. */
-static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p, hipStream_t s)
+static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p, hipStream_t s,
+                              bool meanDone = false)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *skip = &d.state->skipNet;
-  coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s);
+  coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s, meanDone);
   if (n.scales == 1) {
     multiplyLaunch(g, 0, true, c.sc[0].z, m0.d_ch[C_R], c.sc[0].q, nullptr, skip, s);
     /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
@@ -1881,8 +1956,22 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   zeroActive(g, 0, m0.d_ch[C_Z], &d.state->done, s);
   vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->done, s);
   std::swap(m0.d_ch[C_R], c.rs);
-  hipLaunchKernelGGL(k_nn_add_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], c.zc, m0.d_ch[C_R], c.n0,
-                     skip, &d.state->done, d.partials);
+  DirSet ds{};
+  for (int a = 0; a < n.scales; a++) ds.z[a] = c.sc[a].z;
+  switch (n.scales) {
+    case 1:
+      hipLaunchKernelGGL(k_nn_add_dot<1>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], ds, m0.d_ch[C_R],
+                         c.n0, d.state, d.partials);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_nn_add_dot<2>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], ds, m0.d_ch[C_R],
+                         c.n0, d.state, d.partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_nn_add_dot<3>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], ds, m0.d_ch[C_R],
+                         c.n0, d.state, d.partials);
+      break;
+  }
 }
 
 static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_params &p)
@@ -1893,12 +1982,16 @@ static void pcgIterationLearnedCoarse(Grid &g, Net &n, LearnedCtx &c, const msbg
   const int64_t n0 = c.n0;
   const int nPart = multiplyLaunch(g, 0, false, m0.d_ch[C_D], nullptr, m0.d_ch[C_Q], d.partials, done, d.stream);
   finalize(d, nPart, FIN_ALPHA, nullptr);
-  /* <r_k+1, z_k> of the flexible beta comes with the update, before z_k is replaced */
-  hipLaunchKernelGGL(k_update_xr_rz, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
-                     m0.d_ch[C_D], m0.d_ch[C_Q], m0.d_ch[C_Z], n0, d.state, d.partials);
-  finalize(d, kVecBlocks, FIN_CONV, d.hist);
-  finalizeAt(d, kVecBlocks, kVecBlocks, FIN_RZOLD, nullptr);
-  coarseLearnedPath(g, n, c, p, d.stream);   // its V-cycle is the plain one while the network is off
+  /* <r_k+1, z_k> of the flexible beta comes with the update, before z_k is
+   * replaced, and so does the network's block-mean input */
+  const int nb = g.nActive0;
+  if (nb > 0)
+    hipLaunchKernelGGL(k_update_xr_rz_mean, dim3(nb), dim3(256), 0, d.stream, m0.d_ch[C_X], m0.d_ch[C_R],
+                       m0.d_ch[C_D], m0.d_ch[C_Q], m0.d_ch[C_Z], d.dActive0, m0.d_off, g.bsx0 * g.bsx0 * g.bsx0,
+                       d.state, d.partials, c.sc[0].cin);
+  finalize(d, nb, FIN_CONV, d.hist);
+  finalizeAt(d, nb, nb, FIN_RZOLD, nullptr);
+  coarseLearnedPath(g, n, c, p, d.stream, true);   // its V-cycle is the plain one while the network is off
   finalize(d, kVecBlocks, FIN_BETA, nullptr);
   /* a rejected corrected z restarts along r */
   hipLaunchKernelGGL(k_nn_axpy, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z],

@@ -2041,44 +2041,6 @@ __global__ void __launch_bounds__(kVecThreads) k_update_xr(float *__restrict__ x
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-/* k_update_xr and, unless st->skipNet, the partial <r_new, z> of the
- * z that is about to be replaced (the flexible PCG's <r_k+1, z_k>) at
- * partials[gridDim.x + w], its products summed in k_dot's order. */
-__global__ void __launch_bounds__(kVecThreads) k_update_xr_rz(float *__restrict__ x, float *__restrict__ r,
-                                                              const float *__restrict__ d, const float *__restrict__ q,
-                                                              const float *__restrict__ z, int64_t n,
-                                                              const PcgState *st, double *partials)
-{
-  if (st->done) return;
-  __shared__ double red[kVecThreads / 64];
-  const float alpha = st->alpha;
-  const bool withZ = !st->skipNet;
-  double acc = 0.0, acz = 0.0;
-  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
-  if (withZ) {
-    for (int64_t i = i0; i < n; i += di) {
-      x[i] = fmaf(alpha, d[i], x[i]);
-      const float rv = fmaf(-alpha, q[i], r[i]);
-      r[i] = rv;
-      acc += (double)rv * (double)rv;
-      acz += (double)rv * (double)z[i];
-    }
-  } else {
-    for (int64_t i = i0; i < n; i += di) {
-      x[i] = fmaf(alpha, d[i], x[i]);
-      const float rv = fmaf(-alpha, q[i], r[i]);
-      r[i] = rv;
-      acc += (double)rv * (double)rv;
-    }
-  }
-  const double s = blockReduceSum(acc, red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-  if (withZ) {
-    const double sz = blockReduceSum(acz, red);
-    if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = sz;
-  }
-}
-
 /* d = z + beta d (src/msbg3.cpp:3224-3235) */
 __global__ void __launch_bounds__(kVecThreads) k_axpy_d(float *__restrict__ d, const float *__restrict__ z, int64_t n,
                                                         const PcgState *st)
@@ -2568,7 +2530,8 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   d.learned = nullptr;
   freeFlip(d);
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
-  int maxBlocksPerLaunch = std::max(g.nBlocks, 2 * kVecBlocks);   // k_update_xr_rz writes two sets
+  /* partials: one per block of a launch, per vector workgroup, and two sets per PCG block (k_update_xr_rz_mean) */
+  int maxBlocksPerLaunch = std::max({g.nBlocks, 2 * kVecBlocks, 2 * g.nActive0});
   if (d.partialsCap < maxBlocksPerLaunch) {
     if (d.partials) (void)hipFree(d.partials);
     d.partials = nullptr;
