@@ -413,44 +413,70 @@ __global__ void k_nn_reduce_groups(const float *__restrict__ partial, int nPart,
 
 /* Per-workgroup partials of the two 432-entry gradients:
  *   FIRST: dW1[t][co] = sum_pos X0[pos + off_t] dZ1[pos][co]   (planes 1..18)
- *   LAST:  dW4[t][ci] = sum_cell A3[pos(cell) + off_t][ci] dY[cell] */
+ *   LAST:  dW4[t][ci] = sum_cell A3[pos(cell) + off_t][ci] dY[cell]
+ * The workgroup's slab (and the planes its taps reach) is staged in LDS
+ * first; each thread then sums its entries over the slab's positions in
+ * ascending order, so the sums are those of reading them from memory. */
 template <bool LAST>
 __global__ void __launch_bounds__(256) k_nn_dw_small(const uint16_t *__restrict__ X0, const int *__restrict__ idx,
                                                      const uint16_t *__restrict__ G1, const uint16_t *__restrict__ A3,
                                                      const float *__restrict__ dY, float *__restrict__ partial)
 {
+  constexpr int NST = 4 * nn::PL;   // staged positions: the slab's two planes and one on either side
+  __shared__ uint4 lds[LAST ? NST * nn::C / 8 : (NST + 2 * nn::PL * nn::C) / 8];
   const int i = blockIdx.x;
   float acc[2] = {0.0f, 0.0f};
   int ent[2];
   for (int k = 0; k < 2; k++) ent[k] = threadIdx.x + 256 * k;
   if (!LAST) {
     const int s = idx ? idx[i] : i;
-    const uint16_t *x = X0 + (int64_t)s * nn::PS + nn::SL;
-    const uint16_t *g = G1 + ((int64_t)i * nn::PS + nn::SL) * nn::C;
     const int z0 = 1 + nn::ZP * blockIdx.y;
-    for (int q = 0; q < nn::ZP * nn::PL; q++) {
-      const int pos = z0 * nn::PL + q;
-      if (nnRing(pos)) continue;
-#pragma unroll
-      for (int k = 0; k < 2; k++) {
-        if (ent[k] >= nn::NW1) continue;
-        const int t = ent[k] / nn::C, co = ent[k] % nn::C;
-        acc[k] = fmaf(nnBf2f(x[pos + nn::tapOff(t)]), nnBf2f(g[(int64_t)pos * nn::C + co]), acc[k]);
-      }
+    const uint16_t *x = X0 + (int64_t)s * nn::PS + nn::SL + (int64_t)(z0 - 1) * nn::PL;   // planes z0-1 .. z0+2
+    const uint16_t *g = G1 + ((int64_t)i * nn::PS + nn::SL + (int64_t)z0 * nn::PL) * nn::C;   // planes z0, z0+1
+    const uint4 *xs4 = reinterpret_cast<const uint4 *>(x), *gs4 = reinterpret_cast<const uint4 *>(g);
+    for (int c = threadIdx.x; c < NST / 8; c += blockDim.x) lds[c] = xs4[c];
+    for (int c = threadIdx.x; c < 2 * nn::PL * nn::C / 8; c += blockDim.x) lds[NST / 8 + c] = gs4[c];
+    __syncthreads();
+    const uint16_t *xl = reinterpret_cast<const uint16_t *>(lds);
+    const uint16_t *gl = xl + NST;
+    int xo[2], co[2];
+    for (int k = 0; k < 2; k++) {
+      const int t = ent[k] < nn::NW1 ? ent[k] / nn::C : 0;
+      co[k] = ent[k] % nn::C;
+      xo[k] = nn::PL + nn::tapOff(t);   // staged index of the tap's source for slab position 0
     }
+    /* slab positions q = plane * PL + v * E + u in ascending order, ring positions skipped */
+    for (int pz = 0; pz < nn::ZP; pz++)
+      for (int v = 1; v < nn::E - 1; v++)
+        for (int u = 1; u < nn::E - 1; u++) {
+          const int q = pz * nn::PL + v * nn::E + u;
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            if (ent[k] >= nn::NW1) continue;
+            acc[k] = fmaf(nnBf2f(xl[q + xo[k]]), nnBf2f(gl[q * nn::C + co[k]]), acc[k]);
+          }
+        }
   } else {
-    const uint16_t *a = A3 + ((int64_t)i * nn::PS + nn::SL) * nn::C;
-    const float *dy = dY + (int64_t)i * nn::NCELL;
-    const int cz0 = 2 * blockIdx.y;
+    const int cz0 = 2 * blockIdx.y;   // cell planes cz0, cz0 + 1 = patch planes cz0 + 2, cz0 + 3
+    const uint16_t *a = A3 + ((int64_t)i * nn::PS + nn::SL + (int64_t)(cz0 + 1) * nn::PL) * nn::C;
+    const uint4 *as4 = reinterpret_cast<const uint4 *>(a);
+    for (int c = threadIdx.x; c < NST * nn::C / 8; c += blockDim.x) lds[c] = as4[c];
+    __syncthreads();
+    const uint16_t *al = reinterpret_cast<const uint16_t *>(lds);
+    const float *dy = dY + (int64_t)i * nn::NCELL + 256 * cz0;
+    int ao[2];
+    for (int k = 0; k < 2; k++) {
+      const int t = ent[k] < nn::NW4 ? ent[k] / nn::C : 0, ci = ent[k] % nn::C;
+      ao[k] = (nn::PL + nn::tapOff(t)) * nn::C + ci;
+    }
     for (int q = 0; q < 512; q++) {
-      const int cx = q & 15, cy = (q >> 4) & 15, cz = cz0 + (q >> 8);
-      const float g = dy[cx + 16 * cy + 256 * cz];
-      const int pos = (cz + 2) * nn::PL + (cy + 2) * nn::E + (cx + 2);
+      const int cx = q & 15, cy = (q >> 4) & 15, cz = q >> 8;
+      const float gv = dy[q];
+      const int pos = cz * nn::PL + (cy + 2) * nn::E + (cx + 2);   // staged plane index from cz0 + 2
 #pragma unroll
       for (int k = 0; k < 2; k++) {
         if (ent[k] >= nn::NW4) continue;
-        const int t = ent[k] / nn::C, ci = ent[k] % nn::C;
-        acc[k] = fmaf(nnBf2f(a[(int64_t)(pos + nn::tapOff(t)) * nn::C + ci]), g, acc[k]);
+        acc[k] = fmaf(nnBf2f(al[pos * nn::C + ao[k]]), gv, acc[k]);
       }
     }
   }
@@ -459,14 +485,27 @@ __global__ void __launch_bounds__(256) k_nn_dw_small(const uint16_t *__restrict_
     if (ent[k] < nn::NW1) o[ent[k]] = acc[k];
 }
 
-/* grad[e] = sum over nPart partial rows (fixed order, FP64). */
-__global__ void k_nn_reduce(const float *__restrict__ partial, int nPart, int nEnt, float *__restrict__ grad)
+/* grad[e] = sum over nPart partial rows in FP64, in a fixed order: 64
+ * entries per workgroup of kReduceWaves waves, wave w summing rows w,
+ * w + kReduceWaves, ... for its lane's entry, then the waves' sums in wave
+ * order.  Launch: grid ceil(nEnt / 64), block 64 * kReduceWaves. */
+constexpr int kReduceWaves = 16;
+__global__ void __launch_bounds__(64 * kReduceWaves) k_nn_reduce(const float *__restrict__ partial, int nPart,
+                                                                  int nEnt, float *__restrict__ grad)
 {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nEnt) return;
+  __shared__ double red[kReduceWaves][64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int e = blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int p = 0; p < nPart; p++) s += (double)partial[(int64_t)p * nEnt + e];
-  grad[e] = (float)s;
+  if (e < nEnt)
+    for (int p = w; p < nPart; p += kReduceWaves) s += (double)partial[(int64_t)p * nEnt + e];
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && e < nEnt) {
+    double t = red[0][lane];
+    for (int k = 1; k < kReduceWaves; k++) t += red[k][lane];
+    grad[e] = (float)t;
+  }
 }
 
 __global__ void k_nn_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
@@ -1402,7 +1441,8 @@ static void netWeightGrad16(const uint16_t *A, const uint16_t *dZ, NetBuffers &b
   hipLaunchKernelGGL(k_nn_dw16_mfma, dim3(nb, 18), dim3(256), 0, s, A, dZ, b.part);
   hipLaunchKernelGGL(k_nn_reduce_groups, dim3((nn::NW2 + 255) / 256, kDwGroups), dim3(256), 0, s, b.part, rows,
                      nn::NW2, part2);
-  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 255) / 256), dim3(256), 0, s, part2, kDwGroups, nn::NW2, grad);
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW2 + 63) / 64), dim3(64 * kReduceWaves), 0, s, part2, kDwGroups, nn::NW2,
+                     grad);
 }
 
 /* Loss and gradient over `nb` samples of the training set addressed by the
@@ -1419,7 +1459,7 @@ static void netLossGrad(Net &n, NetBuffers &b, int nb, float invN)
   /* layer 4 */
   hipLaunchKernelGGL(k_nn_dw_small<true>, dim3(nb, 8), dim3(256), 0, s, (const uint16_t *)nullptr,
                      (const int *)nullptr, (const uint16_t *)nullptr, b.A3, b.dY, b.part);
-  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW4 + 255) / 256), dim3(256), 0, s, b.part, nb * 8, nn::NW4,
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW4 + 63) / 64), dim3(64 * kReduceWaves), 0, s, b.part, nb * 8, nn::NW4,
                      n.grad + nn::OFF_W4);
   hipLaunchKernelGGL(k_nn_bwd4, dim3(nb, nn::NZG), dim3(256), 0, s, b.dY, n.w4f, b.A3, b.G3);
   /* layer 3: dW3 = corr(A2, G3); G2 = conv(G3, W3b) * (A2 > 0) */
@@ -1433,7 +1473,8 @@ static void netLossGrad(Net &n, NetBuffers &b, int nb, float invN)
   /* layer 1 */
   hipLaunchKernelGGL(k_nn_dw_small<false>, dim3(nb, nn::NZG), dim3(256), 0, s, n.setX, b.idx, b.G1,
                      (const uint16_t *)nullptr, (const float *)nullptr, b.part);
-  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW1 + 255) / 256), dim3(256), 0, s, b.part, nb * nn::NZG, nn::NW1,
+  hipLaunchKernelGGL(k_nn_reduce, dim3((nn::NW1 + 63) / 64), dim3(64 * kReduceWaves), 0, s, b.part, nb * nn::NZG,
+                     nn::NW1,
                      n.grad + nn::OFF_W1);
 }
 
