@@ -448,6 +448,7 @@ __global__ void __launch_bounds__(256) k_nn_dw_small(const uint16_t *__restrict_
     /* slab positions q = plane * PL + v * E + u in ascending order, ring positions skipped */
     for (int pz = 0; pz < nn::ZP; pz++)
       for (int v = 1; v < nn::E - 1; v++)
+#pragma unroll
         for (int u = 1; u < nn::E - 1; u++) {
           const int q = pz * nn::PL + v * nn::E + u;
 #pragma unroll
@@ -469,6 +470,7 @@ __global__ void __launch_bounds__(256) k_nn_dw_small(const uint16_t *__restrict_
       const int t = ent[k] < nn::NW4 ? ent[k] / nn::C : 0, ci = ent[k] % nn::C;
       ao[k] = (nn::PL + nn::tapOff(t)) * nn::C + ci;
     }
+#pragma unroll 16
     for (int q = 0; q < 512; q++) {
       const int cx = q & 15, cy = (q >> 4) & 15, cz = q >> 8;
       const float gv = dy[q];
