@@ -68,10 +68,13 @@ struct LevelArgs {
    * smoother, two float4 per cell {D, W-x, W+x, W-y}, {W+y, W-z, W+z, 0}, at
    * record off[bid] + slot * 1024 + (x + 32 y), slot = (z + x/8 + y) mod 32 */
   const float4 *pack;
-  /* the same for 16-bit operands (precision form): one uint4 per cell with the
-   * 16-bit encodings {D | W-x << 16, W-y | W+y << 16, W-z | W+x << 16,
-   * flags | W+z << 16} at record off[bid] + slot * 1024 + (x + 32 y) */
-  const uint4 *pack16;
+  /* the same for 16-bit operands (precision form): one uint2 per cell with the
+   * 16-bit encodings {D | W-x << 16, W-z | W+x << 16} at record
+   * off[bid] + slot * 1024 + (x + 32 y), and per z-column (x, y) at
+   * (off[bid] >> 5) + x + 32 y one uint2 {bit s: the column's cell at slot s
+   * lies on an all-non-fluid SIMD segment, W+z of its top cell} */
+  const uint2 *pack16;
+  const uint2 *pack16col;
   /* {W-y | W+y << 16} of each cell again, alone, in the same slot-plane order
    * (read once per local sweep by the 32^3 16-bit smoother) */
   const uint32_t *pack16y;
@@ -990,8 +993,8 @@ __device__ __forceinline__ uint32_t enc16(float v, int prec)
  * level's 16-bit encoding, with the cell flags, at the record of the
  * register slot (z + x/8 + y) mod 32. */
 __global__ void __launch_bounds__(1024) k_pack_col32h(LevelArgs a, const int *__restrict__ blocks,
-                                                      const float *__restrict__ X, uint4 *__restrict__ pack,
-                                                      uint32_t *__restrict__ packy)
+                                                      const float *__restrict__ X, uint2 *__restrict__ pack,
+                                                      uint2 *__restrict__ packcol, uint32_t *__restrict__ packy)
 {
   constexpr int B = 32, S = Cfg<B>::S, NT = Cfg<B>::NT, NF = Cfg<B>::NF;
   __shared__ float faces[6 * NF];
@@ -1010,19 +1013,21 @@ __global__ void __launch_bounds__(1024) k_pack_col32h(LevelArgs a, const int *__
   c.w6top = fkind[5] == FK_SAME ? smCoef(a, 2, foff[5] + c.tx + B * c.ty) : (fkind[5] == FK_DOMAIN ? a.bcW[2] : 1.0f);
   c.om = 0.f; c.omm = 0.f; c.HR = 1.f;
   const int p = a.prec;
+  uint32_t skip = 0, top = 0;
   for (int z = 0; z < B; z++) {
     float D, Bb, w1, w2, w3, w4, w5, w6;
     loadCoeffs<B>(c, z, D, Bb, w1, w2, w3, w4, w5, w6);
     const int64_t q = off + c.tx + B * c.ty + B * B * z;
     const int sl = (z + c.o) & (B - 1);
-    uint4 r;
-    r.x = enc16(D, p) | (enc16(w1, p) << 16);
-    r.y = enc16(w3, p) | (enc16(w4, p) << 16);
-    r.z = enc16(w5, p) | (enc16(w2, p) << 16);
-    r.w = (uint32_t)a.flags[q] | (enc16(w6, p) << 16);
-    pack[off + (int64_t)sl * NT + threadIdx.x] = r;
-    packy[off + (int64_t)sl * NT + threadIdx.x] = r.y;
+    pack[off + (int64_t)sl * NT + threadIdx.x] =
+        make_uint2(enc16(D, p) | (enc16(w1, p) << 16), enc16(w5, p) | (enc16(w2, p) << 16));
+    packy[off + (int64_t)sl * NT + threadIdx.x] = enc16(w3, p) | (enc16(w4, p) << 16);
+    /* the lanes of a SIMD segment share the slot at every z (x / 8 and y alike) */
+    const bool sk = segmentAllNonFluid<S>((a.flags[q] & (CELL_SOLID | CELL_VOID)) != 0);
+    skip |= (sk ? 1u : 0u) << sl;
+    if (z == B - 1) top = enc16(w6, p);
   }
+  packcol[(off >> 5) + threadIdx.x] = make_uint2(skip, top);
 }
 
 /* Register-resident smoother for 32^3 blocks with 16-bit operands (the
@@ -1289,33 +1294,27 @@ template <int PREC> struct C32JacobiAll<PREC, C32::B> {
  * group of loads in flight at a time keeps the prologue within the register
  * budget. */
 template <int SL, int CI>
-__device__ __forceinline__ void c32LoadRecords(C32Col &r, const __amdgpu_buffer_rsrc_t &rr, C32Ctx &c, uint16_t *wxp,
-                                               uint32_t &w6bits)
+__device__ __forceinline__ void c32LoadRecords(C32Col &r, const __amdgpu_buffer_rsrc_t &rr, C32Ctx &c, uint16_t *wxp)
 {
-  constexpr int B = C32::B, S = C32::S, NF = B * B;
+  constexpr int B = C32::B, NF = B * B, NR = 16;
   const int ty = c.ty0 + C32::YS * CI, o = c.o0 + C32::YS * CI;
-  uint4 v[8];
+  uint32_t vx[NR], vz[NR];
 #pragma unroll
-  for (int k = 0; k < 8; k++)
-    v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                                         rr, (uint32_t)(c.tx + B * ty) * 16u, (uint32_t)(SL + k) * (NF * 16u), 0));
+  for (int k = 0; k < NR; k++) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, (uint32_t)(c.tx + B * ty) * 8u, (uint32_t)(SL + k) * (NF * 8u), 0);
+    vx[k] = v[0];
+    vz[k] = v[1];
+  }
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < NR; k++) {
     const int sl = SL + k;
-    /* copies through an empty asm: the kept words get registers of their
-     * own instead of holding the whole 4-register load tuple live */
-    uint32_t vx = v[k].x, vz = v[k].z, vw = v[k].w;
-    asm volatile("" : "+v"(vx), "+v"(vz), "+v"(vw));
-    r.c0[sl] = vx;
-    if (sl & 1) r.c5[sl / 2] |= vz << 16;
-    else r.c5[sl / 2] = vz & 0xffffu;
+    r.c0[sl] = vx[k];
+    if (sl & 1) r.c5[sl / 2] |= vz[k] << 16;
+    else r.c5[sl / 2] = vz[k] & 0xffffu;
     int slv = sl;   // laundered: z is formed here, not precomputed for every slot
     asm volatile("" : "+s"(slv));
     const int z = (slv - o + 4 * B) & (B - 1);
-    if (c.tx == B - 1) wxp[z * B + ty] = (uint16_t)(vz >> 16);
-    if (z == B - 1) w6bits = vw;
-    const bool sk = segmentAllNonFluid<S>(((uint16_t)vw & (CELL_SOLID | CELL_VOID)) != 0);
-    c.skip[CI] |= (sk ? 1u : 0u) << sl;
+    if (c.tx == B - 1) wxp[z * B + ty] = (uint16_t)(vz[k] >> 16);
   }
   __builtin_amdgcn_sched_barrier(0);
 }
@@ -1395,26 +1394,22 @@ __global__ void __launch_bounds__(C32::NT) k_relax_c32h(LevelArgs a, const int *
   for (int k = 0; k < NC; k++) {
     c.fz0[k] = fv[k][4];
     c.fz1[k] = fv[k][5];
-    c.skip[k] = 0;
   }
-  /* coefficient records, then x and b */
+  /* per-column skip mask and top W+z, coefficient records, then x and b */
   C32Col r[NC];
-  uint32_t w6b[NC];
-  const __amdgpu_buffer_rsrc_t rr = blockRsrc(a.pack16 + off, B * B * B * 16);
-  w6b[0] = 0;
-  c32LoadRecords<0, 0>(r[0], rr, c, wxp, w6b[0]);
-  c32LoadRecords<8, 0>(r[0], rr, c, wxp, w6b[0]);
-  c32LoadRecords<16, 0>(r[0], rr, c, wxp, w6b[0]);
-  c32LoadRecords<24, 0>(r[0], rr, c, wxp, w6b[0]);
-  if constexpr (NC > 1) {
-    w6b[1] = 0;
-    c32LoadRecords<0, 1>(r[1], rr, c, wxp, w6b[1]);
-    c32LoadRecords<8, 1>(r[1], rr, c, wxp, w6b[1]);
-    c32LoadRecords<16, 1>(r[1], rr, c, wxp, w6b[1]);
-    c32LoadRecords<24, 1>(r[1], rr, c, wxp, w6b[1]);
-  }
 #pragma unroll
-  for (int k = 0; k < NC; k++) c.w6top[k] = hi16<PREC>(w6b[k]);
+  for (int k = 0; k < NC; k++) {
+    const uint2 cw = a.pack16col[(off >> 5) + threadIdx.x + NT * k];
+    c.skip[k] = cw.x;
+    c.w6top[k] = lo16<PREC>(cw.y);
+  }
+  const __amdgpu_buffer_rsrc_t rr = blockRsrc(a.pack16 + off, B * B * B * 8);
+  c32LoadRecords<0, 0>(r[0], rr, c, wxp);
+  c32LoadRecords<16, 0>(r[0], rr, c, wxp);
+  if constexpr (NC > 1) {
+    c32LoadRecords<0, 1>(r[1], rr, c, wxp);
+    c32LoadRecords<16, 1>(r[1], rr, c, wxp);
+  }
   c32StageX<0>(r[0], h3, c, h);
   if constexpr (NC > 1) c32StageX<1>(r[1], h3, c, h);
 #pragma unroll
@@ -1441,12 +1436,25 @@ __global__ void __launch_bounds__(C32::NT) k_relax_c32h(LevelArgs a, const int *
       for (int q = 0; q < NC; q++) c32Mirror(r[q]);
     }
   }
+  /* the fifth sweep's lane constants formed afresh from the thread index
+   * (an opaque copy) and 1/h kept uniform: otherwise they come back from
+   * spill slots, and each such reload waits for every store before it */
+  C32Ctx cj = c;
+  {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    cj.tx = t % B;
+    cj.ty0 = t / B;
+    cj.o0 = cj.tx / S + cj.ty0;
+    cj.h30 = (cj.ty0 + 1) * P + cj.tx + 1;
+    cj.HR = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(c.HR)));
+  }
 #pragma unroll
   for (int k = 0; k < C32::K; k++) {
-    r[0].wq[k] = c32LoadWy<0>(c, k);
-    if constexpr (NC > 1) r[1].wq[k] = c32LoadWy<1>(c, k);
+    r[0].wq[k] = c32LoadWy<0>(cj, k);
+    if constexpr (NC > 1) r[1].wq[k] = c32LoadWy<1>(cj, k);
   }
-  C32JacobiAll<PREC, 0>::run(r, h3, c);
+  C32JacobiAll<PREC, 0>::run(r, h3, cj);
 }
 
 __device__ __forceinline__ double blockReduceSum(double v, double *scratch)
@@ -2296,7 +2304,7 @@ struct Dev {
   /* 32^3 column smoother: coefficient records over the MG0 pool (LevelArgs::pack,
    * LevelArgs::pack16 for 16-bit operands, built for precision pack16Prec) */
   float4 *pack32 = nullptr;
-  uint4 *pack16 = nullptr;
+  uint2 *pack16 = nullptr, *pack16col = nullptr;
   uint32_t *pack16y = nullptr;
   int pack16Prec = MSBGX_PREC_FP32;
   /* FLIP caller steps: velocity channels (MG0 pool layout) and the level-0 block list */
@@ -2382,6 +2390,7 @@ void deviceFree(Grid &g)
     freeFlip(d);
     if (d.pack32) (void)hipFree(d.pack32);
     if (d.pack16) (void)hipFree(d.pack16);
+    if (d.pack16col) (void)hipFree(d.pack16col);
     if (d.pack16y) (void)hipFree(d.pack16y);
     if (BatchDev *b = d.batch) {
       f(b->cProb); f(b->cb); f(b->ce); f(b->cBeg); f(b->cEnd); f(b->states); f(b->allDone); f(b->hist);
@@ -2414,6 +2423,7 @@ static LevelArgs levelArgs(const Grid &g, int L)
   a.pack16 = (L == 0 && g.bsx0 == 32 && dd && a.prec != MSBGX_PREC_FP32 && dd->pack16Prec == a.prec) ? dd->pack16
                                                                                                       : nullptr;
   a.pack16y = a.pack16 ? dd->pack16y : nullptr;
+  a.pack16col = a.pack16 ? dd->pack16col : nullptr;
   return a;
 }
 
@@ -2478,15 +2488,17 @@ static int buildPack32(Grid &g)
   d.pack16Prec = MSBGX_PREC_FP32;
   if (a0.prec != MSBGX_PREC_FP32) {
     const size_t nc = std::max<size_t>((size_t)m.ncells, 1);
-    if (!d.pack16 && hipMalloc((void **)&d.pack16, nc * sizeof(uint4)) != hipSuccess) d.pack16 = nullptr;
+    if (!d.pack16 && hipMalloc((void **)&d.pack16, nc * sizeof(uint2)) != hipSuccess) d.pack16 = nullptr;
+    if (!d.pack16col && hipMalloc((void **)&d.pack16col, (nc / 32 + 1) * sizeof(uint2)) != hipSuccess)
+      d.pack16col = nullptr;
     if (!d.pack16y && hipMalloc((void **)&d.pack16y, nc * sizeof(uint32_t)) != hipSuccess) d.pack16y = nullptr;
-    if (!d.pack16 || !d.pack16y) {
+    if (!d.pack16 || !d.pack16col || !d.pack16y) {
       (void)hipFree(dl);
       setError("out of device memory for the 16-bit smoother records");
       return MSBGX_ENOMEM;
     }
     hipLaunchKernelGGL(k_pack_col32h, dim3((unsigned)l0.size()), dim3(1024), 0, d.stream, a0, dl, m.d_ch[C_X],
-                       d.pack16, d.pack16y);
+                       d.pack16, d.pack16col, d.pack16y);
     d.pack16Prec = a0.prec;
   }
   const hipError_t e = hipGetLastError();
@@ -2594,6 +2606,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   HIPCHK(hipStreamSynchronize(d.stream));
   if (d.pack32) { (void)hipFree(d.pack32); d.pack32 = nullptr; }   // the pool layout may have changed
   if (d.pack16) { (void)hipFree(d.pack16); d.pack16 = nullptr; }
+  if (d.pack16col) { (void)hipFree(d.pack16col); d.pack16col = nullptr; }
   if (d.pack16y) { (void)hipFree(d.pack16y); d.pack16y = nullptr; }
   d.pack16Prec = MSBGX_PREC_FP32;
   return buildPack32(g);
