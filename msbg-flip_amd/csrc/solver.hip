@@ -1320,21 +1320,30 @@ __device__ __forceinline__ void c32LoadRecords(C32Col &r, const __amdgpu_buffer_
 }
 
 /* x of column CI into the halo array (h*x, 0 on skipped segments) and b into
- * its registers. */
+ * its registers, in groups of 8 slots whose loads are all issued before the
+ * group's LDS writes (one memory round trip per group, not per slot). */
 template <int CI>
 __device__ __forceinline__ void c32StageX(C32Col &r, float *h3, const C32Ctx &c, float h)
 {
-  constexpr int B = C32::B, P = C32::P, P2 = C32::P2;
+  constexpr int B = C32::B, P = C32::P, P2 = C32::P2, G = 8;
   const int o = c.o0 + C32::YS * CI;
 #pragma unroll
-  for (int sl = 0; sl < B; sl++) {
-    int slv = sl;
-    asm volatile("" : "+s"(slv));
-    const int z = (slv - o + 4 * B) & (B - 1);
-    const uint32_t q = c32Cell(c.tx, c.ty0 + C32::YS * CI, z);
-    const float xv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rx, q, 0, 0));
-    r.bb[sl] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rb, q, 0, 0));
-    h3[c.h30 + C32::YS * CI * P + z * P2] = ((c.skip[CI] >> sl) & 1u) ? 0.0f : xv * h;
+  for (int g = 0; g < B; g += G) {
+    float xv[G];
+    int zz[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      int slv = g + k;
+      asm volatile("" : "+s"(slv));
+      zz[k] = (slv - o + 4 * B) & (B - 1);
+      const uint32_t q = c32Cell(c.tx, c.ty0 + C32::YS * CI, zz[k]);
+      xv[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rx, q, 0, 0));
+      r.bb[g + k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(c.rb, q, 0, 0));
+    }
+#pragma unroll
+    for (int k = 0; k < G; k++)
+      h3[c.h30 + C32::YS * CI * P + zz[k] * P2] = ((c.skip[CI] >> (g + k)) & 1u) ? 0.0f : xv[k] * h;
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
