@@ -908,6 +908,17 @@ struct DirSet {
 };
 constexpr int kGramBlocks = 512;
 
+/* unk[o + c] = 1 on the unknowns (not SOLID/VOID, D != 0) of PCG block
+ * blocks[w]'s cells, 0 elsewhere in the block. */
+__global__ void __launch_bounds__(256) k_nn_unk_mask(const int *__restrict__ blocks, const int64_t *__restrict__ off,
+                                                     const uint16_t *__restrict__ flags, const float *__restrict__ D,
+                                                     int ncell, uint8_t *__restrict__ unk)
+{
+  const int64_t o = off[blocks[blockIdx.x]];
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x)
+    unk[o + c] = (!(flags[o + c] & (CELL_SOLID | CELL_VOID)) && D[o + c] != 0.0f) ? 1 : 0;
+}
+
 /* The scales' coarse grids, PCG-block weights and outputs for one pass of
  * k_nn_prolong_lin_b. */
 struct ProlongScales {
@@ -917,8 +928,9 @@ struct ProlongScales {
   int f[kMaxScales], sx[kMaxScales], sy[kMaxScales], sz[kMaxScales];
 };
 
-/* k_nn_prolong_lin for NS scales at once on B^3 blocks (B = 16, 32), so the
- * flags and D are read once for all of them: thread t takes the (x, y)
+/* k_nn_prolong_lin for NS scales at once on B^3 blocks (B = 16, 32), with the
+ * unknowns from a byte mask (k_nn_unk_mask) read once for all of them
+ * instead of the flags and D: thread t takes the (x, y)
  * columns t, t + 256, ... and walks z.  The interpolation factors are per
  * axis, so per column and scale the x/y corners of each of the three z
  * layers of the neighbourhood are summed once (masked weight and weighted
@@ -929,8 +941,7 @@ template <int B, int NS>
 __global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int nbx, int nby,
                                                           const int *__restrict__ blocks,
                                                           const int64_t *__restrict__ off,
-                                                          const uint16_t *__restrict__ flags,
-                                                          const float *__restrict__ D, PcgState *st)
+                                                          const uint8_t *__restrict__ unkMask, PcgState *st)
 {
   static_assert(B * B % 256 == 0, "whole columns per thread");
   if (st && st->skipNet) return;
@@ -999,7 +1010,7 @@ __global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int 
 #pragma unroll 4
     for (int z = 0; z < B; z++) {
       const int64_t c = o + col + (int64_t)B * B * z;
-      const bool unk = !(flags[c] & (CELL_SOLID | CELL_VOID)) && D[c] != 0.0f;
+      const bool unk = unkMask[c] != 0;
 #pragma unroll
       for (int k = 0; k < NS; k++) {
         int lz;
@@ -1695,6 +1706,7 @@ struct LearnedCtx {
   } sc[kMaxScales];
   int nScales = 0;                 // scales allocated
   float *zc = nullptr, *rs = nullptr;
+  uint8_t *unk = nullptr;          // 1 on the unknowns of the PCG blocks' cells [0, n0)
   double *gram = nullptr;          // Galerkin partials
   int64_t n0 = 0;
   hipGraphExec_t graph = nullptr;
@@ -1716,6 +1728,7 @@ static void freeLearned(LearnedCtx *c)
   }
   for (float *p : {c->zc, c->rs})
     if (p) (void)hipFree(p);
+  if (c->unk) (void)hipFree(c->unk);
   if (c->gram) (void)hipFree(c->gram);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   delete c;
@@ -1793,6 +1806,12 @@ static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
     if (int rc = dalloc(&c.gram, (size_t)(kMaxScales * (kMaxScales + 3) / 2) * nPart)) return rc;
     HIPCHK(hipMemset(c.zc, 0, nc * sizeof(float)));
     HIPCHK(hipMemset(c.rs, 0, nc * sizeof(float)));
+    /* the PCG blocks' unknowns as bytes, for the prolongation */
+    const int64_t n0 = (int64_t)g.nActive0 * g.bsx0 * g.bsx0 * g.bsx0;
+    if (int rc = dalloc(&c.unk, (size_t)std::max<int64_t>(n0, 1))) return rc;
+    if (g.nActive0)
+      hipLaunchKernelGGL(k_nn_unk_mask, dim3(g.nActive0), dim3(256), 0, s, dev(g).dActive0, g.mg[0].d_off,
+                         g.mg[0].d_flags, g.mg[0].d_D, g.bsx0 * g.bsx0 * g.bsx0, c.unk);
   }
   /* PCG blocks per cell, scale by scale */
   std::vector<float> w((size_t)g.nbx * g.nby * g.nbz, 0.0f);
@@ -1891,8 +1910,7 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
       ps.sz[k] = q.sz;
     }
     auto launch = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(g.nActive0), dim3(256), 0, s, ps, g.nbx, g.nby, d.dActive0, m0.d_off, m0.d_flags,
-                         m0.d_D, st);
+      hipLaunchKernelGGL(kern, dim3(g.nActive0), dim3(256), 0, s, ps, g.nbx, g.nby, d.dActive0, m0.d_off, c.unk, st);
     };
     const int sel = (g.bsx0 == 32 ? 3 : 0) + n.scales - 1;
     switch (sel) {
