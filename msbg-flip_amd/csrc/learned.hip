@@ -762,8 +762,8 @@ __global__ void __launch_bounds__(256) k_nn_out_dense(const uint16_t *__restrict
     const int lx = q & 15, ly = (q >> 4) & 15, lz = cz0 + (q >> 8);
     const int pos = (lz + 2) * nn::PL + (ly + 2) * nn::E + (lx + 2);
     float acc = 0.0f;
-#pragma unroll
-    for (int tp = 0; tp < nn::T; tp++) {
+#pragma unroll 3
+    for (int tp = 0; tp < nn::T; tp++) {   // a few taps at a time: all 27 at once spill
       const int lp = pos + nn::tapOff(tp) - ldsOrigin;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
