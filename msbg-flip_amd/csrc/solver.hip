@@ -1691,9 +1691,10 @@ struct MulVecs {
  * z-column; each step publishes the plane's h*x (0 on all-non-fluid
  * segments) with its x/y halo ring into a double-buffered LDS plane (one
  * LDS-only barrier per step), takes the z neighbours from registers, and
- * issues the next step's loads (x and flags two planes ahead, coefficients
- * and b one plane ahead, the ring's face values one plane ahead) before it
- * computes, so that they are in flight across the barrier.  Cells, face
+ * issues later planes' loads (for one vector: coefficients, b and the
+ * ring's face values two planes ahead, x, flags and W-z three; for several,
+ * one and two) before it computes, so that they are in flight across the
+ * barriers.  Cells, face
  * values, operation order and the FP64 partial are those of k_multiply.
  *
  * NV > 1 (residual form, the coarse correction's directions): NV vectors
@@ -1787,23 +1788,40 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
   auto qx = [&](int z) { return x < B - 1 ? q0 + BB * z + 1 : (fkind[1] == FK_SAME ? foff[1] + B * y + BB * z : q0 + BB * z); };
   auto qy = [&](int z) { return y < B - 1 ? q0 + BB * z + B : (fkind[3] == FK_SAME ? foff[3] + x + BB * z : q0 + BB * z); };
 
-  /* prologue: plane 0's coefficients, x and flags of planes 0 and 1 */
-  float xr0[NV], xr1[NV];
+  /* loads run ahead of the plane being computed: coefficients, b and the
+   * ring's face values (group A) PA planes, x, flags and W-z (group B) PA + 1
+   * planes; index k of a group's arrays holds plane z + k */
+  constexpr int PA = NV == 1 ? 2 : 1, PB = PA + 1;   // deeper did not pay (3 and 4 ahead measured)
+  float aw0[PA + 1], aw1[PA + 1], add[PA + 1], aw0x[PA + 1], aw1y[PA + 1], abb[PA + 1], ar2[PA + 1];
+  float ar1[PA + 1][NV];
+  float bx[PB + 1][NV], bw2[PB + 1];
+  uint16_t bfl[PB + 1];
+  auto loadA = [&](int k, int zp) {
+    const int64_t q = q0 + BB * zp;
+    aw0[k] = a.W0[q];
+    aw1[k] = a.W1[q];
+    add[k] = a.D[q];
+    aw0x[k] = a.W0[qx(zp)];
+    aw1y[k] = a.W1[qy(zp)];
+    abb[k] = bsrc[q];
+    ringLoad(zp, ar1[k], ar2[k]);
+  };
+  auto loadB = [&](int k, int zp) {
+    const int64_t q = q0 + BB * zp;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    xr0[v] = mv.x[v][q0];
-    xr1[v] = mv.x[v][q0 + BB];
-  }
-  uint16_t fl0 = a.flags[q0], fl1 = a.flags[q0 + BB];
-  float w0 = a.W0[q0], w1 = a.W1[q0], w2 = a.W2[q0], w2n = a.W2[q0 + BB], dd = a.D[q0];
-  float w0x = a.W0[qx(0)], w1y = a.W1[qy(0)], bb = bsrc[q0];
-  float rr1[NV], rr2;
-  ringLoad(0, rr1, rr2);
-  bool sk = segmentAllNonFluid<S>((fl0 & (CELL_SOLID | CELL_VOID)) != 0);
+    for (int v = 0; v < NV; v++) bx[k][v] = mv.x[v][q];
+    bfl[k] = a.flags[q];
+    bw2[k] = a.W2[q];
+  };
+#pragma unroll
+  for (int k = 0; k < PA; k++) loadA(k, k);
+#pragma unroll
+  for (int k = 0; k < PB; k++) loadB(k, k);
+  bool sk = segmentAllNonFluid<S>((bfl[0] & (CELL_SOLID | CELL_VOID)) != 0);
   float xs[NV], xsPrev[NV];   // h*x of plane z as the stencil reads it, and of plane z-1
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    xs[v] = sk ? 0.0f : xr0[v] * h;
+    xs[v] = sk ? 0.0f : bx[0][v] * h;
     xsPrev[v] = 0.0f;
   }
   double acc[NG];
@@ -1811,56 +1829,34 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
   for (int j = 0; j < NG; j++) acc[j] = 0.0;
 #pragma unroll
   for (int z = 0; z < B; z++) {
-    /* next step's loads */
-    float xr2[NV], nr1[NV];
-    float nw0 = 0.0f, nw1 = 0.0f, nw2n = 0.0f, ndd = 0.0f, nw0x = 0.0f, nw1y = 0.0f, nbb = 0.0f;
-    float nr2 = 1.0f;
-    uint16_t fl2 = 0;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      xr2[v] = 0.0f;
-      nr1[v] = 0.0f;
-    }
-    if (z + 2 < B) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) xr2[v] = mv.x[v][q0 + BB * (z + 2)];
-      fl2 = a.flags[q0 + BB * (z + 2)];
-      nw2n = a.W2[q0 + BB * (z + 2)];
-    }
-    if (z + 1 < B) {
-      const int64_t q = q0 + BB * (z + 1);
-      nw0 = a.W0[q];
-      nw1 = a.W1[q];
-      ndd = a.D[q];
-      nw0x = a.W0[qx(z + 1)];
-      nw1y = a.W1[qy(z + 1)];
-      nbb = bsrc[q];
-      ringLoad(z + 1, nr1, nr2);
-    }
+    /* this step's loads, for planes z + PA and z + PB */
+    if (z + PA < B) loadA(PA, z + PA);
+    if (z + PB < B) loadB(PB, z + PB);
     /* h*x of plane z+1 (the +z neighbour) and its skip state */
     bool skn = false;
     float xsn[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) xsn[v] = 0.0f;
     if (z + 1 < B) {
-      skn = segmentAllNonFluid<S>((fl1 & (CELL_SOLID | CELL_VOID)) != 0);
+      skn = segmentAllNonFluid<S>((bfl[1] & (CELL_SOLID | CELL_VOID)) != 0);
 #pragma unroll
-      for (int v = 0; v < NV; v++) xsn[v] = skn ? 0.0f : xr1[v] * h;
+      for (int v = 0; v < NV; v++) xsn[v] = skn ? 0.0f : bx[1][v] * h;
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       float *p = pl[v] + (z & 1) * P2;
       p[(y + 1) * P + x + 1] = xs[v];
-      if (ring) p[rpos] = ringValue(v, z, rr1[v], rr2);
+      if (ring) p[rpos] = ringValue(v, z, ar1[0][v], ar2[0]);
     }
     ldsBarrier();
     if (!sk) {
-      const float W1 = m0 ? 1.0f : w0;
-      const float W3 = m2 ? 1.0f : w1;
-      const float W5 = (z == 0 && m4) ? 1.0f : w2;
-      const float W2 = xIn ? w0x : xOut;
-      const float W4 = yIn ? w1y : yOut;
-      const float W6 = z < B - 1 ? w2n : zOut;
+      const float dd = add[0], bb = abb[0];
+      const float W1 = m0 ? 1.0f : aw0[0];
+      const float W3 = m2 ? 1.0f : aw1[0];
+      const float W5 = (z == 0 && m4) ? 1.0f : bw2[0];
+      const float W2 = xIn ? aw0x[0] : xOut;
+      const float W4 = yIn ? aw1y[0] : yOut;
+      const float W6 = z < B - 1 ? bw2[1] : zOut;
       float res[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
@@ -1887,7 +1883,7 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
         double xv[NV], ax[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          xv[v] = xr0[v];
+          xv[v] = bx[0][v];
           ax[v] = bd - (double)res[v];
         }
         int j = 0;
@@ -1904,21 +1900,27 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
     for (int v = 0; v < NV; v++) {
       xsPrev[v] = xs[v];
       xs[v] = xsn[v];
-      xr0[v] = xr1[v];
-      xr1[v] = xr2[v];
-      rr1[v] = nr1[v];
     }
     sk = skn;
-    fl1 = fl2;
-    w0 = nw0;
-    w1 = nw1;
-    w2 = w2n;
-    w2n = nw2n;
-    dd = ndd;
-    w0x = nw0x;
-    w1y = nw1y;
-    bb = nbb;
-    rr2 = nr2;
+#pragma unroll
+    for (int k = 0; k < PA; k++) {
+      aw0[k] = aw0[k + 1];
+      aw1[k] = aw1[k + 1];
+      add[k] = add[k + 1];
+      aw0x[k] = aw0x[k + 1];
+      aw1y[k] = aw1y[k + 1];
+      abb[k] = abb[k + 1];
+      ar2[k] = ar2[k + 1];
+#pragma unroll
+      for (int v = 0; v < NV; v++) ar1[k][v] = ar1[k + 1][v];
+    }
+#pragma unroll
+    for (int k = 0; k < PB; k++) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) bx[k][v] = bx[k + 1][v];
+      bfl[k] = bfl[k + 1];
+      bw2[k] = bw2[k + 1];
+    }
   }
   if (partials) {
 #pragma unroll
