@@ -604,6 +604,7 @@ __global__ void __launch_bounds__(256) k_nn_sample(const float *__restrict__ rho
 /* a - b over [0, n) */
 __global__ void k_nn_sub(const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ out, int64_t n)
 {
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = a[i] - b[i];
 }
@@ -612,6 +613,7 @@ __global__ void k_nn_sub(const float *__restrict__ a, const float *__restrict__ 
 __global__ void k_nn_copy(const float *__restrict__ src, float *__restrict__ dst, int64_t n, const int *flag, int want)
 {
   if (*flag != want) return;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
@@ -624,6 +626,7 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_axpy(float *__restrict__ d, 
   if (st->done) return;
   const float beta = st->beta;
   const float *zz = st->usePlain ? zplain : z;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     d[i] = fmaf(beta, d[i], zz[i]);
 }
@@ -1038,6 +1041,7 @@ __global__ void __launch_bounds__(256) k_nn_gram(DirSet ds, const float *__restr
   double acc[NG + K];
 #pragma unroll
   for (int j = 0; j < NG + K; j++) acc[j] = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double ri = r[i];
     double zv[K], aq[K];
@@ -1141,6 +1145,7 @@ __global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const 
   if (st->done) return;
   const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
   if (st->skipNet) {
+#pragma unroll 4
     for (int64_t i = i0; i < n; i += di) rs[i] = r[i];
     return;
   }
@@ -1150,6 +1155,7 @@ __global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const 
     cf[a] = st->coef[a];
     cs += cf[a];
   }
+#pragma unroll 4
   for (int64_t i = i0; i < n; i += di) {
     float rv = (1.0f - cs) * r[i];
 #pragma unroll
@@ -1162,6 +1168,7 @@ __global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const 
 __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, int64_t n, const int *skip)
 {
   if (skip && *skip) return;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     z[i] += zc[i];
 }
@@ -1181,6 +1188,7 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ 
 #pragma unroll
   for (int a = 0; a < K; a++) cf[a] = K == 1 ? 1.0f : st->coef[a];
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float zv = z[i];
     if (add) {
@@ -1211,10 +1219,12 @@ __global__ void k_nn_gamma(float *__restrict__ zc, float *__restrict__ rs, const
   if (st->done) return;
   const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
   if (st->skipNet) {
+#pragma unroll 4
     for (int64_t i = i0; i < n; i += di) rs[i] = r[i];
     return;
   }
   const float gm = st->gamma;
+#pragma unroll 4
   for (int64_t i = i0; i < n; i += di) {
     zc[i] = gm * zc[i];
     rs[i] = fmaf(gm, rs[i] - r[i], r[i]);
@@ -1229,6 +1239,7 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_dot_diff(const float *__rest
   if (*skip) return;
   __shared__ double red[kVecThreads / 64];
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     acc += (double)a[i] * ((double)b[i] - (double)c[i]);
   const double t = blockReduceSum(acc, red);

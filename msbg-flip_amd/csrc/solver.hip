@@ -439,6 +439,7 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_relax64(LevelArgs a, const int *
 __global__ void k_fill64(double *__restrict__ x, int64_t n, const int *done)
 {
   if (*done) return;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] = 0.0;
 }
@@ -446,6 +447,7 @@ __global__ void k_fill64(double *__restrict__ x, int64_t n, const int *done)
 __global__ void k_x64_to_f32(const double *__restrict__ x64, float *__restrict__ x, int64_t n, const int *done)
 {
   if (*done) return;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] = (float)x64[i];
 }
@@ -2020,6 +2022,7 @@ __global__ void __launch_bounds__(kVecThreads) k_dot(const float *__restrict__ a
   if (*done) return;
   __shared__ double red[kVecThreads / 64];
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     acc += (double)a[i] * (double)b[i];
   const double s = blockReduceSum(acc, red);
@@ -2032,6 +2035,7 @@ __global__ void __launch_bounds__(kVecThreads) k_copy_norm(const float *__restri
 {
   __shared__ double red[kVecThreads / 64];
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const float v = b[i];
     r[i] = v;
@@ -2050,6 +2054,7 @@ __global__ void __launch_bounds__(kVecThreads) k_update_xr(float *__restrict__ x
   __shared__ double red[kVecThreads / 64];
   const float alpha = st->alpha;
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     x[i] = fmaf(alpha, d[i], x[i]);
     const float rv = fmaf(-alpha, q[i], r[i]);
@@ -2066,6 +2071,7 @@ __global__ void __launch_bounds__(kVecThreads) k_axpy_d(float *__restrict__ d, c
 {
   if (st->done) return;
   const float beta = st->beta;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     d[i] = fmaf(beta, d[i], z[i]);
 }
@@ -3130,6 +3136,7 @@ int deviceTimeKernel(Grid &g, int which, int reps, double *ms, double *bytes, in
 /* Round-to-nearest-even conversion of FP32 coefficients to FP16 or BF16 bits. */
 __global__ void k_to16(const float *__restrict__ src, uint16_t *__restrict__ dst, int64_t n, int prec)
 {
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const float v = src[i];
     uint16_t o;
@@ -3228,6 +3235,7 @@ __global__ void __launch_bounds__(256) k_residual64(LevelArgs a, const int *__re
 /* x64 += x (and x = (float) x64 when store) over [0, n) */
 __global__ void k_accum64(double *__restrict__ x64, float *__restrict__ x, int64_t n, int store)
 {
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     if (store) x[i] = (float)x64[i];
     else x64[i] += (double)x[i];
