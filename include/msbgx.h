@@ -81,6 +81,10 @@ typedef struct msbgx_solve_params {
                                           blocks: their residuals one at a time and the
                                           Galerkin sums in a pass of their own (the form of
                                           other block sizes and scale counts)                */
+#define MSBGX_OPT_FP32_DIRECTIONS 0x10u /* learned coarse form, two-direction pass: keep the
+                                          directions and their residuals in FP32 (default:
+                                          BF16 storage, FP32 arithmetic, A zc on the PCG
+                                          blocks only)                                       */
 
 const char *msbgx_last_error(void);
 void msbgx_default_solve_params(msbgx_solve_params *p);

@@ -903,12 +903,14 @@ __global__ void __launch_bounds__(256) k_nn_prolong_lin(const float *__restrict_
   }
 }
 
-/* Directions of the coarse form at several scales: z[a] and q[a] = r - A z[a]. */
+/* Directions of the coarse form at several scales: z[a] and q[a] = r - A z[a],
+ * stored in FP32 or (T = uint16_t) as BF16 bits. */
 constexpr int kMaxScales = 3;
-struct DirSet {
-  const float *z[kMaxScales];
-  const float *q[kMaxScales];
+template <typename T> struct DirSetT {
+  const T *z[kMaxScales];
+  const T *q[kMaxScales];
 };
+using DirSet = DirSetT<float>;
 constexpr int kGramBlocks = 512;
 
 /* unk[o + c] = 1 on the unknowns (not SOLID/VOID, D != 0) of PCG block
@@ -928,6 +930,7 @@ struct ProlongScales {
   const float *coarse[kMaxScales];
   const float *wgt[kMaxScales];
   float *z[kMaxScales];
+  uint16_t *z16[kMaxScales];   // the BF16 outputs of a T = uint16_t pass
   int f[kMaxScales], sx[kMaxScales], sy[kMaxScales], sz[kMaxScales];
 };
 
@@ -939,8 +942,9 @@ struct ProlongScales {
  * layers of the neighbourhood are summed once (masked weight and weighted
  * value); a cell then blends the two layers around it and divides.  The same
  * weights and mask as k_nn_prolong_lin, summed in another grouping (the
- * results agree to rounding). */
-template <int B, int NS>
+ * results agree to rounding).  T = uint16_t: each value is stored at
+ * ps.z16[k] as BF16 bits (rounded to nearest even) instead. */
+template <int B, int NS, typename T = float>
 __global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int nbx, int nby,
                                                           const int *__restrict__ blocks,
                                                           const int64_t *__restrict__ off,
@@ -1022,7 +1026,9 @@ __global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int 
         const float w0 = SW[k][0], w1 = SW[k][1], w2 = SW[k][2], v0 = SV[k][0], v1 = SV[k][1], v2 = SV[k][2];
         const float swl = lz ? w1 : w0, swh = lz ? w2 : w1, svl = lz ? v1 : v0, svh = lz ? v2 : v1;
         const float sw = fmaf(tz, swh, (1.0f - tz) * swl), sv = fmaf(tz, svh, (1.0f - tz) * svl);
-        ps.z[k][c] = unk ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
+        const float zv = unk ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
+        if constexpr (std::is_same_v<T, float>) ps.z[k][c] = zv;
+        else ps.z16[k][c] = bf16Bits(zv);
       }
     }
   }
@@ -1135,12 +1141,13 @@ __global__ void __launch_bounds__(256) k_nn_gram_solve(const double *partials, i
   for (int a = 0; a < K; a++) st->coef[a] = isfinite(c[a]) ? (float)c[a] : 0.0f;
 }
 
-/* rs = r - A zc = (1 - sum_a c_a) r + sum_a c_a q_a over the MG0 pool for
+/* rs = r - A zc = (1 - sum_a c_a) r + sum_a c_a q_a over [0, nDir) for
  * zc = sum_a c_a z_a, which k_nn_add_dot forms where it adds it (q[0] may be
- * rs); with the network off, rs = r exactly (see k_nn_gamma). */
-template <int K>
-__global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const float *__restrict__ r, int64_t n,
-                                                    const PcgState *st)
+ * rs), and rs = r over [nDir, n) (the MG0 pool has size n); with the
+ * network off, rs = r exactly (see k_nn_gamma). */
+template <int K, typename T = float>
+__global__ void __launch_bounds__(256) k_nn_combine(DirSetT<T> ds, float *rs, const float *__restrict__ r, int64_t n,
+                                                    int64_t nDir, const PcgState *st)
 {
   if (st->done) return;
   const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, di = (int64_t)gridDim.x * blockDim.x;
@@ -1156,12 +1163,13 @@ __global__ void __launch_bounds__(256) k_nn_combine(DirSet ds, float *rs, const 
     cs += cf[a];
   }
 #pragma unroll 4
-  for (int64_t i = i0; i < n; i += di) {
+  for (int64_t i = i0; i < nDir; i += di) {
     float rv = (1.0f - cs) * r[i];
 #pragma unroll
-    for (int a = 0; a < K; a++) rv = fmaf(cf[a], ds.q[a][i], rv);
+    for (int a = 0; a < K; a++) rv = fmaf(cf[a], vload(ds.q[a], i), rv);
     rs[i] = rv;
   }
+  for (int64_t i = nDir + i0; i < n; i += di) rs[i] = r[i];
 }
 
 /* z += zc over [0, n) unless *skip */
@@ -1176,8 +1184,8 @@ __global__ void k_nn_add(float *__restrict__ z, const float *__restrict__ zc, in
 /* z += zc unless st->skipNet, then (unless st->done) the partial <r, z> of
  * the result, its products summed in k_dot's order.  zc: the scaled
  * direction ds.z[0] (K = 1, k_nn_gamma), else sum_a c_a z_a formed here. */
-template <int K>
-__global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ z, DirSet ds,
+template <int K, typename T = float>
+__global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ z, DirSetT<T> ds,
                                                             const float *__restrict__ r, int64_t n,
                                                             const PcgState *st, double *partials)
 {
@@ -1194,11 +1202,11 @@ __global__ void __launch_bounds__(kVecThreads) k_nn_add_dot(float *__restrict__ 
     if (add) {
       float zc;
       if constexpr (K == 1) {
-        zc = ds.z[0][i];
+        zc = vload(ds.z[0], i);
       } else {
         zc = 0.0f;
 #pragma unroll
-        for (int a = 0; a < K; a++) zc = fmaf(cf[a], ds.z[a][i], zc);
+        for (int a = 0; a < K; a++) zc = fmaf(cf[a], vload(ds.z[a], i), zc);
       }
       zv += zc;
       z[i] = zv;
@@ -1718,6 +1726,9 @@ struct LearnedCtx {
   int nScales = 0;                 // scales allocated
   float *zc = nullptr, *rs = nullptr;
   uint8_t *unk = nullptr;          // 1 on the unknowns of the PCG blocks' cells [0, n0)
+  /* two directions on 32^3 level-0 blocks: z_0, z_1, q_0, q_1 as BF16 bits,
+   * each over the MG0 pool (written on [0, n0), 0 beyond) */
+  uint16_t *dir16 = nullptr;
   double *gram = nullptr;          // Galerkin partials
   int64_t n0 = 0;
   hipGraphExec_t graph = nullptr;
@@ -1740,6 +1751,7 @@ static void freeLearned(LearnedCtx *c)
   for (float *p : {c->zc, c->rs})
     if (p) (void)hipFree(p);
   if (c->unk) (void)hipFree(c->unk);
+  if (c->dir16) (void)hipFree(c->dir16);
   if (c->gram) (void)hipFree(c->gram);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   delete c;
@@ -1763,7 +1775,7 @@ static void netCorrect(Grid &g, Net &n, LearnedCtx &c, hipStream_t s)
 
 static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s);
 static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s,
-                          bool meanDone = false);
+                          bool meanDone = false, bool dir16 = false);
 
 /* Per-kernel entry point on the level-0 PCG blocks: fine form
  * ch_out += N(ch_in); coarse form ch_out += sum over the net's scales of
@@ -1804,9 +1816,27 @@ int netApply(Net &n, Grid &g, int chIn, int chOut)
 /* Buffers of the coarse form on grid g for the first `scales` scales: the
  * dense grids, their tiles and PCG block counts, the directions and the
  * residuals they leave, zc and rs. */
+/* Whether the two-direction pass may keep its directions in BF16: 32^3
+ * level-0 blocks that are all PCG blocks (the fused residual pass's case). */
+static bool coarseDir16Shape(const Grid &g)
+{
+  return g.bsx0 == 32 && g.mg[0].dAct[0].n == g.nActive0 && g.nActive0 > 0;
+}
+/* Whether this iteration's K directions go through BF16 storage. */
+static bool coarseDir16(const Grid &g, const LearnedCtx &c, int K)
+{
+  return K == 2 && c.dir16 && coarseDir16Shape(g) &&
+         !(g.options & (MSBGX_OPT_UNFUSED_COARSE | MSBGX_OPT_FP32_DIRECTIONS));
+}
+
 static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
 {
   const size_t nc = (size_t)g.mg[0].ncells;
+  if (scales >= 2 && !c.dir16 && coarseDir16Shape(g)) {
+    HIPCHK(hipStreamSynchronize(s));
+    if (int rc = dalloc(&c.dir16, 4 * nc)) return rc;
+    HIPCHK(hipMemset(c.dir16, 0, 4 * nc * sizeof(uint16_t)));
+  }
   if (c.nScales >= scales) return MSBGX_OK;
   HIPCHK(hipStreamSynchronize(s));
   if (!c.zc) {
@@ -1880,7 +1910,7 @@ static int coarseSetup(Grid &g, LearnedCtx &c, int scales, hipStream_t s)
  * place already (k_update_xr_rz_mean; the grid's other cells are still 0
  * from the first call). */
 static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgState *st, hipStream_t s,
-                          bool meanDone)
+                          bool meanDone, bool dir16)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
@@ -1923,6 +1953,11 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
     auto launch = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(g.nActive0), dim3(256), 0, s, ps, g.nbx, g.nby, d.dActive0, m0.d_off, c.unk, st);
     };
+    if (dir16) {   // the two-direction pass's BF16 directions (coarseDir16)
+      for (int k = 0; k < 2; k++) ps.z16[k] = c.dir16 + (size_t)k * m0.ncells;
+      launch(k_nn_prolong_lin_b<32, 2, uint16_t>);
+      return;
+    }
     const int sel = (g.bsx0 == 32 ? 3 : 0) + n.scales - 1;
     switch (sel) {
       case 0: launch(k_nn_prolong_lin_b<16, 1>); break;
@@ -1949,11 +1984,32 @@ static void coarseNetwork(Grid &g, Net &n, LearnedCtx &c, const float *src, PcgS
  * and the Galerkin sums (for two directions on 32^3 level-0 blocks in the
  * same pass), then zc
  * and rs = r - A zc with the coefficients that minimise the A-norm error
- * over their span. */
+ * over their span.  With BF16 directions (coarseDir16) the directions and
+ * their residuals are stored as BF16 bits, and A zc is taken on the PCG
+ * blocks alone: rs = r on the rest of the MG0 pool. */
 template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s)
 {
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
+  if constexpr (K == 2) {
+    if (coarseDir16(g, c, K)) {
+      const size_t nc = (size_t)m0.ncells;
+      MulVecs<K, uint16_t> mv{};
+      DirSetT<uint16_t> ds{};
+      for (int a = 0; a < K; a++) {
+        mv.x[a] = ds.z[a] = c.dir16 + (size_t)a * nc;
+        mv.y[a] = c.dir16 + (size_t)(K + a) * nc;
+        ds.q[a] = mv.y[a];
+      }
+      const DevList &l0 = m0.dAct[0];
+      hipLaunchKernelGGL((k_multiply_z<32, K, uint16_t>), dim3(l0.n), dim3(32 * 32), 0, s, levelArgs(g, 0), l0.ptr,
+                         mv, m0.d_ch[C_R], 1, c.gram, 0, &d.state->skipNet);
+      hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, l0.n, d.state);
+      hipLaunchKernelGGL((k_nn_combine<K, uint16_t>), dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.rs,
+                         m0.d_ch[C_R], (int64_t)m0.ncells, c.n0, d.state);
+      return;
+    }
+  }
   DirSet ds{};
   for (int a = 0; a < K; a++) {
     ds.z[a] = c.sc[a].z;
@@ -1995,7 +2051,7 @@ template <int K> static void coarseCombine(Grid &g, LearnedCtx &c, hipStream_t s
     hipLaunchKernelGGL(k_nn_gram_solve<K>, dim3(1), dim3(256), 0, s, c.gram, kGramBlocks, d.state);
   }
   hipLaunchKernelGGL(k_nn_combine<K>, dim3(kVecBlocks), dim3(kVecThreads), 0, s, ds, c.rs, m0.d_ch[C_R],
-                     (int64_t)m0.ncells, d.state);
+                     (int64_t)m0.ncells, (int64_t)m0.ncells, d.state);
 }
 
 /* Coarse form, learned path (skipped while skipNet): directions
@@ -2008,7 +2064,8 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   Dev &d = dev(g);
   MgLevel &m0 = g.mg[0];
   const int *skip = &d.state->skipNet;
-  coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s, meanDone);
+  const bool dir16 = coarseDir16(g, c, n.scales);
+  coarseNetwork(g, n, c, m0.d_ch[C_R], d.state, s, meanDone, dir16);
   if (n.scales == 1) {
     multiplyLaunch(g, 0, true, c.sc[0].z, m0.d_ch[C_R], c.sc[0].q, nullptr, skip, s);
     /* Galerkin step along zc: gamma = <zc, r> / <zc, A zc>, A zc = r - rs */
@@ -2028,6 +2085,13 @@ static void coarseLearnedPath(Grid &g, Net &n, LearnedCtx &c, const msbgx_solve_
   zeroActive(g, 0, m0.d_ch[C_Z], &d.state->done, s);
   vcycleLaunch(g, 0, p.n_pre, p.n_post, p.n_coarse, &d.state->done, s);
   std::swap(m0.d_ch[C_R], c.rs);
+  if (dir16) {
+    DirSetT<uint16_t> ds16{};
+    for (int a = 0; a < 2; a++) ds16.z[a] = c.dir16 + (size_t)a * m0.ncells;
+    hipLaunchKernelGGL((k_nn_add_dot<2, uint16_t>), dim3(kVecBlocks), dim3(kVecThreads), 0, s, m0.d_ch[C_Z], ds16,
+                       m0.d_ch[C_R], c.n0, d.state, d.partials);
+    return;
+  }
   DirSet ds{};
   for (int a = 0; a < n.scales; a++) ds.z[a] = c.sc[a].z;
   switch (n.scales) {

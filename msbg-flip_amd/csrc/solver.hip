@@ -85,6 +85,18 @@ __device__ __forceinline__ float fromBits16(uint16_t v, int prec)
   if (prec == MSBGX_PREC_FP16) return (float)__builtin_bit_cast(_Float16, v);
   return __uint_as_float((uint32_t)v << 16);
 }
+/* BF16 bits of v, rounded to nearest even; NaN stays NaN */
+__device__ __forceinline__ uint16_t bf16Bits(float v)
+{
+  const uint32_t b = __float_as_uint(v);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+  return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+/* element i of a vector stored in FP32 or as BF16 bits */
+__device__ __forceinline__ float vload(const float *p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float vload(const uint16_t *p, int64_t i) { return __uint_as_float((uint32_t)p[i] << 16); }
+__device__ __forceinline__ void vstore(float *p, int64_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void vstore(uint16_t *p, int64_t i, float v) { p[i] = bf16Bits(v); }
 /* smoother coefficient k with the operand precision P fixed at compile time */
 template <int P> __device__ __forceinline__ float smCoefP(const LevelArgs &a, int k, int64_t q)
 {
@@ -1636,8 +1648,8 @@ __global__ void __launch_bounds__(Cfg<B>::NT) k_multiply(LevelArgs a, const int 
 /* Halo value of face `face` (kind, neighbour offset off2) at face cell
  * (u, w): the value stageFaces writes at faces[face][u + B w], by the same
  * float operations. */
-template <int B>
-__device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int64_t off, const float *__restrict__ X,
+template <int B, typename T>
+__device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int64_t off, const T *__restrict__ X,
                                                int face, int kind, int64_t off2, int u, int w)
 {
   const float a13 = (float)(1.0 * (1. / 3.));
@@ -1648,7 +1660,7 @@ __device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int6
   if (kind == FK_SAME) {
     const int e = side ? 0 : B - 1;
     const int src = dir == 0 ? e + B * u + B * B * w : (dir == 1 ? u + B * e + B * B * w : u + B * w + B * B * e);
-    return X[off2 + src] * h;
+    return vload(X, off2 + src) * h;
   }
   if (kind == FK_COARSER) {
     constexpr int Bc = B / 2 > 0 ? B / 2 : 1;
@@ -1661,7 +1673,7 @@ __device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int6
       const int own = dir == 0 ? B * u + B * B * w : (dir == 1 ? u + B * B * w : u + B * w);
       aa *= Wd[dir][off + own];
     }
-    const double sv = (double)(aa * X[off2 + src]);
+    const double sv = (double)(aa * vload(X, off2 + src));
     return (float)(sv * (double)h);
   }
   constexpr int Bf = 2 * B;
@@ -1675,17 +1687,17 @@ __device__ __forceinline__ float faceCellValue(const LevelArgs &a, float h, int6
                              : (dir == 1 ? fu + Bf * e + Bf * Bf * fw : fu + Bf * fw + Bf * Bf * e);
     float aa = a13;
     if (side) aa *= Wd[dir][off2 + src];
-    sv += (double)(aa * X[off2 + src]);
+    sv += (double)(aa * vload(X, off2 + src));
   }
   return (float)(sv * (double)h);
 }
 
 /* Vectors of one k_multiply_z launch: y_v = A x_v (or r_v = b - A x_v) for
  * v < NV, sharing the coefficient, flag and b loads. */
-template <int NV>
+template <int NV, typename T = float>
 struct MulVecs {
-  const float *x[NV];
-  float *y[NV];
+  const T *x[NV];
+  T *y[NV];
 };
 
 /* y = A x (or r = b - A x) for B = 32: one workgroup of B*B threads per
@@ -1702,9 +1714,12 @@ struct MulVecs {
  * NV > 1 (residual form, the coarse correction's directions): NV vectors
  * in one pass, and instead of the <x, y> partial the Galerkin partials of
  * level-0 blocks in FP64, for a <= b <x_a, b - y_b> (= <x_a, A x_b>) then
- * <x_a, b>, value j at partials[j * gridDim.x + list position]. */
-template <int B, int NV>
-__global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__restrict__ blocks, MulVecs<NV> mv,
+ * <x_a, b>, value j at partials[j * gridDim.x + list position].
+ * T = uint16_t (residual form only): x and y held as BF16 bits, x widened to
+ * FP32 as it is read and y rounded to nearest even as it is stored; the
+ * arithmetic and the Galerkin sums are those of the FP32 form on the stored x. */
+template <int B, int NV, typename T = float>
+__global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__restrict__ blocks, MulVecs<NV, T> mv,
                                                       const float *__restrict__ Bv, int residual, double *partials,
                                                       int partialBase, const int *done)
 {
@@ -1730,7 +1745,9 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
     fkind[face] = l2 < 0 ? FK_DOMAIN : (l2 == lev ? FK_SAME : (l2 > lev ? FK_COARSER : FK_FINER));
     foff[face] = l2 < 0 ? 0 : nd.off[face];
   }
-  const float *bsrc = residual ? Bv : mv.x[0];   // x stands in when b is not read
+  /* x stands in when b is not read; BF16 vectors are always in residual form */
+  const float *bsrc = Bv;
+  if constexpr (std::is_same_v<T, float>) bsrc = residual ? Bv : mv.x[0];
   const int64_t q0 = off + x + B * y;
   /* ring role: threads 0 .. 4B-1 load the x/y halo of each plane, face
    * rf = t / B at face cell (ri, z) */
@@ -1753,14 +1770,14 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
       const int e = rside ? 0 : B - 1;
       const int src = rdir == 0 ? e + B * ri + BB * z : ri + B * e + BB * z;
 #pragma unroll
-      for (int v = 0; v < NV; v++) r1[v] = mv.x[v][roff + src];
+      for (int v = 0; v < NV; v++) r1[v] = vload(mv.x[v], roff + src);
     } else if (rkind == FK_COARSER) {
       constexpr int Bc = B / 2;
       const int e = rside ? 0 : Bc - 1;
       const int cu = ri / 2, cw = z / 2;
       const int src = rdir == 0 ? e + Bc * cu + Bc * Bc * cw : cu + Bc * e + Bc * Bc * cw;
 #pragma unroll
-      for (int v = 0; v < NV; v++) r1[v] = mv.x[v][roff + src];
+      for (int v = 0; v < NV; v++) r1[v] = vload(mv.x[v], roff + src);
       if (!rside) r2 = rW[off + (rdir == 0 ? B * ri + BB * z : ri + BB * z)];
     }
   };
@@ -1811,7 +1828,7 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
   auto loadB = [&](int k, int zp) {
     const int64_t q = q0 + BB * zp;
 #pragma unroll
-    for (int v = 0; v < NV; v++) bx[k][v] = mv.x[v][q];
+    for (int v = 0; v < NV; v++) bx[k][v] = vload(mv.x[v], q);
     bfl[k] = a.flags[q];
     bw2[k] = a.W2[q];
   };
@@ -1875,7 +1892,7 @@ __global__ void __launch_bounds__(B * B) k_multiply_z(LevelArgs a, const int *__
         s = fmaf(F6, W6, s);
         float r = dd != 0.0f ? fmaf(1.0f / dd, F0, -s) : 0.0f;
         if (residual) r = bb - r;
-        mv.y[v][q0 + BB * z] = r;
+        vstore(mv.y[v], q0 + BB * z, r);
         res[v] = r;
         if constexpr (NV == 1) acc[0] += (double)F0 * (double)r;
       }
@@ -3140,13 +3157,8 @@ __global__ void k_to16(const float *__restrict__ src, uint16_t *__restrict__ dst
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const float v = src[i];
     uint16_t o;
-    if (prec == MSBGX_PREC_FP16) {
-      o = __builtin_bit_cast(uint16_t, (_Float16)v);   // IEEE round-to-nearest-even
-    } else {
-      uint32_t b = __float_as_uint(v);
-      if ((b & 0x7fffffffu) > 0x7f800000u) o = (uint16_t)((b >> 16) | 0x40u);   // NaN stays NaN
-      else o = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
-    }
+    if (prec == MSBGX_PREC_FP16) o = __builtin_bit_cast(uint16_t, (_Float16)v);   // IEEE round-to-nearest-even
+    else o = bf16Bits(v);
     dst[i] = o;
   }
 }

@@ -27,6 +27,7 @@ CH_PRESSURE, CH_DIVERGENCE, CH_R, CH_Z, CH_D, CH_Q = 0, 1, 2, 3, 4, 5
 CH_FACE_U, CH_FACE_V, CH_FACE_W, CH_DIAGONAL, CH_FLAGS = 7, 8, 9, 10, 11
 DBC_OPEN, DBC_SOLID = 0, 1
 OPT_LITERAL_VCYCLE, OPT_NO_GRAPH, OPT_NO_COMPACT, OPT_UNFUSED_COARSE = 0x1, 0x2, 0x4, 0x8
+OPT_FP32_DIRECTIONS = 0x10
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
 CELL_SOLID, CELL_VOID = 0x0001, 0x1000
 BLK_COARSE_FINE, BLK_FINE_COARSE, BLK_NO_FLUID, BLK_EXISTS = 1 << 2, 1 << 3, 1 << 5, 1 << 12

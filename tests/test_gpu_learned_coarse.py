@@ -235,7 +235,8 @@ def test_two_direction_pass_matches_separate_passes():
     directions and accumulates the Galerkin sums gives the iterates of the
     separate residual passes and Galerkin pass (MSBGX_OPT_UNFUSED_COARSE) --
     the residuals are the same cell by cell and the FP64 sums differ only in
-    their order, so a wrong or missing Galerkin term would show at once."""
+    their order, so a wrong or missing Galerkin term would show at once.  The
+    fused pass keeps FP32 directions here (MSBGX_OPT_FP32_DIRECTIONS)."""
     cfg = msbgx.config_c5(n=256)
     print(f"two-direction pass {cfg.name} n={cfg.n}", flush=True)
     g = msbgx.build_grid(cfg)
@@ -246,7 +247,7 @@ def test_two_direction_pass_matches_separate_passes():
     net.scales = 2
     net.set_stall(0, 0.0)
     runs = []
-    for opts in (0, msbgx.OPT_UNFUSED_COARSE):
+    for opts in (msbgx.OPT_FP32_DIRECTIONS, msbgx.OPT_UNFUSED_COARSE):
         g.set_options(opts)
         g.zero_channel(msbgx.CH_PRESSURE, 0)
         it, rr, fb = net.solve(g, max_iter=40, tol=0.0)
@@ -257,6 +258,44 @@ def test_two_direction_pass_matches_separate_passes():
     assert it_a == it_b == 40 and fb_a == fb_b == -1
     np.testing.assert_allclose(h_a, h_b, rtol=1e-4)
     assert rel(x_a, x_b) <= 1e-4
+    g.destroy()
+
+
+@pytest.mark.gpu
+def test_bf16_directions_follow_fp32_directions():
+    """Two scales on 32^3 blocks with a briefly trained network: the default
+    BF16 storage of the directions and their residuals (FP32 arithmetic, A zc
+    on the PCG blocks) follows the FP32-direction solve -- the same early
+    residuals to rounding, iteration counts within 10 % -- and both keep the
+    network's gain over plain MG-PCG, which a misplaced or mis-rounded
+    direction would lose."""
+    cfg = msbgx.config_c5(n=256)
+    print(f"BF16 directions {cfg.name} n={cfg.n}", flush=True)
+    g = msbgx.build_grid(cfg)
+    g.prepare_pressure_solve_flip(scene=cfg.scene)
+    g.set_precision(msbgx.PREC_FP16)
+    vc = {"n_pre": 1, "n_post": 1}
+    it_p, _ = g.solve_pressure_flip(max_iter=1000, tol=1e-6, **vc)
+    net = msbgx.LearnedNet(seed=3, mode=msbgx.NET_COARSE)
+    net.prolongation = msbgx.PROLONG_LINEAR
+    net.scales = 2
+    net.collect(g, max_pairs=200, seed=5, tol=1e-7, max_iter=1000)
+    net.train(600, batch=32, lr=1e-3, seed=1)
+    net.set_stall(0, 0.0)
+    runs = {}
+    for name, opts in (("fp32", msbgx.OPT_FP32_DIRECTIONS), ("bf16", 0)):
+        g.set_options(opts)
+        g.zero_channel(msbgx.CH_PRESSURE, 0)
+        it, rr, fb = net.solve(g, max_iter=1000, tol=1e-6, **vc)
+        runs[name] = (it, rr, fb, np.array(g.residual_history()))
+        print(f"  {name} directions: {it} iterations rel {rr:.2e} fallback {fb} (plain {it_p})", flush=True)
+    g.set_options(0)
+    (it32, rr32, fb32, h32), (it16, rr16, fb16, h16) = runs["fp32"], runs["bf16"]
+    assert rr32 <= 1e-6 and rr16 <= 1e-6 and fb32 == fb16 == -1
+    assert it32 < it_p and it16 < it_p
+    assert abs(it16 - it32) <= max(2, it32 // 10)
+    np.testing.assert_allclose(h16[:4], h32[:4], rtol=2e-2)
+    net.destroy()
     g.destroy()
 
 
