@@ -1985,7 +1985,11 @@ __global__ void k_restrict(const int *__restrict__ blocks, const int *__restrict
   }
 }
 
-/* x_L += P x_{L+1} on non-SOLID/VOID cells. */
+/* x_L += P x_{L+1} on non-SOLID/VOID cells.  The block edge is a power of
+ * two, so the cell coordinates are shifts and masks; each cell's flag, x_L
+ * and coarse value are loaded together (the coarse index stays inside the
+ * coarse block for every cell) and only the add depends on the flag, so
+ * several cells' loads are in flight at once. */
 __global__ void k_prolong(const int *__restrict__ blocks, const int *__restrict__ level0, const int *__restrict__ levF,
                           const int64_t *__restrict__ offF, const int64_t *__restrict__ offC,
                           const uint16_t *__restrict__ flagsF, int Lf, int bsx0, float *__restrict__ xf,
@@ -1994,19 +1998,17 @@ __global__ void k_prolong(const int *__restrict__ blocks, const int *__restrict_
   if (*done) return;
   const int bid = blocks[xcdListIndex()];
   const int lev = levF[bid];
-  const int b = bsx0 >> lev, n = b * b * b;
+  const int b = bsx0 >> lev, lg = __ffs(b) - 1, n = b * b * b, m = b - 1;
   const int64_t of = offF[bid], oc = offC[bid];
   const bool same = level0[bid] >= Lf + 1;
   const int bc = b / 2;
+#pragma unroll 4
   for (int v = threadIdx.x; v < n; v += blockDim.x) {
-    if (flagsF[of + v] & (CELL_SOLID | CELL_VOID)) continue;
-    float e;
-    if (same) e = xc[oc + v];
-    else {
-      const int vx = v % b, vy = (v / b) % b, vz = v / (b * b);
-      e = xc[oc + (vx >> 1) + bc * ((vy >> 1) + bc * (vz >> 1))];
-    }
-    xf[of + v] = xf[of + v] + e;
+    const uint16_t fl = flagsF[of + v];
+    const int vx = v & m, vy = (v >> lg) & m, vz = v >> (2 * lg);
+    const float e = xc[oc + (same ? v : (vx >> 1) + bc * ((vy >> 1) + bc * (vz >> 1)))];
+    const float xv = xf[of + v];
+    if (!(fl & (CELL_SOLID | CELL_VOID))) xf[of + v] = xv + e;
   }
 }
 
@@ -2017,6 +2019,7 @@ __global__ void k_zero_blocks(const int *__restrict__ blocks, const int *__restr
   const int bid = blocks[xcdListIndex()];
   const int b = bsx0 >> lev[bid], n = b * b * b;
   const int64_t o = off[bid];
+#pragma unroll 4
   for (int v = threadIdx.x; v < n; v += blockDim.x) x[o + v] = 0.0f;
 }
 
@@ -2027,6 +2030,7 @@ __global__ void k_copy_blocks(const int *__restrict__ blocks, const int *__restr
   const int bid = blocks[xcdListIndex()];
   const int b = bsx0 >> lev[bid], n = b * b * b;
   const int64_t o = off[bid];
+#pragma unroll 4
   for (int v = threadIdx.x; v < n; v += blockDim.x) d[o + v] = s[o + v];
 }
 
