@@ -991,44 +991,100 @@ __global__ void __launch_bounds__(256) k_nn_prolong_lin_b(ProlongScales ps, int 
     lo = u < 0.0f ? 0 : 1;
     t = u < 0.0f ? u + 1.0f : u;
   };
-  for (int col = threadIdx.x; col < B * B; col += 256) {
-    const int x = col % B, y = col / B;
-    float SW[NS][3], SV[NS][3];
-#pragma unroll
-    for (int k = 0; k < NS; k++) {
-      int lx, ly;
-      float tx, ty;
-      axis(k, 0, x, lx, tx);
-      axis(k, 1, y, ly, ty);
-#pragma unroll
-      for (int iz = 0; iz < 3; iz++) {
-        float sw = 0.0f, sv = 0.0f;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const float w = ((e & 1) ? tx : 1.0f - tx) * ((e & 2) ? ty : 1.0f - ty);
-          const int q = lx + (e & 1) + 3 * (ly + ((e >> 1) & 1)) + 9 * iz;
-          sw = fmaf(w, cm[k][q], sw);
-          sv = fmaf(w * cm[k][q], cv[k][q], sv);
+  if constexpr (std::is_same_v<T, float>) {
+    for (int col = threadIdx.x; col < B * B; col += 256) {
+      const int x = col % B, y = col / B;
+      float SW[NS][3], SV[NS][3];
+  #pragma unroll
+      for (int k = 0; k < NS; k++) {
+        int lx, ly;
+        float tx, ty;
+        axis(k, 0, x, lx, tx);
+        axis(k, 1, y, ly, ty);
+  #pragma unroll
+        for (int iz = 0; iz < 3; iz++) {
+          float sw = 0.0f, sv = 0.0f;
+  #pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const float w = ((e & 1) ? tx : 1.0f - tx) * ((e & 2) ? ty : 1.0f - ty);
+            const int q = lx + (e & 1) + 3 * (ly + ((e >> 1) & 1)) + 9 * iz;
+            sw = fmaf(w, cm[k][q], sw);
+            sv = fmaf(w * cm[k][q], cv[k][q], sv);
+          }
+          SW[k][iz] = sw;
+          SV[k][iz] = sv;
         }
-        SW[k][iz] = sw;
-        SV[k][iz] = sv;
+      }
+  #pragma unroll 4
+      for (int z = 0; z < B; z++) {
+        const int64_t c = o + col + (int64_t)B * B * z;
+        const bool unk = unkMask[c] != 0;
+  #pragma unroll
+        for (int k = 0; k < NS; k++) {
+          int lz;
+          float tz;
+          axis(k, 2, z, lz, tz);
+          const float w0 = SW[k][0], w1 = SW[k][1], w2 = SW[k][2], v0 = SV[k][0], v1 = SV[k][1], v2 = SV[k][2];
+          const float swl = lz ? w1 : w0, swh = lz ? w2 : w1, svl = lz ? v1 : v0, svh = lz ? v2 : v1;
+          const float sw = fmaf(tz, swh, (1.0f - tz) * swl), sv = fmaf(tz, svh, (1.0f - tz) * svl);
+          const float zv = unk ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
+          ps.z[k][c] = zv;
+        }
       }
     }
-#pragma unroll 4
-    for (int z = 0; z < B; z++) {
-      const int64_t c = o + col + (int64_t)B * B * z;
-      const bool unk = unkMask[c] != 0;
+  } else {
+    /* BF16 outputs: two adjacent columns per thread, each scale's two values
+     * stored as one 32-bit word (column pairs start at even cells) */
+    for (int col = 2 * threadIdx.x; col < B * B; col += 2 * 256) {
+      const int x0 = col % B, y = col / B;
+      float SW[NS][2][3], SV[NS][2][3];
 #pragma unroll
       for (int k = 0; k < NS; k++) {
-        int lz;
-        float tz;
-        axis(k, 2, z, lz, tz);
-        const float w0 = SW[k][0], w1 = SW[k][1], w2 = SW[k][2], v0 = SV[k][0], v1 = SV[k][1], v2 = SV[k][2];
-        const float swl = lz ? w1 : w0, swh = lz ? w2 : w1, svl = lz ? v1 : v0, svh = lz ? v2 : v1;
-        const float sw = fmaf(tz, swh, (1.0f - tz) * swl), sv = fmaf(tz, svh, (1.0f - tz) * svl);
-        const float zv = unk ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
-        if constexpr (std::is_same_v<T, float>) ps.z[k][c] = zv;
-        else ps.z16[k][c] = bf16Bits(zv);
+        int ly;
+        float ty;
+        axis(k, 1, y, ly, ty);
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+          int lx;
+          float tx;
+          axis(k, 0, x0 + p, lx, tx);
+#pragma unroll
+          for (int iz = 0; iz < 3; iz++) {
+            float sw = 0.0f, sv = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              const float w = ((e & 1) ? tx : 1.0f - tx) * ((e & 2) ? ty : 1.0f - ty);
+              const int q = lx + (e & 1) + 3 * (ly + ((e >> 1) & 1)) + 9 * iz;
+              sw = fmaf(w, cm[k][q], sw);
+              sv = fmaf(w * cm[k][q], cv[k][q], sv);
+            }
+            SW[k][p][iz] = sw;
+            SV[k][p][iz] = sv;
+          }
+        }
+      }
+#pragma unroll 4
+      for (int z = 0; z < B; z++) {
+        const int64_t c = o + col + (int64_t)B * B * z;
+        const uint16_t u2 = *reinterpret_cast<const uint16_t *>(unkMask + c);
+        const bool unk[2] = {(u2 & 0xffu) != 0, (u2 >> 8) != 0};
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+          int lz;
+          float tz;
+          axis(k, 2, z, lz, tz);
+          uint32_t word = 0;
+#pragma unroll
+          for (int p = 0; p < 2; p++) {
+            const float w0 = SW[k][p][0], w1 = SW[k][p][1], w2 = SW[k][p][2];
+            const float v0 = SV[k][p][0], v1 = SV[k][p][1], v2 = SV[k][p][2];
+            const float swl = lz ? w1 : w0, swh = lz ? w2 : w1, svl = lz ? v1 : v0, svh = lz ? v2 : v1;
+            const float sw = fmaf(tz, swh, (1.0f - tz) * swl), sv = fmaf(tz, svh, (1.0f - tz) * svl);
+            const float zv = unk[p] ? (sw > 0.0f ? sv / sw : cv[k][13]) : 0.0f;
+            word |= (uint32_t)bf16Bits(zv) << (16 * p);
+          }
+          *reinterpret_cast<uint32_t *>(ps.z16[k] + c) = word;
+        }
       }
     }
   }
