@@ -2035,6 +2035,9 @@ __global__ void k_copy_blocks(const int *__restrict__ blocks, const int *__restr
 }
 
 constexpr int kVecBlocks = 1024, kVecThreads = 256;
+/* workgroups of grid-stride passes without a reduction (d update): twice as
+ * many waves in flight as kVecBlocks gives */
+constexpr int kVecBlocksNoSum = 2048;
 
 /* partial <a, b> over [0, n) (FP64 products) */
 __global__ void __launch_bounds__(kVecThreads) k_dot(const float *__restrict__ a, const float *__restrict__ b, int64_t n,
@@ -2581,7 +2584,7 @@ int deviceUpload(Grid &g, const std::vector<float> &rhs, const msbgx_scene *scen
   freeFlip(d);
   for (int L = 0; L < kMaxLevels; L++) freeLevel(g.mg[L]);
   /* partials: one per block of a launch, per vector workgroup, and two sets per PCG block (k_update_xr_rz_mean) */
-  int maxBlocksPerLaunch = std::max({g.nBlocks, 2 * kVecBlocks, 2 * g.nActive0});
+  int maxBlocksPerLaunch = std::max({g.nBlocks, 2 * kVecBlocks, 4096, 2 * g.nActive0});
   if (d.partialsCap < maxBlocksPerLaunch) {
     if (d.partials) (void)hipFree(d.partials);
     d.partials = nullptr;
@@ -2870,7 +2873,7 @@ static void pcgIteration(Grid &g, const msbgx_solve_params &p)
   hipLaunchKernelGGL(k_dot, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_R], m0.d_ch[C_Z], n0, d.partials,
                      done);
   finalize(d, kVecBlocks, FIN_BETA, nullptr);
-  hipLaunchKernelGGL(k_axpy_d, dim3(kVecBlocks), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], n0,
+  hipLaunchKernelGGL(k_axpy_d, dim3(kVecBlocksNoSum), dim3(kVecThreads), 0, d.stream, m0.d_ch[C_D], m0.d_ch[C_Z], n0,
                      d.state);
 }
 
